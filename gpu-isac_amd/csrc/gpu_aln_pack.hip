@@ -1,0 +1,850 @@
+// C-ABI host layer of gpu_aln_pack.so for MI355X: process-global context,
+// device memory, and the reference's entry points (see include/gpu_aln_pack.h).
+//
+// Context lifetime follows cuda/gpu_aln_noref.cu:31-48, :86-180: *_init sets the
+// device (fixed for the process), builds the polar template and the shift grid
+// and allocates the library-owned AlignParam records; gpu_clear frees
+// everything but keeps the device id.
+#include "../../include/gpu_aln_pack.h"
+#include "isac_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+using isac::DevParam;
+using isac::PeakRec;
+using isac::ShiftView;
+using isac::kRingLen;
+using isac::kSlots;
+
+static_assert(sizeof(AlignParam) == 24, "AlignParam must keep the reference layout");
+static_assert(sizeof(AlignConfig) == 40, "AlignConfig must keep the reference layout");
+static_assert(offsetof(AlignParam, shift_x) == 8 && offsetof(AlignParam, mirror) == 20, "layout");
+
+namespace {
+
+// ------------------------------------------------------------------ errors
+int g_err = 0;
+char g_errmsg[512] = {0};
+
+void set_error(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    fprintf(stderr, "gpu_aln_pack: error %d: %s\n", code, buf);
+    fflush(stderr);
+    if (g_err == 0) {
+        g_err = code;
+        snprintf(g_errmsg, sizeof(g_errmsg), "%s", buf);
+    }
+    const char *ex = getenv("ISAC_EXIT_ON_ERROR");
+    if (ex && ex[0] == '1') exit(EXIT_FAILURE);
+}
+
+enum : int {
+    kErrHip = 1,
+    kErrArg = 2,
+    kErrState = 3,
+    kErrUnsupported = 4,
+};
+
+#define HIP_OK(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            set_error(kErrHip, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),     \
+                      __FILE__, __LINE__);                                                \
+            return kErrHip;                                                               \
+        }                                                                                 \
+    } while (0)
+
+// ------------------------------------------------------------ device buffers
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t reserve(size_t bytes) {
+        if (bytes <= cap && p) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 256));
+        if (e == hipSuccess) cap = std::max<size_t>(bytes, 256);
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+// Work buffers shared by every entry point (grow-only until gpu_clear).
+struct Workspace {
+    int dev = -1;
+    hipStream_t stream = nullptr;
+    int tmpl_rings = 0;
+    DevBuf tmpl, twiddle, zero_shift, shifts_i, shifts_f;
+    DevBuf spec, norm, rspec, rnorm, rsum0, rec;
+    DevBuf peak, dvals, acc;
+    std::vector<float2> shifts_uploaded;
+    void release() {
+        DevBuf *all[] = {&tmpl, &twiddle, &zero_shift, &shifts_i, &shifts_f, &spec, &norm,
+                         &rspec, &rnorm, &rsum0, &rec, &peak, &dvals, &acc};
+        for (DevBuf *b : all) b->release();
+        tmpl_rings = 0;
+        shifts_uploaded.clear();
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+};
+
+// Process-global alignment context (cuda/gpu_aln_noref.cu:29-48).
+struct Context {
+    int device = -1;                       // fixed once set (cuda/gpu_aln_noref.cu:103)
+    const AlignConfig *cfg = nullptr;      // retained by pointer
+    std::vector<float2> shifts;            // shift grid (x-major)
+    AlignParam *h_param = nullptr;         // pinned host, returned to the caller
+    size_t n_param = 0;
+    enum Mode { kNone, kMref, kPre, kRefFree } mode = kNone;
+    // pre-align / reference-free device data
+    DevBuf sbj_img, ref_img, prm, tr, cid_idx, ref_of, twd, tanl_work;
+    unsigned sbj_cap = 0, ref_cap = 0, sbj_count = 0, ref_count = 0;
+    int twd_dim = 0;
+    // multi_ref_align host staging
+    float *h_stage = nullptr;
+    size_t h_stage_cap = 0;
+    // profiling
+    bool prof = false;
+    int prof_ccf_n = 0, prof_polar_n = 0;
+    double prof_ccf_ms = 0.0, prof_polar_ms = 0.0;
+};
+
+Context g;
+Workspace ws;
+
+// -------------------------------------------------------------- grid helpers
+// cuda/gpu_aln_common.cu:64-86: float loop, x-major.
+std::vector<float2> make_shift_grid(float rx, float ry, float step) {
+    std::vector<float2> v;
+    if (!(step > 0.0f)) return v;
+    for (float sx = -rx; sx <= rx; sx += step)
+        for (float sy = -ry; sy <= ry; sy += step) v.push_back(make_float2(sx, sy));
+    return v;
+}
+
+// cuda/gpu_aln_common.cu:53-59 (same float operations).
+std::vector<float2> make_template(int R) {
+    std::vector<float2> t((size_t)R * kRingLen);
+    for (int i = 0; i < R; ++i)
+        for (int j = 0; j < kRingLen; ++j) {
+            const float angle = (float)((double)((float)j / (float)kRingLen) * 3.14159265358979 * 2);
+            t[(size_t)i * kRingLen + j] =
+                make_float2(cosf(angle) * (float)(i + 1) * (float)R / (float)R,
+                            sinf(angle) * (float)(i + 1) * (float)R / (float)R);
+        }
+    return t;
+}
+
+int ensure_device_ws(int dev) {
+    int cur = -1;
+    HIP_OK(hipGetDevice(&cur));
+    if (ws.dev != cur) {
+        ws.release();
+        ws.dev = cur;
+    }
+    if (!ws.stream) HIP_OK(hipStreamCreateWithFlags(&ws.stream, hipStreamNonBlocking));
+    (void)dev;
+    if (!ws.twiddle.p) {
+        std::vector<float2> tw(256);
+        for (int m = 0; m < 256; ++m) {
+            const double a = -2.0 * M_PI * (double)m / 256.0;
+            tw[m] = make_float2((float)cos(a), (float)sin(a));
+        }
+        HIP_OK(ws.twiddle.reserve(256 * sizeof(float2)));
+        HIP_OK(hipMemcpy(ws.twiddle.p, tw.data(), 256 * sizeof(float2), hipMemcpyHostToDevice));
+        HIP_OK(ws.zero_shift.reserve(sizeof(int2)));
+        HIP_OK(hipMemset(ws.zero_shift.p, 0, sizeof(int2)));
+    }
+    return 0;
+}
+
+int ensure_template(int R) {
+    if (ws.tmpl_rings == R && ws.tmpl.p) return 0;
+    std::vector<float2> t = make_template(R);
+    HIP_OK(ws.tmpl.reserve(t.size() * sizeof(float2)));
+    HIP_OK(hipMemcpy(ws.tmpl.p, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice));
+    ws.tmpl_rings = R;
+    return 0;
+}
+
+int ensure_shifts(const std::vector<float2> &sh) {
+    if (sh.size() == ws.shifts_uploaded.size() &&
+        std::equal(sh.begin(), sh.end(), ws.shifts_uploaded.begin(),
+                   [](float2 a, float2 b) { return a.x == b.x && a.y == b.y; }))
+        return 0;
+    std::vector<int2> si(sh.size());
+    for (size_t i = 0; i < sh.size(); ++i)  // the kernel argument is int (quirk Q1)
+        si[i] = make_int2((int)sh[i].x, (int)sh[i].y);
+    HIP_OK(ws.shifts_i.reserve(si.size() * sizeof(int2)));
+    HIP_OK(ws.shifts_f.reserve(sh.size() * sizeof(float2)));
+    HIP_OK(hipMemcpy(ws.shifts_i.p, si.data(), si.size() * sizeof(int2), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(ws.shifts_f.p, sh.data(), sh.size() * sizeof(float2), hipMemcpyHostToDevice));
+    ws.shifts_uploaded = sh;
+    return 0;
+}
+
+int check_shape(int D, int R, int L, size_t S) {
+    if (L != kRingLen) {
+        set_error(kErrUnsupported, "ring_len %d unsupported (this build computes 256-sample rings)", L);
+        return kErrUnsupported;
+    }
+    if (D < 2 || D > isac::kMaxImgDim) {
+        set_error(kErrUnsupported, "img_dim %d outside [2, %d]", D, isac::kMaxImgDim);
+        return kErrUnsupported;
+    }
+    if (R < 1 || R > isac::kMaxRings) {
+        set_error(kErrArg, "ring_num %d outside [1, %d]", R, isac::kMaxRings);
+        return kErrArg;
+    }
+    if (S < 1 || S > 65536) {
+        set_error(kErrArg, "shift grid of %zu shifts (need 1..65536)", S);
+        return kErrArg;
+    }
+    return 0;
+}
+
+size_t free_device_bytes() {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 0;
+    return fr;
+}
+
+// Particle-batch size for the spectra workspace.
+int batch_for(size_t per_item, size_t fixed_bytes, float mem_frac, int total) {
+    double frac = (mem_frac > 0.0f && mem_frac <= 1.0f) ? mem_frac : 0.9;
+    size_t have = (size_t)(frac * (double)(free_device_bytes() + ws.spec.cap + ws.rec.cap + ws.norm.cap));
+    const size_t cap = (size_t)16 << 30;  // beyond 16 GiB a larger batch buys nothing
+    size_t budget = std::min(have > fixed_bytes ? have - fixed_bytes : 0, cap);
+    size_t b = per_item ? budget / per_item : (size_t)total;
+    b = std::max<size_t>(b, 1);
+    return (int)std::min<size_t>(b, (size_t)total);
+}
+
+struct ProfScope {
+    hipEvent_t a = nullptr, b = nullptr;
+    bool on;
+    hipStream_t st;
+    explicit ProfScope(bool enabled, hipStream_t s) : on(enabled), st(s) {
+        if (on) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, st);
+        }
+    }
+    double finish() {
+        if (!on) return 0.0;
+        (void)hipEventRecord(b, st);
+        (void)hipEventSynchronize(b);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return ms;
+    }
+};
+
+// Reference spectra (centre D//2, no shift), normalised and ring-weighted.
+int ref_spectra(const float *d_ref, int N, int D, int R, hipStream_t st) {
+    HIP_OK(ws.rspec.reserve((size_t)N * R * kSlots * sizeof(float2)));
+    HIP_OK(ws.rnorm.reserve((size_t)N * sizeof(float2)));
+    HIP_OK(ws.rsum0.reserve((size_t)N * sizeof(float)));
+    HIP_OK(isac::launch_polar_spectra(d_ref, N, D, ShiftView{nullptr, 0}, ws.zero_shift.as<int2>(),
+                                      1, ws.tmpl.as<float2>(), R, ws.twiddle.as<float2>(),
+                                      ws.rspec.as<float2>(), ws.rnorm.as<float2>(), st));
+    HIP_OK(isac::launch_ref_finalize(ws.rspec.as<float2>(), ws.rnorm.as<float2>(), N, R,
+                                     ws.rsum0.as<float>(), st));
+    return 0;
+}
+
+// Device-resident multi-reference pipeline.
+int run_mref(const float *d_sbj, const float *d_ref, ShiftView acc, int P, int N, int D, int R,
+             const std::vector<float2> &shifts, float *d_peak, int peak_row, float *d_d,
+             int d_pitch, float mem_frac, hipStream_t st) {
+    const int S = (int)shifts.size();
+    if (int e = check_shape(D, R, kRingLen, shifts.size())) return e;
+    if (P <= 0 || N <= 0) return 0;
+    if (int e = ensure_template(R)) return e;
+    if (int e = ensure_shifts(shifts)) return e;
+    if (int e = ref_spectra(d_ref, N, D, R, st)) return e;
+    const size_t per = (size_t)S * R * kSlots * sizeof(float2) + (size_t)S * sizeof(float2) +
+                       (size_t)N * sizeof(PeakRec);
+    const int pb = batch_for(per, 0, mem_frac, P);
+    HIP_OK(ws.spec.reserve((size_t)pb * S * R * kSlots * sizeof(float2)));
+    HIP_OK(ws.norm.reserve((size_t)pb * S * sizeof(float2)));
+    HIP_OK(ws.rec.reserve((size_t)pb * N * sizeof(PeakRec)));
+    for (int p0 = 0; p0 < P; p0 += pb) {
+        const int n = std::min(pb, P - p0);
+        ShiftView a = acc;
+        if (a.base) a.base += (size_t)p0 * a.stride;
+        {
+            ProfScope ps(g.prof, st);
+            HIP_OK(isac::launch_polar_spectra(d_sbj + (size_t)p0 * D * D, n, D, a,
+                                              ws.shifts_i.as<int2>(), S, ws.tmpl.as<float2>(), R,
+                                              ws.twiddle.as<float2>(), ws.spec.as<float2>(),
+                                              ws.norm.as<float2>(), st));
+            if (g.prof) { g.prof_polar_ms += ps.finish(); g.prof_polar_n++; }
+        }
+        {
+            ProfScope ps(g.prof, st);
+            HIP_OK(isac::launch_ccf_mref(ws.spec.as<float2>(), ws.norm.as<float2>(),
+                                         ws.rspec.as<float2>(), ws.rsum0.as<float>(), n, S, R, N,
+                                         ws.twiddle.as<float2>(), ws.rec.as<PeakRec>(), st));
+            if (g.prof) { g.prof_ccf_ms += ps.finish(); g.prof_ccf_n++; }
+        }
+        HIP_OK(isac::launch_decode_mref(ws.rec.as<PeakRec>(), n, N, a, ws.shifts_f.as<float2>(),
+                                        d_peak, peak_row, p0, d_d, d_pitch, st));
+    }
+    HIP_OK(hipStreamSynchronize(st));
+    return 0;
+}
+
+// Device-resident single-reference pipeline (pre-align / reference-free):
+// particles d_sbj[0..cnt), accumulated shifts and outputs in prm[0..cnt),
+// reference index of particle i at ref_of[i * ref_stride].
+int run_single(const float *d_sbj, DevParam *prm, const int *ref_of, int ref_stride, int cnt,
+               const float *d_ref, int N, int D, int R, const std::vector<float2> &shifts,
+               float shift_limit, hipStream_t st) {
+    const int S = (int)shifts.size();
+    if (int e = check_shape(D, R, kRingLen, shifts.size())) return e;
+    if (cnt <= 0) return 0;
+    if (N <= 0) {
+        set_error(kErrState, "no reference has been fetched");
+        return kErrState;
+    }
+    if (int e = ensure_template(R)) return e;
+    if (int e = ensure_shifts(shifts)) return e;
+    if (int e = ref_spectra(d_ref, N, D, R, st)) return e;
+    const size_t per = (size_t)S * R * kSlots * sizeof(float2) + (size_t)S * sizeof(float2) +
+                       sizeof(PeakRec);
+    const int pb = batch_for(per, 0, 0.9f, cnt);
+    HIP_OK(ws.spec.reserve((size_t)pb * S * R * kSlots * sizeof(float2)));
+    HIP_OK(ws.norm.reserve((size_t)pb * S * sizeof(float2)));
+    HIP_OK(ws.rec.reserve((size_t)pb * sizeof(PeakRec)));
+    for (int p0 = 0; p0 < cnt; p0 += pb) {
+        const int n = std::min(pb, cnt - p0);
+        ShiftView a{&prm[p0].shift_x, (int)(sizeof(DevParam) / sizeof(float))};
+        HIP_OK(isac::launch_polar_spectra(d_sbj + (size_t)p0 * D * D, n, D, a,
+                                          ws.shifts_i.as<int2>(), S, ws.tmpl.as<float2>(), R,
+                                          ws.twiddle.as<float2>(), ws.spec.as<float2>(),
+                                          ws.norm.as<float2>(), st));
+        HIP_OK(isac::launch_ccf_single(ws.spec.as<float2>(), ws.norm.as<float2>(),
+                                       ws.rspec.as<float2>(), ws.rsum0.as<float>(),
+                                       ref_of + (size_t)p0 * ref_stride, ref_stride, n, S, R, N,
+                                       ws.twiddle.as<float2>(), ws.rec.as<PeakRec>(), st));
+        HIP_OK(isac::launch_decode_single(ws.rec.as<PeakRec>(), n, prm + p0,
+                                          ws.shifts_f.as<float2>(), shift_limit, st));
+    }
+    HIP_OK(hipStreamSynchronize(st));
+    return 0;
+}
+
+// cuda/gpu_aln_noref.cu:86-115
+int alignment_init(const AlignConfig *cfg, unsigned dev) {
+    if (!cfg) {
+        set_error(kErrArg, "NULL AlignConfig");
+        return kErrArg;
+    }
+    if (g.device != -1 && g.device != (int)dev) {
+        set_error(kErrState, "device %u requested but this process already uses device %d", dev,
+                  g.device);
+        return kErrState;
+    }
+    if (g.mode != Context::kNone) {
+        set_error(kErrState, "a previous *_init was not followed by gpu_clear()");
+        return kErrState;
+    }
+    HIP_OK(hipSetDevice((int)dev));
+    g.device = (int)dev;
+    g.cfg = cfg;
+    g.shifts = make_shift_grid(cfg->shift_rng_x, cfg->shift_rng_y, cfg->shift_step);
+    if (g.shifts.empty()) {
+        set_error(kErrArg, "empty shift grid (step %g)", (double)cfg->shift_step);
+        return kErrArg;
+    }
+    if (std::fmod(cfg->shift_rng_x, cfg->shift_step) != 0.0f ||
+        std::fmod(cfg->shift_rng_y, cfg->shift_step) != 0.0f) {
+        // cuda/gpu_aln_noref.cu:2380-2381 asserts this
+        set_error(kErrArg, "shift range (%g, %g) is not a multiple of the step %g",
+                  (double)cfg->shift_rng_x, (double)cfg->shift_rng_y, (double)cfg->shift_step);
+        return kErrArg;
+    }
+    return ensure_device_ws((int)dev);
+}
+
+int alloc_params(size_t n, int ref_id) {
+    if (g.h_param) (void)hipHostFree(g.h_param);
+    g.h_param = nullptr;
+    g.n_param = 0;
+    if (n == 0) return 0;
+    HIP_OK(hipHostMalloc((void **)&g.h_param, n * sizeof(AlignParam), hipHostMallocDefault));
+    for (size_t i = 0; i < n; ++i) {
+        g.h_param[i].sbj_id = -1;
+        g.h_param[i].ref_id = ref_id;
+        g.h_param[i].shift_x = 0.0f;
+        g.h_param[i].shift_y = 0.0f;
+        g.h_param[i].angle = 0.0f;
+        g.h_param[i].mirror = false;
+    }
+    g.n_param = n;
+    return 0;
+}
+
+int upload_images(DevBuf &dst, const float **imgs, unsigned n, int D, hipStream_t st) {
+    if (n == 0) return 0;
+    if (!imgs) {
+        set_error(kErrArg, "NULL image pointer array");
+        return kErrArg;
+    }
+    const size_t img_bytes = (size_t)D * D * sizeof(float);
+    const size_t need = (size_t)n * img_bytes;
+    if (g.h_stage_cap < need) {
+        if (g.h_stage) (void)hipHostFree(g.h_stage);
+        g.h_stage = nullptr;
+        g.h_stage_cap = 0;
+        HIP_OK(hipHostMalloc((void **)&g.h_stage, need, hipHostMallocDefault));
+        g.h_stage_cap = need;
+    }
+    for (unsigned i = 0; i < n; ++i) {
+        if (!imgs[i]) {
+            set_error(kErrArg, "NULL image pointer at index %u", i);
+            return kErrArg;
+        }
+        memcpy(reinterpret_cast<char *>(g.h_stage) + (size_t)i * img_bytes, imgs[i], img_bytes);
+    }
+    HIP_OK(dst.reserve(need));
+    HIP_OK(hipMemcpyAsync(dst.p, g.h_stage, need, hipMemcpyHostToDevice, st));
+    HIP_OK(hipStreamSynchronize(st));
+    return 0;
+}
+
+int upload_params(size_t first, size_t cnt, hipStream_t st) {
+    HIP_OK(g.prm.reserve(std::max<size_t>(cnt, 1) * sizeof(DevParam)));
+    if (cnt == 0) return 0;
+    HIP_OK(hipMemcpyAsync(g.prm.p, g.h_param + first, cnt * sizeof(AlignParam),
+                          hipMemcpyHostToDevice, st));
+    return 0;
+}
+
+int download_params(size_t first, size_t cnt, hipStream_t st) {
+    if (cnt == 0) return 0;
+    HIP_OK(hipMemcpyAsync(g.h_param + first, g.prm.p, cnt * sizeof(AlignParam),
+                          hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    return 0;
+}
+
+bool require_mode(Context::Mode m, const char *fn) {
+    if (g.mode != m || !g.cfg) {
+        set_error(kErrState, "%s called without the matching *_init", fn);
+        return false;
+    }
+    return true;
+}
+
+size_t spec_bytes_per_particle(const AlignConfig *c) {
+    const size_t S = make_shift_grid(c->shift_rng_x, c->shift_rng_y, c->shift_step).size();
+    return S * c->ring_num * kSlots * sizeof(float2) + S * sizeof(float2) + sizeof(PeakRec);
+}
+
+}  // namespace
+
+// ======================================================================= ABI
+extern "C" {
+
+int gpu_last_error(void) { return g_err; }
+const char *gpu_last_error_string(void) { return g_errmsg; }
+void gpu_clear_error(void) {
+    g_err = 0;
+    g_errmsg[0] = 0;
+}
+
+void print_gpu_info(unsigned int device_idx) {
+    int drv = 0, rt = 0;
+    (void)hipDriverGetVersion(&drv);
+    (void)hipRuntimeGetVersion(&rt);
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, (int)device_idx) != hipSuccess) {
+        set_error(kErrHip, "hipGetDeviceProperties(%u) failed", device_idx);
+        return;
+    }
+    printf("HIP driver  version: %d\n", drv);
+    printf("HIP runtime version: %d\n", rt);
+    printf("HIP device[%u] info for %s (%s):\n", device_idx, pr.name, pr.gcnArchName);
+    printf("No. of compute units: %d\n", pr.multiProcessorCount);
+    printf("____[ Memory ]____\n");
+    printf("Total global memory: %.3f GB\n", (double)pr.totalGlobalMem / (1024.0 * 1024.0 * 1024.0));
+    printf("Shared memory (LDS) per block: %zu bytes\n", (size_t)pr.sharedMemPerBlock);
+    printf("L2 cache: %d bytes\n", pr.l2CacheSize);
+    printf("____[ Grid ]____\n");
+    printf("Maximum grid dim:   (%d, %d, %d)\n", pr.maxGridSize[0], pr.maxGridSize[1], pr.maxGridSize[2]);
+    printf("Max. no. of threads per block: %d\n", pr.maxThreadsPerBlock);
+    printf("Wavefront size: %d\n", pr.warpSize);
+    fflush(stdout);
+}
+
+int get_texture_limit(unsigned int img_dim) {
+    if (img_dim == 0) return 0;
+    // references whose spectra ([rings <= img_dim/2][128] float2) keep 32-bit offsets
+    const long long per = (long long)std::max(1u, img_dim / 2) * kSlots;
+    return (int)std::min<long long>((long long)std::numeric_limits<int>::max() / per, 1 << 20);
+}
+
+void gpu_clear(void) {
+    if (g.h_param) (void)hipHostFree(g.h_param);
+    g.h_param = nullptr;
+    g.n_param = 0;
+    if (g.h_stage) (void)hipHostFree(g.h_stage);
+    g.h_stage = nullptr;
+    g.h_stage_cap = 0;
+    DevBuf *bufs[] = {&g.sbj_img, &g.ref_img, &g.prm, &g.tr, &g.cid_idx, &g.ref_of, &g.twd, &g.tanl_work};
+    for (DevBuf *b : bufs) b->release();
+    g.sbj_cap = g.ref_cap = g.sbj_count = g.ref_count = 0;
+    g.twd_dim = 0;
+    g.shifts.clear();
+    g.cfg = nullptr;
+    g.mode = Context::kNone;
+    ws.release();
+    // the device id stays fixed (cuda/gpu_aln_noref.cu:139-180 leaves CUDA_DEVICE_ID set)
+}
+
+void reset_shifts(float shift_range, float shift_step) {
+    if (g.shifts.empty()) {
+        set_error(kErrState, "reset_shifts called before *_init");
+        return;
+    }
+    std::vector<float2> ns = make_shift_grid(shift_range, shift_range, shift_step);
+    if (ns.size() != g.shifts.size()) {
+        set_error(kErrArg, "reset_shifts(%g, %g) gives %zu shifts, the context holds %zu",
+                  (double)shift_range, (double)shift_step, ns.size(), g.shifts.size());
+        return;
+    }
+    g.shifts = ns;
+}
+
+// ----------------------------------------------------------- multi-reference
+AlignParam *multi_ref_align_init(const AlignConfig *aln_cfg, unsigned int cuda_device_id) {
+    if (alignment_init(aln_cfg, cuda_device_id)) return nullptr;
+    if (alloc_params(aln_cfg->sbj_num, -1)) return nullptr;
+    g.mode = Context::kMref;
+    return g.h_param;
+}
+
+void multi_ref_align(const float **sbj_data, const float **ref_data, float *isac_peak_list,
+                     float *isac_d, unsigned int isac_d_pitch, float cuda_mem_request) {
+    if (!require_mode(Context::kMref, "multi_ref_align")) return;
+    const AlignConfig *c = g.cfg;
+    const int P = (int)c->sbj_num, N = (int)c->ref_num, D = (int)c->img_dim, R = (int)c->ring_num;
+    if (check_shape(D, R, (int)c->ring_len, g.shifts.size())) return;
+    if (P == 0 || N == 0) return;
+    if (!isac_peak_list || !isac_d || isac_d_pitch < (unsigned)P) {
+        set_error(kErrArg, "bad output buffers (d_pitch %u < sbj_num %d?)", isac_d_pitch, P);
+        return;
+    }
+    hipStream_t st = ws.stream;
+    if (upload_images(g.sbj_img, sbj_data, (unsigned)P, D, st)) return;
+    if (upload_images(g.ref_img, ref_data, (unsigned)N, D, st)) return;
+    if (upload_params(0, (size_t)P, st)) return;
+    if (ws.peak.reserve((size_t)N * 4 * P * sizeof(float)) != hipSuccess ||
+        ws.dvals.reserve((size_t)N * P * sizeof(float)) != hipSuccess) {
+        set_error(kErrHip, "output buffer allocation failed");
+        return;
+    }
+    ShiftView acc{&g.prm.as<DevParam>()[0].shift_x, (int)(sizeof(DevParam) / sizeof(float))};
+    if (run_mref(g.sbj_img.as<float>(), g.ref_img.as<float>(), acc, P, N, D, R, g.shifts,
+                 ws.peak.as<float>(), 4 * P, ws.dvals.as<float>(), P, cuda_mem_request, st))
+        return;
+    if (hipMemcpy(isac_peak_list, ws.peak.p, (size_t)N * 4 * P * sizeof(float),
+                  hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy2D(isac_d, (size_t)isac_d_pitch * sizeof(float), ws.dvals.p,
+                    (size_t)P * sizeof(float), (size_t)P * sizeof(float), (size_t)N,
+                    hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error(kErrHip, "copying results to the host failed");
+        return;
+    }
+}
+
+int isac_mref_align_device(const float *d_sbj, const float *d_ref, const float *d_acc_xy,
+                           unsigned int n_sbj, unsigned int n_ref, unsigned int img_dim,
+                           unsigned int ring_num, unsigned int ring_len, float shift_step,
+                           float shift_rng_x, float shift_rng_y, float *d_peak_list, float *d_d,
+                           unsigned int d_pitch, void *stream) {
+    if (int e = ensure_device_ws(-1)) return e;
+    std::vector<float2> sh = make_shift_grid(shift_rng_x, shift_rng_y, shift_step);
+    if (int e = check_shape((int)img_dim, (int)ring_num, (int)ring_len, sh.size())) return e;
+    if (n_sbj == 0 || n_ref == 0) return 0;
+    if (!d_sbj || !d_ref || !d_peak_list || !d_d || d_pitch < n_sbj) {
+        set_error(kErrArg, "isac_mref_align_device: bad pointers or d_pitch < n_sbj");
+        return kErrArg;
+    }
+    hipStream_t st = stream ? (hipStream_t)stream : ws.stream;
+    ShiftView acc{d_acc_xy, 2};
+    return run_mref(d_sbj, d_ref, acc, (int)n_sbj, (int)n_ref, (int)img_dim, (int)ring_num, sh,
+                    d_peak_list, 4 * (int)n_sbj, d_d, (int)d_pitch, 0.9f, st);
+}
+
+void isac_profile_enable(int on) {
+    g.prof = on != 0;
+    g.prof_ccf_n = g.prof_polar_n = 0;
+    g.prof_ccf_ms = g.prof_polar_ms = 0.0;
+}
+
+void isac_profile_read(int *launches, double *total_ms, int *polar_launches, double *polar_total_ms) {
+    if (launches) *launches = g.prof_ccf_n;
+    if (total_ms) *total_ms = g.prof_ccf_ms;
+    if (polar_launches) *polar_launches = g.prof_polar_n;
+    if (polar_total_ms) *polar_total_ms = g.prof_polar_ms;
+}
+
+// -------------------------------------------------------------- pre-alignment
+AlignParam *pre_align_init(unsigned int num_particles, const AlignConfig *aln_cfg,
+                           unsigned int cuda_device_id) {
+    if (alignment_init(aln_cfg, cuda_device_id)) return nullptr;
+    if (alloc_params(num_particles, 0)) return nullptr;
+    g.sbj_cap = aln_cfg->sbj_num;
+    g.ref_cap = aln_cfg->ref_num;
+    const size_t ib = (size_t)aln_cfg->img_dim * aln_cfg->img_dim * sizeof(float);
+    if (g.sbj_img.reserve(std::max<size_t>(1, g.sbj_cap) * ib) != hipSuccess ||
+        g.ref_img.reserve(std::max<size_t>(1, g.ref_cap) * ib) != hipSuccess) {
+        set_error(kErrHip, "pre_align_init: image buffer allocation failed");
+        return nullptr;
+    }
+    g.mode = Context::kPre;
+    return g.h_param;
+}
+
+int pre_align_size_check(unsigned int num_particles, const AlignConfig *cfg,
+                         unsigned int cuda_device_id, float request, bool verbose) {
+    if (!cfg) return 0;
+    if (hipSetDevice((int)cuda_device_id) != hipSuccess) return 0;
+    const double mb = 1024.0 * 1024.0;
+    const size_t ib = (size_t)cfg->img_dim * cfg->img_dim * sizeof(float);
+    const size_t params = (size_t)num_particles * sizeof(AlignParam) * 2;
+    const size_t sbj = (size_t)cfg->sbj_num * ib, ref = (size_t)cfg->ref_num * ib;
+    // the spectra workspace is batched internally; require room for 64 particles
+    const size_t work = 64 * spec_bytes_per_particle(cfg) +
+                        (size_t)std::max(1u, cfg->ref_num) * cfg->ring_num * kSlots * sizeof(float2);
+    const size_t total = params + sbj + ref + work;
+    const size_t avail = free_device_bytes();
+    if (verbose) {
+        printf("GPU[%u] SIZE CHECK\n  in/out param: %.1f MB\n  sbj images: %.1f MB\n  ref images: %.1f MB\n"
+               "  min. workspace: %.1f MB\nTOTAL: %.1f MB of %.1f MB x %.2f\n",
+               cuda_device_id, params / mb, sbj / mb, ref / mb, work / mb, total / mb, avail / mb,
+               (double)request);
+    }
+    return ((double)avail * (double)request >= (double)total) ? 1 : 0;
+}
+
+void pre_align_fetch(const float **img_data, unsigned int img_num, const char *batch_type) {
+    if (!require_mode(Context::kPre, "pre_align_fetch")) return;
+    const int D = (int)g.cfg->img_dim;
+    if (!batch_type) {
+        set_error(kErrArg, "pre_align_fetch: NULL batch type");
+        return;
+    }
+    if (strcmp(batch_type, "sbj_batch") == 0) {
+        if (img_num > g.sbj_cap) {
+            set_error(kErrArg, "pre_align_fetch: %u images exceed the batch capacity %u", img_num, g.sbj_cap);
+            return;
+        }
+        if (upload_images(g.sbj_img, img_data, img_num, D, ws.stream)) return;
+        g.sbj_count = img_num;
+    } else if (strcmp(batch_type, "ref_batch") == 0) {
+        if (img_num > std::max(1u, g.ref_cap)) {
+            set_error(kErrArg, "pre_align_fetch: %u references exceed the capacity %u", img_num, g.ref_cap);
+            return;
+        }
+        if (upload_images(g.ref_img, img_data, img_num, D, ws.stream)) return;
+        g.ref_count = img_num;
+    } else {
+        set_error(kErrArg, "fetch_data() :: Unknown batch type '%s' specified.", batch_type);
+    }
+}
+
+void pre_align_run(int start_idx, int stop_idx) {
+    if (!require_mode(Context::kPre, "pre_align_run")) return;
+    const AlignConfig *c = g.cfg;
+    if (start_idx < 0 || stop_idx < start_idx || (size_t)start_idx > g.n_param) {
+        set_error(kErrArg, "pre_align_run(%d, %d): bad range for %zu particles", start_idx, stop_idx, g.n_param);
+        return;
+    }
+    // rows decoded: the fetched batch, not beyond stop_idx (cuda/gpu_aln_noref.cu:2493-2496)
+    size_t cnt = std::min<size_t>(g.sbj_count, (size_t)(stop_idx - start_idx));
+    cnt = std::min(cnt, g.n_param - (size_t)start_idx);
+    if (cnt == 0) return;
+    hipStream_t st = ws.stream;
+    // accumulated shifts of particles start..start+cnt; the reference index comes
+    // from records 0..cnt-1 (ccf_mult is called with data index 0, :762)
+    if (g.ref_of.reserve(cnt * sizeof(int)) != hipSuccess) {
+        set_error(kErrHip, "pre_align_run: allocation failed");
+        return;
+    }
+    std::vector<int> refs(cnt);
+    for (size_t i = 0; i < cnt; ++i) {
+        refs[i] = g.h_param[i].ref_id;
+        if (refs[i] < 0 || refs[i] >= (int)std::max(1u, g.ref_count)) {
+            set_error(kErrArg, "pre_align_run: particle %zu has ref_id %d", i, refs[i]);
+            return;
+        }
+    }
+    if (hipMemcpy(g.ref_of.p, refs.data(), cnt * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+        set_error(kErrHip, "pre_align_run: copy failed");
+        return;
+    }
+    if (upload_params((size_t)start_idx, cnt, st)) return;
+    const float lim = (float)(c->img_dim - c->ring_num - 2);
+    if (run_single(g.sbj_img.as<float>(), g.prm.as<DevParam>(), g.ref_of.as<int>(), 1, (int)cnt,
+                   g.ref_img.as<float>(), (int)g.ref_count, (int)c->img_dim, (int)c->ring_num,
+                   g.shifts, lim, st))
+        return;
+    (void)download_params((size_t)start_idx, cnt, st);
+}
+
+// ------------------------------------------------------ reference-free (2D)
+AlignParam *ref_free_alignment_2D_init(const AlignConfig *aln_cfg, const float **sbj_data_list,
+                                       const float **ref_data_list, const int *sbj_cid_list,
+                                       unsigned int cuda_device_id) {
+    if (alignment_init(aln_cfg, cuda_device_id)) return nullptr;
+    const unsigned P = aln_cfg->sbj_num, N = aln_cfg->ref_num;
+    if (!sbj_cid_list && P) {
+        set_error(kErrArg, "NULL class id list");
+        return nullptr;
+    }
+    // class index runs (cuda/gpu_aln_noref.cu:825-835)
+    std::vector<int> cid_idx(N + 1, (int)P);
+    int cid = -1;
+    unsigned idx = 0;
+    for (unsigned i = 0; i < P; ++i) {
+        const int ci = sbj_cid_list[i];
+        if (ci < 0 || ci >= (int)N) {
+            set_error(kErrArg, "particle %u has class id %d outside [0, %u)", i, ci, N);
+            return nullptr;
+        }
+        if (ci != cid) {
+            if (idx >= N) {
+                set_error(kErrArg, "class ids form more than %u contiguous runs", N);
+                return nullptr;
+            }
+            cid_idx[idx++] = (int)i;
+            cid = ci;
+        }
+    }
+    if (alloc_params(P, 0)) return nullptr;
+    for (unsigned i = 0; i < P; ++i) g.h_param[i].ref_id = sbj_cid_list[i];
+    hipStream_t st = ws.stream;
+    if (g.cid_idx.reserve((N + 1) * sizeof(int)) != hipSuccess ||
+        hipMemcpy(g.cid_idx.p, cid_idx.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+        set_error(kErrHip, "ref_free_alignment_2D_init: allocation failed");
+        return nullptr;
+    }
+    if (upload_images(g.sbj_img, sbj_data_list, P, (int)aln_cfg->img_dim, st)) return nullptr;
+    if (upload_images(g.ref_img, ref_data_list, N, (int)aln_cfg->img_dim, st)) return nullptr;
+    g.sbj_count = g.sbj_cap = P;
+    g.ref_count = g.ref_cap = N;
+    g.mode = Context::kRefFree;
+    return g.h_param;
+}
+
+int ref_free_alignment_2D_size_check(const AlignConfig *cfg, unsigned int cuda_device_id,
+                                     float request, bool verbose) {
+    if (!cfg) return 0;
+    if (hipSetDevice((int)cuda_device_id) != hipSuccess) return 0;
+    const double mb = 1024.0 * 1024.0;
+    const size_t ib = (size_t)cfg->img_dim * cfg->img_dim * sizeof(float);
+    const size_t params = (size_t)cfg->sbj_num * sizeof(AlignParam) * 2 + (cfg->ref_num + 1) * sizeof(int);
+    const size_t imgs = (size_t)cfg->sbj_num * ib * 2 + (size_t)cfg->ref_num * ib;  // + transformed copy
+    const size_t work = 64 * spec_bytes_per_particle(cfg) +
+                        (size_t)cfg->ref_num * cfg->ring_num * kSlots * sizeof(float2) +
+                        (size_t)cfg->ref_num * cfg->img_dim * (cfg->img_dim / 2 + 1) * sizeof(float2);
+    const size_t total = params + imgs + work;
+    const size_t avail = free_device_bytes();
+    if (verbose) {
+        printf("GPU[%u] SIZE CHECK\n  in/out param: %.1f MB\n  images: %.1f MB\n  min. workspace: %.1f MB\n"
+               "TOTAL: %.1f MB of %.1f MB x %.2f\n",
+               cuda_device_id, params / mb, imgs / mb, work / mb, total / mb, avail / mb, (double)request);
+    }
+    return ((double)avail * (double)request >= (double)total) ? 1 : 0;
+}
+
+void ref_free_alignment_2D(void) {
+    if (!require_mode(Context::kRefFree, "ref_free_alignment_2D")) return;
+    const AlignConfig *c = g.cfg;
+    const int P = (int)g.sbj_count, N = (int)g.ref_count, D = (int)c->img_dim, R = (int)c->ring_num;
+    if (check_shape(D, R, (int)c->ring_len, g.shifts.size())) return;
+    hipStream_t st = ws.stream;
+    for (int i = 0; i < P; ++i)
+        if (g.h_param[i].ref_id < 0 || g.h_param[i].ref_id >= N) {
+            set_error(kErrArg, "ref_free_alignment_2D: particle %d has ref_id %d outside [0, %d)", i,
+                      g.h_param[i].ref_id, N);
+            return;
+        }
+    if (upload_params(0, (size_t)P, st)) return;
+    DevParam *prm = g.prm.as<DevParam>();
+    const float lim = (float)(c->img_dim - c->ring_num - 2);
+    if (run_single(g.sbj_img.as<float>(), prm, &prm[0].ref_id, (int)(sizeof(DevParam) / sizeof(int)),
+                   P, g.ref_img.as<float>(), N, D, R, g.shifts, lim, st))
+        return;
+    if (g.tr.reserve((size_t)std::max(P, 1) * D * D * sizeof(float)) != hipSuccess) {
+        set_error(kErrHip, "ref_free_alignment_2D: allocation failed");
+        return;
+    }
+    hipError_t e = isac::launch_transform(g.sbj_img.as<float>(), P, D, prm, g.tr.as<float>(), st);
+    if (e == hipSuccess)
+        e = isac::launch_class_average(g.tr.as<float>(), D, g.cid_idx.as<int>(), N,
+                                       g.ref_img.as<float>(), st);
+    if (e != hipSuccess) {
+        set_error(kErrHip, "ref_free_alignment_2D: kernel launch failed: %s", hipGetErrorString(e));
+        return;
+    }
+    (void)download_params(0, (size_t)P, st);
+}
+
+void ref_free_alignment_2D_filter_references(float cutoff_freq, float falloff) {
+    if (!require_mode(Context::kRefFree, "ref_free_alignment_2D_filter_references")) return;
+    const int D = (int)g.cfg->img_dim, N = (int)g.ref_count;
+    if (N <= 0) return;
+    hipStream_t st = ws.stream;
+    if (g.twd_dim != D) {
+        std::vector<float2> tw(D);
+        for (int m = 0; m < D; ++m) {
+            const double a = -2.0 * M_PI * (double)m / (double)D;
+            tw[m] = make_float2((float)cos(a), (float)sin(a));
+        }
+        if (g.twd.reserve(D * sizeof(float2)) != hipSuccess ||
+            hipMemcpy(g.twd.p, tw.data(), D * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess) {
+            set_error(kErrHip, "filter: twiddle upload failed");
+            return;
+        }
+        g.twd_dim = D;
+    }
+    if (g.tanl_work.reserve((size_t)N * D * (D / 2 + 1) * sizeof(float2)) != hipSuccess) {
+        set_error(kErrHip, "filter: allocation failed");
+        return;
+    }
+    hipError_t e = isac::launch_tanl_filter(g.ref_img.as<float>(), N, D, cutoff_freq, falloff,
+                                            g.tanl_work.as<float2>(), g.twd.as<float2>(), st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) set_error(kErrHip, "filter failed: %s", hipGetErrorString(e));
+}
+
+}  // extern "C"

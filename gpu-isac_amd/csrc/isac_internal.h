@@ -1,0 +1,93 @@
+// Private interface between the HIP kernels (kernels.hip) and the C-ABI
+// host layer (gpu_aln_pack.hip).  Not installed; not part of the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace isac {
+
+constexpr int kRingLen = 256;   // samples per ring (fixed by every caller)
+constexpr int kSlots = 128;     // packed spectrum slots per ring: slot 0 = (X[0], X[128])
+constexpr int kMaxImgDim = 128; // image staged whole in LDS by the polar kernel
+constexpr int kMaxRings = 127;  // rings must fit inside a 256-sample ring FFT budget
+
+// Same memory layout as the reference's AlignParam (cuda/gpu_aln_common.h:78-85).
+struct DevParam {
+    int sbj_id;
+    int ref_id;
+    float shift_x;
+    float shift_y;
+    float angle;
+    bool mirror;
+};
+static_assert(sizeof(DevParam) == 24, "AlignParam layout");
+
+// Best psi-CCF peak of one (particle, reference) pair over (mirror, shift, psi):
+// raw maximum, its mirror / shift index / psi sample, and the seven samples
+// around the peak in the winning (mirror, shift) slot for the parabola.
+struct PeakRec {
+    float val;
+    int m;
+    int s;
+    int t;
+    float x[7];
+    int pad;
+};
+static_assert(sizeof(PeakRec) == 48, "PeakRec layout");
+
+// Strided view of per-image accumulated shifts: x at base[i*stride], y at +1.
+// float2 arrays use stride 2; DevParam arrays use base = &p[0].shift_x, stride 6.
+struct ShiftView {
+    const float *base;
+    int stride;
+};
+
+// Stage A: normalised-later ring spectra of every image at every grid shift.
+// spec[img][s][ring][kSlots] float2 (raw, unnormalised), norm[img][s] = (mu*L, 1/(sd+1e-7)).
+hipError_t launch_polar_spectra(const float *imgs, int n_img, int img_dim, ShiftView acc,
+                                const int2 *shift_i, int n_shift, const float2 *tmpl,
+                                int n_ring, const float2 *twiddle, float2 *spec, float2 *norm,
+                                hipStream_t st);
+
+// Normalise and ring-weight reference spectra in place (n_shift == 1 layout);
+// rsum0[n] = sum_i w_i R'_i[0] (needed for the particle DC correction).
+hipError_t launch_ref_finalize(float2 *rspec, const float2 *rnorm, int n_ref, int n_ring,
+                               float *rsum0, hipStream_t st);
+
+// Stage B, multi-reference: every particle x every reference; rec[p*n_ref + n].
+hipError_t launch_ccf_mref(const float2 *spec, const float2 *norm, const float2 *rspec,
+                           const float *rsum0, int n_img, int n_shift, int n_ring, int n_ref,
+                           const float2 *twiddle, PeakRec *rec, hipStream_t st);
+
+// Stage B, one reference per particle (pre-alignment / reference-free):
+// ref index of image i at ref_of[i*ref_stride]; rec[i].
+hipError_t launch_ccf_single(const float2 *spec, const float2 *norm, const float2 *rspec,
+                             const float *rsum0, const int *ref_of, int ref_stride, int n_img,
+                             int n_shift, int n_ring, int n_ref, const float2 *twiddle,
+                             PeakRec *rec, hipStream_t st);
+
+// Multi-reference decode into the caller's d / peak_list layout.
+hipError_t launch_decode_mref(const PeakRec *rec, int n_img, int n_ref, ShiftView acc,
+                              const float2 *shifts_f, float *peak, int peak_row, int p_off,
+                              float *d, int d_pitch, hipStream_t st);
+
+// Pre-align / reference-free decode: update AlignParam records in place.
+hipError_t launch_decode_single(const PeakRec *rec, int cnt, DevParam *prm,
+                                const float2 *shifts_f, float shift_limit, hipStream_t st);
+
+// Apply found parameters to images: out[i] = T_i(imgs[i]).
+hipError_t launch_transform(const float *imgs, int n, int img_dim, const DevParam *prm,
+                            float *out, hipStream_t st);
+
+// Per-class mean of transformed images into refs.
+hipError_t launch_class_average(const float *tr, int img_dim, const int *cid_idx, int n_cls,
+                                float *refs, hipStream_t st);
+
+// Tangent low-pass filter of refs in place; work holds n*img_dim*(img_dim/2+1) float2,
+// tw_d holds img_dim twiddles e^{-2 pi i m / img_dim}.
+hipError_t launch_tanl_filter(float *refs, int n, int img_dim, float cutoff, float falloff,
+                              float2 *work, const float2 *tw_d, hipStream_t st);
+
+}  // namespace isac

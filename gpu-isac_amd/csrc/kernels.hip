@@ -1,0 +1,702 @@
+// HIP kernels of the polar-ring 2D alignment path for CDNA4 (gfx950, wave64).
+//
+// Data layout in HBM:
+//   images       [n][D][D] float32, row-major (y, x)
+//   spec         [n][S][R][128] float2: raw ring spectra of every image at every
+//                grid shift, packed: slot 0 = (X[0], X[128]), slot k = X[k], k=1..127
+//   norm         [n][S] float2 = (mu*L, 1/(sd+1e-7)) of the radius-weighted
+//                normalisation (reference cuda/gpu_aln_noref.cu:1294-1351)
+//   rspec        [N][R][128] float2: reference spectra, normalised and weighted (i+1)
+//   rec          PeakRec per (particle, reference)
+// The reference's psi-CCF table (particles x refs x shifts x 2 x (L+2) floats,
+// cuda/gpu_aln_noref.cu:2343-2461) is never materialised: the contraction over
+// rings, the psi inverse FFT of both mirrors and the first-max-wins search run
+// fused in registers / LDS of one wavefront.
+#include "isac_internal.h"
+
+#include <cfloat>
+#include <climits>
+
+namespace isac {
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ float2 f2(float x, float y) { return make_float2(x, y); }
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return f2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ float2 shfl2(float2 v, int src) {
+    return f2(__shfl(v.x, src, kWave), __shfl(v.y, src, kWave));
+}
+
+// Radix-4 butterfly: forward Y_m = sum_r v_r e^{-2 pi i rm/4}; INV uses e^{+}.
+template <bool INV>
+__device__ __forceinline__ void radix4(float2 (&v)[4]) {
+    const float2 a0 = f2(v[0].x + v[2].x, v[0].y + v[2].y);
+    const float2 a1 = f2(v[0].x - v[2].x, v[0].y - v[2].y);
+    const float2 a2 = f2(v[1].x + v[3].x, v[1].y + v[3].y);
+    const float2 a3 = f2(v[1].x - v[3].x, v[1].y - v[3].y);
+    v[0] = f2(a0.x + a2.x, a0.y + a2.y);
+    v[2] = f2(a0.x - a2.x, a0.y - a2.y);
+    if (!INV) {  // a1 -/+ i a3
+        v[1] = f2(a1.x + a3.y, a1.y - a3.x);
+        v[3] = f2(a1.x - a3.y, a1.y + a3.x);
+    } else {
+        v[1] = f2(a1.x - a3.y, a1.y + a3.x);
+        v[3] = f2(a1.x + a3.y, a1.y - a3.x);
+    }
+}
+
+// One Stockham radix-4 stage (Govindaraju et al., SC'08) with sub-transform size NS.
+template <bool INV, int NS, bool WRITE>
+__device__ __forceinline__ void stockham_stage(float2 (&v)[4], const float2 *in, float2 *out,
+                                               const float2 *tw, int lane) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = in[lane + 64 * q];
+    const int k = lane & (NS - 1);
+#pragma unroll
+    for (int q = 1; q < 4; ++q) {
+        float2 w = tw[q * k * (64 / NS)];
+        if (INV) w.y = -w.y;
+        v[q] = cmul(v[q], w);
+    }
+    radix4<INV>(v);
+    if (WRITE) {
+        const int base = (lane - k) * 4 + k;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) out[base + q * NS] = v[q];
+    }
+}
+
+// 256-point complex FFT by one wavefront.  In: v[q] = x[lane + 64 q].
+// Out: v[q] = X[lane + 64 q] (natural order).  b0/b1: 256-float2 wave-private
+// LDS buffers; tw: e^{-2 pi i m/256}, m < 256, in LDS.  Unnormalised.
+template <bool INV>
+__device__ __forceinline__ void fft256_wave(float2 (&v)[4], float2 *b0, float2 *b1,
+                                            const float2 *tw, int lane) {
+    wave_lds_sync();
+    radix4<INV>(v);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) b0[4 * lane + q] = v[q];
+    wave_lds_sync();
+    stockham_stage<INV, 4, true>(v, b0, b1, tw, lane);
+    wave_lds_sync();
+    stockham_stage<INV, 16, true>(v, b1, b0, tw, lane);
+    wave_lds_sync();
+    stockham_stage<INV, 64, false>(v, b0, nullptr, tw, lane);
+}
+
+__device__ __forceinline__ float bilinear(const float *img, int D, float x, float y) {
+    const float x0 = floorf(x), y0 = floorf(y);
+    const float fx = x - x0, fy = y - y0;
+    const int ix = (int)x0, iy = (int)y0;
+    const int xa = min(max(ix, 0), D - 1), xb = min(max(ix + 1, 0), D - 1);
+    const int ya = min(max(iy, 0), D - 1), yb = min(max(iy + 1, 0), D - 1);
+    const float v00 = img[ya * D + xa], v01 = img[ya * D + xb];
+    const float v10 = img[yb * D + xa], v11 = img[yb * D + xb];
+    return (1.0f - fy) * ((1.0f - fx) * v00 + fx * v01) + fy * ((1.0f - fx) * v10 + fx * v11);
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs; remap so that consecutive
+// logical blocks (which share particle spectra) run on the same XCD and L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7;
+    const int xcd = b & 7, idx = b >> 3;
+    return (xcd < r) ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+}
+
+__device__ __forceinline__ void wave_argmax(float &v, int &t) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const float ov = __shfl_xor(v, off, kWave);
+        const int ot = __shfl_xor(t, off, kWave);
+        if (ov > v || (ov == v && ot < t)) {
+            v = ov;
+            t = ot;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ stage A
+// Block = 4 wavefronts = 4 grid shifts of one image; the image is staged in LDS.
+// A wavefront samples two rings at a time (ring a real part, ring b imaginary
+// part), runs one 256-point FFT and unpacks both real spectra.  The normalisation
+// sums (radius-weighted sum and sum of squares) accumulate in double.
+__global__ __launch_bounds__(256) void k_polar_spectra(
+    const float *__restrict__ imgs, int D, ShiftView acc, const int2 *__restrict__ shift_i,
+    int S, const float2 *__restrict__ tmpl, int R, const float2 *__restrict__ twg,
+    float2 *__restrict__ spec, float2 *__restrict__ norm) {
+    extern __shared__ float4 smem4[];
+    float2 *tw = reinterpret_cast<float2 *>(smem4);
+    float2 *fb = tw + 256;
+    float *img = reinterpret_cast<float *>(fb + 4 * 512);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int p = blockIdx.x;
+    const float *src = imgs + (size_t)p * D * D;
+    for (int i = tid; i < D * D; i += 256) img[i] = src[i];
+    tw[tid] = twg[tid];
+    __syncthreads();
+    const int s = blockIdx.y * 4 + wave;
+    if (s >= S) return;
+    float ax = 0.0f, ay = 0.0f;
+    if (acc.base) {
+        ax = acc.base[(size_t)p * acc.stride];
+        ay = acc.base[(size_t)p * acc.stride + 1];
+    }
+    const int2 si = shift_i[s];
+    const float cx = (float)(D / 2 + si.x) + ax;
+    const float cy = (float)(D / 2 + si.y) + ay;
+    float2 *b0 = fb + wave * 512;
+    float2 *b1 = b0 + 256;
+    double sA = 0.0, sQ = 0.0;
+    float2 *out = spec + ((size_t)p * S + s) * (size_t)R * kSlots;
+    const int pl = (64 - lane) & 63;
+    for (int i = 0; i < R; i += 2) {
+        const bool hb = (i + 1) < R;
+        const double wa = (double)(i + 1), wb = (double)(i + 2);
+        float2 v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = lane + 64 * q;
+            const float2 oa = tmpl[i * kRingLen + j];
+            const float va = bilinear(img, D, cx + oa.x, cy + oa.y);
+            float vb = 0.0f;
+            if (hb) {
+                const float2 ob = tmpl[(i + 1) * kRingLen + j];
+                vb = bilinear(img, D, cx + ob.x, cy + ob.y);
+            }
+            v[q] = f2(va, vb);
+            sA += wa * (double)va + wb * (double)vb;
+            sQ += wa * (double)va * (double)va + wb * (double)vb * (double)vb;
+        }
+        fft256_wave<false>(v, b0, b1, tw, lane);
+        // v[q] = Z[lane + 64 q].  Partners Z[256-k]: k=lane -> lane pl, q=3;
+        // k=lane+64 -> lane pl, q=2 (lane 0: Z[0] itself and Z[192] = own q=3).
+        const float2 u2 = shfl2(v[2], pl), u3 = shfl2(v[3], pl);
+        const float2 zl = v[0], zlp = lane ? u3 : v[0];
+        const float2 zh = v[1], zhp = lane ? u2 : v[3];
+        float2 al = f2(0.5f * (zl.x + zlp.x), 0.5f * (zl.y - zlp.y));
+        float2 bl = f2(0.5f * (zl.y + zlp.y), 0.5f * (zlp.x - zl.x));
+        const float2 ah = f2(0.5f * (zh.x + zhp.x), 0.5f * (zh.y - zhp.y));
+        const float2 bh = f2(0.5f * (zh.y + zhp.y), 0.5f * (zhp.x - zh.x));
+        if (lane == 0) {  // pack Nyquist: A[128] = Re Z[128], B[128] = Im Z[128]
+            al = f2(zl.x, v[2].x);
+            bl = f2(zl.y, v[2].y);
+        }
+        float2 *oa = out + (size_t)i * kSlots;
+        oa[lane] = al;
+        oa[lane + 64] = ah;
+        if (hb) {
+            oa[kSlots + lane] = bl;
+            oa[kSlots + lane + 64] = bh;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        sA += __shfl_xor(sA, off, kWave);
+        sQ += __shfl_xor(sQ, off, kWave);
+    }
+    if (lane == 0) {
+        const double nn = (double)kRingLen * 0.5 * (double)R * (double)(R + 1);
+        const double mu = sA / nn;
+        const double sd = sqrt(fabs(sQ / nn - mu * mu));
+        norm[(size_t)p * S + s] = f2((float)(mu * kRingLen), (float)(1.0 / (sd + 1e-7)));
+    }
+}
+
+__global__ __launch_bounds__(128) void k_ref_finalize(float2 *__restrict__ rspec,
+                                                      const float2 *__restrict__ rnorm, int R,
+                                                      float *__restrict__ rsum0) {
+    const int n = blockIdx.x, k = threadIdx.x;
+    const float2 nm = rnorm[n];
+    double acc0 = 0.0;
+    for (int i = 0; i < R; ++i) {
+        float2 *px = rspec + ((size_t)n * R + i) * kSlots + k;
+        float2 x = *px;
+        const float gw = nm.y * (float)(i + 1);
+        if (k == 0) {
+            x.x = (x.x - nm.x) * gw;
+            x.y = x.y * gw;
+            acc0 += (double)x.x;
+        } else {
+            x.x *= gw;
+            x.y *= gw;
+        }
+        *px = x;
+    }
+    if (k == 0) rsum0[n] = (float)acc0;
+}
+
+// ------------------------------------------------------------------ stage B
+// One wavefront per work item.  Multi-reference: item = (particle, group of
+// NREF references); single-reference: item = particle.  The wavefront walks the
+// grid shifts four at a time: lane l accumulates, over rings, the four real
+// products P1..P4 of spectrum slots l and l+64 for 4 shifts x NREF references
+// (C0 = R conj S = (P1+P2, P4-P3), C1 = conj(R S) = (P1-P2, -(P3+P4))).  Each
+// (shift, reference) pair then becomes one 256-point inverse FFT of
+// Z = C0 + i C1 (Hermitian-extended), so z[t] = c0[t] + i c1[t] gives the psi
+// CCF of both mirrors at once.  The running first-max over (mirror, shift, psi)
+// is kept per reference and mirror, with the seven samples around it.
+template <int NREF, bool SINGLE>
+__global__ __launch_bounds__(256) void k_ccf_psi(
+    const float2 *__restrict__ spec, const float2 *__restrict__ norm,
+    const float2 *__restrict__ rspec, const float *__restrict__ rsum0,
+    const int *__restrict__ ref_of, int ref_stride, int n_img, int S, int R, int N, int G,
+    const float2 *__restrict__ twg, PeakRec *__restrict__ rec) {
+    __shared__ float2 tw[256];
+    __shared__ float2 fbuf[4][512];
+    __shared__ float zbuf[4][256];
+    __shared__ float nbuf[4][NREF][2][8];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    tw[tid] = twg[tid];
+    __syncthreads();
+    const int wid = xcd_remap(blockIdx.x, gridDim.x) * 4 + wave;
+    const int p = SINGLE ? wid : wid / G;
+    const int g = SINGLE ? 0 : wid % G;
+    if (p >= n_img) return;
+
+    int nref[NREF];
+    bool nval[NREF];
+#pragma unroll
+    for (int c = 0; c < NREF; ++c) {
+        if (SINGLE) {  // the host validates ref ids; clamp keeps every read in bounds
+            nref[c] = min(max(ref_of[(size_t)p * ref_stride], 0), N - 1);
+            nval[c] = true;
+        } else {
+            const int n = g * NREF + c;
+            nval[c] = n < N;
+            nref[c] = nval[c] ? n : N - 1;
+        }
+    }
+    float bv[NREF][2];
+    int bf[NREF][2];
+    float rs0[NREF];
+#pragma unroll
+    for (int c = 0; c < NREF; ++c) {
+        rs0[c] = rsum0[nref[c]];
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            bv[c][m] = SINGLE ? 0.0f : -3.40282e+38f;
+            bf[c][m] = SINGLE ? 0 : INT_MAX;
+        }
+    }
+    float2 *b0 = fbuf[wave];
+    float2 *b1 = b0 + 256;
+    float *zb = zbuf[wave];
+    const int pl = (64 - lane) & 63;
+    const size_t img_stride = (size_t)S * R * kSlots;
+
+    for (int s0 = 0; s0 < S; s0 += 4) {
+        float acc[4][NREF][2][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < NREF; ++c)
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) acc[r][c][h][e] = 0.0f;
+        const float2 *arow[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            arow[r] = spec + (size_t)p * img_stride + (size_t)min(s0 + r, S - 1) * R * kSlots;
+        for (int i = 0; i < R; ++i) {
+            float2 a[4][2], b[NREF][2];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                a[r][0] = arow[r][i * kSlots + lane];
+                a[r][1] = arow[r][i * kSlots + lane + 64];
+            }
+#pragma unroll
+            for (int c = 0; c < NREF; ++c) {
+                const float2 *rb = rspec + ((size_t)nref[c] * R + i) * kSlots;
+                b[c][0] = rb[lane];
+                b[c][1] = rb[lane + 64];
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < NREF; ++c)
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        acc[r][c][h][0] += b[c][h].x * a[r][h].x;
+                        acc[r][c][h][1] += b[c][h].y * a[r][h].y;
+                        acc[r][c][h][2] += b[c][h].x * a[r][h].y;
+                        acc[r][c][h][3] += b[c][h].y * a[r][h].x;
+                    }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int s = s0 + r;
+            if (s >= S) break;
+            const float2 nm = norm[(size_t)p * S + s];
+#pragma unroll
+            for (int c = 0; c < NREF; ++c) {
+                if (!nval[c]) continue;
+                const float gs = nm.y;
+                // slot lane+64 (always a plain frequency)
+                const float *P = acc[r][c][1];
+                float A = P[0] + P[1], B = P[3] - P[2], Cc = P[0] - P[1], Dd = -(P[2] + P[3]);
+                const float2 zhi = f2(gs * (A - Dd), gs * (B + Cc));
+                const float2 yhi = f2(gs * (A + Dd), gs * (Cc - B));
+                const float *Q = acc[r][c][0];
+                A = Q[0] + Q[1];
+                B = Q[3] - Q[2];
+                Cc = Q[0] - Q[1];
+                Dd = -(Q[2] + Q[3]);
+                float2 zlo = f2(gs * (A - Dd), gs * (B + Cc));
+                const float2 ylo = f2(gs * (A + Dd), gs * (Cc - B));
+                float2 nyq = f2(0.0f, 0.0f);
+                if (lane == 0) {  // packed slot 0: DC (with the mean correction) and Nyquist
+                    const float c0 = gs * (Q[0] - nm.x * rs0[c]);
+                    const float cn = gs * Q[1];
+                    zlo = f2(c0, c0);
+                    nyq = f2(cn, cn);
+                }
+                const float2 syhi = shfl2(yhi, pl), sylo = shfl2(ylo, pl);
+                float2 v[4];
+                v[0] = zlo;
+                v[1] = zhi;
+                v[2] = lane ? syhi : nyq;
+                v[3] = lane ? sylo : yhi;
+                fft256_wave<true>(v, b0, b1, tw, lane);
+#pragma unroll
+                for (int m = 0; m < 2; ++m) {
+                    float bvl = m ? v[0].y : v[0].x;
+                    int bt = lane;
+#pragma unroll
+                    for (int q = 1; q < 4; ++q) {
+                        const float x = m ? v[q].y : v[q].x;
+                        if (x > bvl) {
+                            bvl = x;
+                            bt = lane + 64 * q;
+                        }
+                    }
+                    wave_argmax(bvl, bt);
+                    const int flat = s * kRingLen + bt;
+                    const bool upd = bvl > bv[c][m] || (bvl == bv[c][m] && flat < bf[c][m]);
+                    const bool sentinel = SINGLE && m == 0 && s == 0 && !upd &&
+                                          bf[c][0] == 0 && bv[c][0] == 0.0f;
+                    if (upd || sentinel) {
+                        const int tc = upd ? bt : 0;
+                        if (upd) {
+                            bv[c][m] = bvl;
+                            bf[c][m] = flat;
+                        }
+                        wave_lds_sync();
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) zb[lane + 64 * q] = m ? v[q].y : v[q].x;
+                        wave_lds_sync();
+                        if (lane < 7) nbuf[wave][c][m][lane] = zb[(tc + lane - 3) & 255];
+                        wave_lds_sync();
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NREF; ++c) {
+        if (!nval[c]) continue;
+        const int m = (bv[c][1] > bv[c][0]) ? 1 : 0;
+        PeakRec *o = SINGLE ? rec + p : rec + (size_t)p * N + nref[c];
+        const int f = bf[c][m];
+        if (lane == 0) {
+            o->val = bv[c][m];
+            o->m = m;
+            o->s = f / kRingLen;
+            o->t = f % kRingLen;
+            o->pad = 0;
+        }
+        if (lane < 7) o->x[lane] = nbuf[wave][c][m][lane];
+    }
+}
+
+// ------------------------------------------------------------------ decode
+// Multi-reference decode (reference wizard, cuda/gpu_aln_noref.cu:1758-1793, with
+// its float/double mix) and output scatter (:517-540).
+__global__ void k_decode_mref(const PeakRec *__restrict__ rec, int n_img, int N, ShiftView acc,
+                              const float2 *__restrict__ shifts_f, float *__restrict__ peak,
+                              int peak_row, int p_off, float *__restrict__ d, int d_pitch) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n_img * N) return;
+    const int p = idx / N, n = idx % N;
+    const PeakRec r = rec[idx];
+    const float *x = r.x;
+    const float c2 = (float)(49. * x[0] + 6. * x[1] - 21. * x[2] - 32. * x[3] - 27. * x[4] -
+                             6. * x[5] + 31. * x[6]);
+    const float c3 = (float)(5. * x[0] - 3. * x[2] - 4. * x[3] - 3. * x[4] + 5. * x[6]);
+    const float step = (float)(360.0 / (double)kRingLen);
+    float angle = step * (float)r.t;
+    if (c3 != 0.0f) angle = (float)((double)angle + (double)step * ((double)c2 / (2.0 * (double)c3) - 4.0));
+    angle = (float)(360.0 - (double)angle);
+    if (r.m == 1) angle = (float)((double)angle + 180.0);
+    if (angle >= 360.0f) angle = (float)((double)angle - 360.0);
+    float ax = 0.0f, ay = 0.0f;
+    if (acc.base) {
+        ax = acc.base[(size_t)p * acc.stride];
+        ay = acc.base[(size_t)p * acc.stride + 1];
+    }
+    const float2 sh = shifts_f[r.s];
+    const size_t pp = (size_t)p_off + p;
+    d[(size_t)n * d_pitch + pp] = r.val;
+    float *pk = peak + (size_t)n * peak_row + 4 * pp;
+    pk[0] = angle;
+    pk[1] = ax + sh.x;
+    pk[2] = ay + sh.y;
+    pk[3] = r.m ? 1.0f : 0.0f;
+}
+
+// Pre-align / reference-free decode (cuda/gpu_aln_noref.cu:2486-2519, FP64
+// parabola :2558-2581).
+__global__ void k_decode_single(const PeakRec *__restrict__ rec, int cnt, DevParam *__restrict__ prm,
+                                const float2 *__restrict__ shifts_f, float lim) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt) return;
+    const PeakRec r = rec[i];
+    DevParam q = prm[i];
+    double x[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) x[k] = (double)r.x[k];
+    const double c2 = 49. * x[0] + 6. * x[1] - 21. * x[2] - 32. * x[3] - 27. * x[4] - 6. * x[5] + 31. * x[6];
+    const double c3 = 5. * x[0] - 3. * x[2] - 4. * x[3] - 3. * x[4] + 5. * x[6];
+    const double step = 360.0 / (double)kRingLen;
+    double interp = step * (double)r.t;
+    if (c3 != 0.0) interp = interp + step * (c2 / (2.0 * c3) - 4.0);
+    q.mirror = (r.m == 1);
+    const float2 sh = shifts_f[r.s];
+    q.shift_x += sh.x;
+    q.shift_y += sh.y;
+    q.shift_x = fminf(fmaxf(q.shift_x, -lim), lim);
+    q.shift_y = fminf(fmaxf(q.shift_y, -lim), lim);
+    q.angle = (float)(360.0 - interp);
+    if (q.mirror) {
+        q.angle = (float)((double)q.angle + 180.0);
+        if (q.angle >= 360.0f) q.angle = (float)((double)q.angle - 360.0);
+    }
+    prm[i] = q;
+}
+
+// ---------------------------------------------------- reference-free support
+// cu_transform_batch (cuda/gpu_aln_noref.cu:1558-1588): output pixel -> mirror
+// (x -> D - x) -> rotate by +angle about D/2 -> + accumulated shift -> bilinear.
+__global__ __launch_bounds__(256) void k_transform(const float *__restrict__ imgs, int D,
+                                                   const DevParam *__restrict__ prm,
+                                                   float *__restrict__ out) {
+    const int p = blockIdx.x;
+    const DevParam q = prm[p];
+    const float c = (float)(D / 2);
+    const float ang = (3.141592653589793f / 180.0f) * q.angle;
+    const float sa = sinf(ang), ca = cosf(ang);
+    const float *img = imgs + (size_t)p * D * D;
+    for (int idx = threadIdx.x; idx < D * D; idx += blockDim.x) {
+        const int x = idx % D, y = idx / D;
+        float sx = q.mirror ? (float)(D - x) : (float)x;
+        float sy = (float)y;
+        sx -= c;
+        sy -= c;
+        const float tx = sx;
+        sx = tx * ca - sy * sa;
+        sy = tx * sa + sy * ca;
+        sx += c;
+        sy += c;
+        out[(size_t)p * D * D + idx] = bilinear(img, D, sx + q.shift_x, sy + q.shift_y);
+    }
+}
+
+// cu_average_batch (cuda/gpu_aln_noref.cu:1605-1621).
+__global__ __launch_bounds__(256) void k_class_average(const float *__restrict__ tr, int D,
+                                                       const int *__restrict__ cid_idx,
+                                                       float *__restrict__ refs) {
+    const int k = blockIdx.x;
+    const int a = cid_idx[k], b = cid_idx[k + 1];
+    if (b <= a) return;  // a class without particles keeps its reference
+    for (int pix = threadIdx.x; pix < D * D; pix += blockDim.x) {
+        float s = 0.0f;
+        for (int i = a; i < b; ++i) s += tr[(size_t)i * D * D + pix];
+        refs[(size_t)k * D * D + pix] = s / (float)(b - a);
+    }
+}
+
+// Tangent filter, three separable passes over a half-plane spectrum
+// F[n][ky][kx], kx = 0..D/2 (cuda/gpu_aln_noref.cu:1007-1037, :2203-2266).
+__global__ void k_row_dft(const float *__restrict__ refs, int D, float2 *__restrict__ F,
+                          const float2 *__restrict__ tw) {
+    const int n = blockIdx.y, y = blockIdx.x, Dh = D / 2 + 1;
+    const float *row = refs + ((size_t)n * D + y) * D;
+    for (int kx = threadIdx.x; kx < Dh; kx += blockDim.x) {
+        float re = 0.0f, im = 0.0f;
+        for (int x = 0; x < D; ++x) {
+            const float2 w = tw[(x * kx) % D];
+            re += row[x] * w.x;
+            im += row[x] * w.y;
+        }
+        F[((size_t)n * D + y) * Dh + kx] = f2(re, im);
+    }
+}
+
+__global__ void k_col_filter(float2 *__restrict__ F, int D, const float2 *__restrict__ tw,
+                             float cutoff, float falloff) {
+    extern __shared__ float2 col[];
+    const int n = blockIdx.y, kx = blockIdx.x, Dh = D / 2 + 1;
+    float2 *base = F + (size_t)n * D * Dh + kx;
+    const float cc = 3.141592653589793f / (2.0f * falloff * cutoff);
+    const float xx = (float)kx / (float)Dh * 0.5f;
+    const float half = (float)D / 2.0f;
+    for (int ky = threadIdx.x; ky < D; ky += blockDim.x) {
+        float re = 0.0f, im = 0.0f;
+        for (int y = 0; y < D; ++y) {
+            const float2 v = base[(size_t)y * Dh];
+            const float2 w = tw[(y * ky) % D];
+            re += v.x * w.x - v.y * w.y;
+            im += v.x * w.y + v.y * w.x;
+        }
+        float yy = ((float)ky < half) ? (float)ky : (float)(D - ky);
+        yy = yy / half * 0.5f;
+        const float dd = sqrtf(xx * xx + yy * yy);
+        const float h = 0.5f * (tanhf(cc * (dd + cutoff)) - tanhf(cc * (dd - cutoff)));
+        col[ky] = f2(re * h, im * h);
+    }
+    __syncthreads();
+    for (int y = threadIdx.x; y < D; y += blockDim.x) {
+        float re = 0.0f, im = 0.0f;
+        for (int ky = 0; ky < D; ++ky) {
+            const float2 v = col[ky];
+            const float2 w = tw[(y * ky) % D];  // conj for the inverse
+            re += v.x * w.x + v.y * w.y;
+            im += v.y * w.x - v.x * w.y;
+        }
+        base[(size_t)y * Dh] = f2(re, im);
+    }
+}
+
+__global__ void k_row_c2r(float *__restrict__ refs, int D, const float2 *__restrict__ F,
+                          const float2 *__restrict__ tw) {
+    const int n = blockIdx.y, y = blockIdx.x, Dh = D / 2 + 1;
+    const float2 *row = F + ((size_t)n * D + y) * Dh;
+    const float dd = (float)D * (float)D;
+    const float gain = (n == 0) ? 1.0f / (dd * dd) : 1.0f;
+    for (int x = threadIdx.x; x < D; x += blockDim.x) {
+        float s = 0.0f;
+        for (int kx = 0; kx < Dh; ++kx) {
+            const float2 v = row[kx];
+            const float2 w = tw[(x * kx) % D];  // e^{+i}: (w.x, -w.y)
+            const float re = v.x * w.x + v.y * w.y;
+            const bool edge = (kx == 0) || ((D % 2 == 0) && kx == D / 2);
+            s += edge ? re : 2.0f * re;
+        }
+        refs[((size_t)n * D + y) * D + x] = s * gain;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+
+hipError_t launch_polar_spectra(const float *imgs, int n_img, int img_dim, ShiftView acc,
+                                const int2 *shift_i, int n_shift, const float2 *tmpl, int n_ring,
+                                const float2 *twiddle, float2 *spec, float2 *norm,
+                                hipStream_t st) {
+    if (n_img <= 0 || n_shift <= 0) return hipSuccess;
+    const size_t lds = (256 + 4 * 512) * sizeof(float2) + (size_t)img_dim * img_dim * sizeof(float);
+    static bool lds_raised = false;
+    if (!lds_raised) {
+        const hipError_t e = hipFuncSetAttribute((const void *)k_polar_spectra,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)((256 + 4 * 512) * sizeof(float2) +
+                                                       kMaxImgDim * kMaxImgDim * sizeof(float)));
+        if (e != hipSuccess) return e;
+        lds_raised = true;
+    }
+    dim3 grid(n_img, (n_shift + 3) / 4);
+    hipLaunchKernelGGL(k_polar_spectra, grid, dim3(256), lds, st, imgs, img_dim, acc, shift_i,
+                       n_shift, tmpl, n_ring, twiddle, spec, norm);
+    return hipGetLastError();
+}
+
+hipError_t launch_ref_finalize(float2 *rspec, const float2 *rnorm, int n_ref, int n_ring,
+                               float *rsum0, hipStream_t st) {
+    if (n_ref <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ref_finalize, dim3(n_ref), dim3(kSlots), 0, st, rspec, rnorm, n_ring, rsum0);
+    return hipGetLastError();
+}
+
+hipError_t launch_ccf_mref(const float2 *spec, const float2 *norm, const float2 *rspec,
+                           const float *rsum0, int n_img, int n_shift, int n_ring, int n_ref,
+                           const float2 *twiddle, PeakRec *rec, hipStream_t st) {
+    if (n_img <= 0 || n_ref <= 0) return hipSuccess;
+    const int G = (n_ref + 3) / 4;
+    const long long items = (long long)n_img * G;
+    const int blocks = (int)((items + 3) / 4);
+    hipLaunchKernelGGL((k_ccf_psi<4, false>), dim3(blocks), dim3(256), 0, st, spec, norm, rspec,
+                       rsum0, (const int *)nullptr, 0, n_img, n_shift, n_ring, n_ref, G, twiddle, rec);
+    return hipGetLastError();
+}
+
+hipError_t launch_ccf_single(const float2 *spec, const float2 *norm, const float2 *rspec,
+                             const float *rsum0, const int *ref_of, int ref_stride, int n_img,
+                             int n_shift, int n_ring, int n_ref, const float2 *twiddle,
+                             PeakRec *rec, hipStream_t st) {
+    if (n_img <= 0) return hipSuccess;
+    const int blocks = (n_img + 3) / 4;
+    hipLaunchKernelGGL((k_ccf_psi<1, true>), dim3(blocks), dim3(256), 0, st, spec, norm, rspec,
+                       rsum0, ref_of, ref_stride, n_img, n_shift, n_ring, n_ref, 1, twiddle, rec);
+    return hipGetLastError();
+}
+
+hipError_t launch_decode_mref(const PeakRec *rec, int n_img, int n_ref, ShiftView acc,
+                              const float2 *shifts_f, float *peak, int peak_row, int p_off,
+                              float *d, int d_pitch, hipStream_t st) {
+    const long long n = (long long)n_img * n_ref;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_decode_mref, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, rec,
+                       n_img, n_ref, acc, shifts_f, peak, peak_row, p_off, d, d_pitch);
+    return hipGetLastError();
+}
+
+hipError_t launch_decode_single(const PeakRec *rec, int cnt, DevParam *prm,
+                                const float2 *shifts_f, float shift_limit, hipStream_t st) {
+    if (cnt <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_decode_single, dim3((cnt + 255) / 256), dim3(256), 0, st, rec, cnt, prm,
+                       shifts_f, shift_limit);
+    return hipGetLastError();
+}
+
+hipError_t launch_transform(const float *imgs, int n, int img_dim, const DevParam *prm,
+                            float *out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_transform, dim3(n), dim3(256), 0, st, imgs, img_dim, prm, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_class_average(const float *tr, int img_dim, const int *cid_idx, int n_cls,
+                                float *refs, hipStream_t st) {
+    if (n_cls <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_class_average, dim3(n_cls), dim3(256), 0, st, tr, img_dim, cid_idx, refs);
+    return hipGetLastError();
+}
+
+hipError_t launch_tanl_filter(float *refs, int n, int img_dim, float cutoff, float falloff,
+                              float2 *work, const float2 *tw_d, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const int D = img_dim, Dh = D / 2 + 1;
+    hipLaunchKernelGGL(k_row_dft, dim3(D, n), dim3(128), 0, st, refs, D, work, tw_d);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_col_filter, dim3(Dh, n), dim3(128), D * sizeof(float2), st, work, D,
+                       tw_d, cutoff, falloff);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_row_c2r, dim3(D, n), dim3(128), 0, st, refs, D, work, tw_d);
+    return hipGetLastError();
+}
+
+}  // namespace isac

@@ -1,0 +1,328 @@
+"""ctypes binding of gpu_aln_pack.so, mirroring the reference's binding layer.
+
+The struct mirrors and the pointer-array helper follow bin/sp_isac2_gpu.py:139-187
+and bin/sp_isac_applications.py:96-157; the driver call sequences follow
+bin/sp_isac2_gpu.py:959-1022 (multi-reference), bin/sp_isac_applications.py:
+1445-1693 (pre-alignment) and :4576-4650 (reference-free).  Images are numpy
+float32 arrays instead of EMAN2 EMData objects.
+
+There is no fallback: if the library is missing or a call fails, an exception
+is raised.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "gpu_aln_pack.so")
+RING_LEN = 256
+
+# Symbols declared in include/gpu_aln_pack.h (reference entry points first).
+REFERENCE_SYMBOLS = (
+    "print_gpu_info", "get_texture_limit", "gpu_clear", "reset_shifts",
+    "multi_ref_align_init", "multi_ref_align", "pre_align_init", "pre_align_size_check",
+    "pre_align_fetch", "pre_align_run", "ref_free_alignment_2D_init",
+    "ref_free_alignment_2D_size_check", "ref_free_alignment_2D",
+    "ref_free_alignment_2D_filter_references",
+)
+EXTRA_SYMBOLS = (
+    "gpu_last_error", "gpu_last_error_string", "gpu_clear_error", "isac_mref_align_device",
+    "isac_profile_enable", "isac_profile_read",
+)
+
+
+class AlignConfig(ctypes.Structure):
+    """Layout of cuda/gpu_aln_common.h:62-76."""
+    _fields_ = [("sbj_num", ctypes.c_uint), ("ref_num", ctypes.c_uint),
+                ("img_dim", ctypes.c_uint), ("ring_num", ctypes.c_uint),
+                ("ring_len", ctypes.c_uint), ("shift_step", ctypes.c_float),
+                ("shift_rng_x", ctypes.c_float), ("shift_rng_y", ctypes.c_float),
+                ("aln_type", ctypes.c_char_p)]
+
+
+class AlignParam(ctypes.Structure):
+    """Layout of cuda/gpu_aln_common.h:78-85."""
+    _fields_ = [("sbj_id", ctypes.c_int), ("ref_id", ctypes.c_int),
+                ("shift_x", ctypes.c_float), ("shift_y", ctypes.c_float),
+                ("angle", ctypes.c_float), ("mirror", ctypes.c_bool)]
+
+
+float_ptr = ctypes.POINTER(ctypes.c_float)
+float_ptr_ptr = ctypes.POINTER(float_ptr)
+aln_cfg_ptr = ctypes.POINTER(AlignConfig)
+aln_param_ptr = ctypes.POINTER(AlignParam)
+
+_lib = None
+
+
+class GpuAlignError(RuntimeError):
+    pass
+
+
+def load(path: Optional[str] = None):
+    """Load gpu_aln_pack.so and declare the argument types (no fallback)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise FileNotFoundError(f"{p} not built; run `make -C {PKG_DIR}` (or __graft_entry__.build())")
+    lib = ctypes.CDLL(p)
+    c_uint, c_int, c_float, c_bool = ctypes.c_uint, ctypes.c_int, ctypes.c_float, ctypes.c_bool
+    lib.print_gpu_info.argtypes = [c_uint]
+    lib.print_gpu_info.restype = None
+    lib.get_texture_limit.argtypes = [c_uint]
+    lib.get_texture_limit.restype = c_int
+    lib.gpu_clear.argtypes = []
+    lib.gpu_clear.restype = None
+    lib.reset_shifts.argtypes = [c_float, c_float]
+    lib.reset_shifts.restype = None
+    lib.multi_ref_align_init.argtypes = [aln_cfg_ptr, c_uint]
+    lib.multi_ref_align_init.restype = aln_param_ptr
+    lib.multi_ref_align.argtypes = [float_ptr_ptr, float_ptr_ptr, float_ptr, float_ptr, c_uint, c_float]
+    lib.multi_ref_align.restype = None
+    lib.pre_align_init.argtypes = [c_uint, aln_cfg_ptr, c_uint]
+    lib.pre_align_init.restype = aln_param_ptr
+    lib.pre_align_size_check.argtypes = [c_uint, aln_cfg_ptr, c_uint, c_float, c_bool]
+    lib.pre_align_size_check.restype = c_int
+    lib.pre_align_fetch.argtypes = [float_ptr_ptr, c_uint, ctypes.c_char_p]
+    lib.pre_align_fetch.restype = None
+    lib.pre_align_run.argtypes = [c_int, c_int]
+    lib.pre_align_run.restype = None
+    lib.ref_free_alignment_2D_init.argtypes = [aln_cfg_ptr, float_ptr_ptr, float_ptr_ptr,
+                                               ctypes.POINTER(c_int), c_uint]
+    lib.ref_free_alignment_2D_init.restype = aln_param_ptr
+    lib.ref_free_alignment_2D_size_check.argtypes = [aln_cfg_ptr, c_uint, c_float, c_bool]
+    lib.ref_free_alignment_2D_size_check.restype = c_int
+    lib.ref_free_alignment_2D.argtypes = []
+    lib.ref_free_alignment_2D.restype = None
+    lib.ref_free_alignment_2D_filter_references.argtypes = [c_float, c_float]
+    lib.ref_free_alignment_2D_filter_references.restype = None
+    lib.gpu_last_error.argtypes = []
+    lib.gpu_last_error.restype = c_int
+    lib.gpu_last_error_string.argtypes = []
+    lib.gpu_last_error_string.restype = ctypes.c_char_p
+    lib.gpu_clear_error.argtypes = []
+    lib.gpu_clear_error.restype = None
+    vp = ctypes.c_void_p
+    lib.isac_mref_align_device.argtypes = [vp, vp, vp, c_uint, c_uint, c_uint, c_uint, c_uint,
+                                           c_float, c_float, c_float, vp, vp, c_uint, vp]
+    lib.isac_mref_align_device.restype = c_int
+    lib.isac_profile_enable.argtypes = [c_int]
+    lib.isac_profile_enable.restype = None
+    lib.isac_profile_read.argtypes = [ctypes.POINTER(c_int), ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(c_int), ctypes.POINTER(ctypes.c_double)]
+    lib.isac_profile_read.restype = None
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(lib=None):
+    """Raise GpuAlignError if the library recorded an error (and clear it)."""
+    lib = lib or load()
+    code = lib.gpu_last_error()
+    if code:
+        msg = lib.gpu_last_error_string().decode(errors="replace")
+        lib.gpu_clear_error()
+        raise GpuAlignError(f"gpu_aln_pack error {code}: {msg}")
+
+
+def get_c_ptr_array(images: Sequence[np.ndarray]):
+    """C array of float* to C-contiguous float32 images (bin/sp_isac2_gpu.py:166-174)."""
+    ptrs = []
+    for img in images:
+        assert img.flags["C_CONTIGUOUS"]
+        assert img.dtype == np.float32
+        ptrs.append(img.ctypes.data_as(float_ptr))
+    return (float_ptr * len(ptrs))(*ptrs)
+
+
+def make_config(sbj_num, ref_num, img_dim, ring_num, shift_step, shift_rng_x, shift_rng_y,
+                aln_type: str, ring_len: int = RING_LEN) -> AlignConfig:
+    cfg = AlignConfig(sbj_num, ref_num, img_dim, ring_num, ring_len, shift_step,
+                      shift_rng_x, shift_rng_y, aln_type.encode("utf-8"))
+    return cfg
+
+
+# ------------------------------------------------------------ driver sequences
+
+def multi_ref_align(sbj: np.ndarray, ref: np.ndarray, ring_num: int, shift_step: float,
+                    shift_rng_x: float, shift_rng_y: float, acc_xy=None, device: int = 0,
+                    d_pitch: Optional[int] = None, mem_request: float = 0.9):
+    """Multi-reference call sequence of bin/sp_isac2_gpu.py:943-1006.
+
+    Returns (peak_list [N, 4P] float32, d [N, d_pitch] float32) exactly as the
+    library writes them (before the driver's post-rotation, see post_rotate).
+    """
+    lib = load()
+    sbj = np.ascontiguousarray(sbj, dtype=np.float32)
+    ref = np.ascontiguousarray(ref, dtype=np.float32)
+    P, D = sbj.shape[0], sbj.shape[-1]
+    N = ref.shape[0]
+    pitch = P if d_pitch is None else d_pitch
+    peak_list = np.zeros((N, 4 * P), dtype=np.float32)
+    d = np.zeros((N, pitch), dtype=np.float32)
+    cfg = make_config(P, N, D, ring_num, shift_step, shift_rng_x, shift_rng_y, "mref_aln")
+    prm = lib.multi_ref_align_init(ctypes.byref(cfg), ctypes.c_uint(device))
+    try:
+        check(lib)
+        if acc_xy is not None:
+            acc = np.asarray(acc_xy, dtype=np.float64).reshape(P, 2)
+            for i in range(P):
+                prm[i].shift_x = float(acc[i, 0])
+                prm[i].shift_y = float(acc[i, 1])
+        else:
+            for i in range(P):
+                prm[i].shift_x = 0.0
+                prm[i].shift_y = 0.0
+        imgs_s = [sbj[i] for i in range(P)]
+        imgs_r = [ref[i] for i in range(N)]
+        lib.multi_ref_align(get_c_ptr_array(imgs_s), get_c_ptr_array(imgs_r),
+                            peak_list.ctypes.data_as(float_ptr), d.ctypes.data_as(float_ptr),
+                            ctypes.c_uint(pitch), ctypes.c_float(mem_request))
+        check(lib)
+    finally:
+        lib.gpu_clear()
+    return peak_list, d
+
+
+def post_rotate(peak_list: np.ndarray) -> np.ndarray:
+    """Driver conversion of shifts after the call (bin/sp_isac2_gpu.py:1016-1022)."""
+    out = peak_list.astype(np.float32).copy()
+    angle = out[:, 0::4]
+    sx_neg = -out[:, 1::4]
+    sy_neg = -out[:, 2::4]
+    c_ang = np.cos(np.radians(angle))
+    s_ang = -np.sin(np.radians(angle))
+    out[:, 1::4] = sx_neg * c_ang - sy_neg * s_ang
+    out[:, 2::4] = sx_neg * s_ang + sy_neg * c_ang
+    return out
+
+
+def params_to_numpy(prm, n: int) -> dict:
+    return {"shift_x": np.array([prm[i].shift_x for i in range(n)], np.float32),
+            "shift_y": np.array([prm[i].shift_y for i in range(n)], np.float32),
+            "angle": np.array([prm[i].angle for i in range(n)], np.float32),
+            "mirror": np.array([int(prm[i].mirror) for i in range(n)], np.int32),
+            "ref_id": np.array([prm[i].ref_id for i in range(n)], np.int32)}
+
+
+def pre_align(data: np.ndarray, ref: np.ndarray, ring_num: int, shift_rng: float,
+              shift_step: float = 1.0, batch: Optional[int] = None, acc_xy=None,
+              iterations: int = 1, device: int = 0):
+    """Pre-alignment call sequence of bin/sp_isac_applications.py:1445-1693
+    (one fixed reference; ``iterations`` repeated runs, each fetching the
+    reference and every batch as the driver does).  Returns the AlignParam
+    records as numpy arrays after the last run."""
+    lib = load()
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    n, D = data.shape[0], data.shape[-1]
+    cfg = make_config(n, 1, D, ring_num, shift_step, shift_rng, shift_rng, "pre_aln")
+    if batch is None:
+        limit = 0
+        for split in [2 ** i for i in range(int(np.log2(n)) + 1)][::-1]:
+            cfg.sbj_num = min(limit + split, n)
+            if lib.pre_align_size_check(ctypes.c_uint(n), ctypes.byref(cfg), ctypes.c_uint(device),
+                                        ctypes.c_float(0.9), ctypes.c_bool(False)) == 1:
+                limit += split
+        batch = max(limit, 1)
+    cfg.sbj_num = batch
+    prm = lib.pre_align_init(ctypes.c_uint(n), ctypes.byref(cfg), ctypes.c_uint(device))
+    try:
+        check(lib)
+        if acc_xy is not None:
+            acc = np.asarray(acc_xy, dtype=np.float64).reshape(n, 2)
+            for i in range(n):
+                prm[i].shift_x = float(acc[i, 0])
+                prm[i].shift_y = float(acc[i, 1])
+        n_batch = (n + batch - 1) // batch
+        ref_img = np.ascontiguousarray(ref, dtype=np.float32).reshape(1, D, D)[0]
+        if n_batch == 1:
+            lib.pre_align_fetch(get_c_ptr_array([data[i] for i in range(n)]), ctypes.c_uint(n),
+                                b"sbj_batch")
+            check(lib)
+        for _ in range(iterations):
+            lib.pre_align_fetch(get_c_ptr_array([ref_img]), ctypes.c_uint(1), b"ref_batch")
+            check(lib)
+            for b in range(n_batch):
+                s0, s1 = b * batch, min(n, (b + 1) * batch)
+                if n_batch > 1:
+                    lib.pre_align_fetch(get_c_ptr_array([data[i] for i in range(s0, s1)]),
+                                        ctypes.c_uint(s1 - s0), b"sbj_batch")
+                    check(lib)
+                lib.pre_align_run(ctypes.c_int(s0), ctypes.c_int(s1))
+                check(lib)
+        return params_to_numpy(prm, n)
+    finally:
+        lib.gpu_clear()
+
+
+def ref_free_align(sbj: np.ndarray, cid: np.ndarray, avg: np.ndarray, ring_num: int,
+                   shift_rng: float, num_iter: int, cutoff_freq: float = 0.0,
+                   falloff: float = 0.0, acc_xy=None, device: int = 0):
+    """Reference-free call sequence of bin/sp_isac_applications.py:4576-4650.
+
+    ``avg`` holds the initial class averages.  Returns the AlignParam records."""
+    lib = load()
+    sbj = np.ascontiguousarray(sbj, dtype=np.float32)
+    P, D = sbj.shape[0], sbj.shape[-1]
+    cid = np.asarray(cid, dtype=np.int32)
+    avg = np.ascontiguousarray(avg, dtype=np.float32)
+    cfg = make_config(P, int(cid[-1]) + 1, D, ring_num, 1.0, shift_rng, shift_rng, "multigroup_aln")
+    cid_c = (ctypes.c_int * P)(*[int(c) for c in cid])
+    prm = lib.ref_free_alignment_2D_init(ctypes.byref(cfg), get_c_ptr_array([sbj[i] for i in range(P)]),
+                                         get_c_ptr_array([avg[k] for k in range(avg.shape[0])]),
+                                         cid_c, ctypes.c_uint(device))
+    try:
+        check(lib)
+        if acc_xy is not None:
+            acc = np.asarray(acc_xy, dtype=np.float64).reshape(P, 2)
+            for i in range(P):
+                prm[i].shift_x = float(acc[i, 0])
+                prm[i].shift_y = float(acc[i, 1])
+        for i in range(num_iter):
+            if cutoff_freq > 0.0:
+                freq = 0.1 + (cutoff_freq - 0.1) * i / float(max(num_iter - 1, 1))
+                lib.ref_free_alignment_2D_filter_references(ctypes.c_float(freq), ctypes.c_float(falloff))
+                check(lib)
+            lib.ref_free_alignment_2D()
+            check(lib)
+        return params_to_numpy(prm, P)
+    finally:
+        lib.gpu_clear()
+
+
+def mref_align_device(sbj, ref, ring_num: int, shift_step: float, shift_rng_x: float,
+                      shift_rng_y: float, acc_xy=None, stream=None, out=None):
+    """Device-resident multi-reference alignment on torch tensors (cuda:0).
+
+    sbj (P, D, D), ref (N, D, D) float32 device tensors; returns (peak_list
+    [N, 4P], d [N, P]) device tensors.  ``stream`` is a torch.cuda.Stream."""
+    import torch
+    lib = load()
+    P, D = sbj.shape[0], sbj.shape[-1]
+    N = ref.shape[0]
+    assert sbj.is_cuda and ref.is_cuda and sbj.dtype == torch.float32 and ref.dtype == torch.float32
+    assert sbj.is_contiguous() and ref.is_contiguous()
+    if out is None:
+        peak = torch.empty((N, 4 * P), dtype=torch.float32, device=sbj.device)
+        d = torch.empty((N, P), dtype=torch.float32, device=sbj.device)
+    else:
+        peak, d = out
+    acc_ptr = None
+    if acc_xy is not None:
+        assert acc_xy.is_cuda and acc_xy.dtype == torch.float32 and acc_xy.is_contiguous()
+        acc_ptr = acc_xy.data_ptr()
+    st = (stream or torch.cuda.current_stream()).cuda_stream
+    rc = lib.isac_mref_align_device(sbj.data_ptr(), ref.data_ptr(), acc_ptr, P, N, D, ring_num,
+                                    RING_LEN, shift_step, shift_rng_x, shift_rng_y,
+                                    peak.data_ptr(), d.data_ptr(), P, st)
+    if rc:
+        check(lib)
+        raise GpuAlignError(f"isac_mref_align_device returned {rc}")
+    return peak, d

@@ -1,0 +1,171 @@
+/*
+ * gpu_aln_pack.h -- C ABI of the MI355X (gfx950) polar-ring 2D alignment library.
+ *
+ * Drop-in for the reference's gpu_aln_pack.so: the two structs have the
+ * reference's exact memory layout and the first fourteen functions keep the
+ * reference's names, argument meaning, lifetime rules and signatures, so the
+ * reference's ctypes drivers bind them unchanged.  The functions after them
+ * have no counterpart in the reference and say so.
+ *
+ * Citations are to the ucdavis/GPU-ISAC tree.
+ *
+ * Errors: the reference prints and exit()s on any device error
+ * (cuda/gpu_aln_common.cu:104-118) and assert()s on bad arguments.  This
+ * library prints a message to stderr, records an error code readable through
+ * gpu_last_error(), and returns; with ISAC_EXIT_ON_ERROR=1 in the
+ * environment it exits(1) like the reference.
+ *
+ * Threading: one process-global context, not reentrant, one device per
+ * process (cuda/gpu_aln_noref.cu:31-48, :103).  Every call is synchronous:
+ * results are complete on return.
+ */
+#ifndef ISAC_AMD_GPU_ALN_PACK_H
+#define ISAC_AMD_GPU_ALN_PACK_H
+
+#include <stdbool.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Search configuration; layout of cuda/gpu_aln_common.h:62-76 (40 bytes on LP64).
+ * Retained BY POINTER from *_init until gpu_clear (cuda/gpu_aln_noref.cu:108):
+ * the caller keeps it, and the aln_type string, alive. */
+typedef struct AlignConfig {
+    unsigned int sbj_num;      /* number of subject (particle) images        */
+    unsigned int ref_num;      /* number of references / classes             */
+    unsigned int img_dim;      /* images are img_dim x img_dim float32       */
+    unsigned int ring_num;     /* polar rings at radius 1..ring_num          */
+    unsigned int ring_len;     /* samples per ring; must be 256 (all callers) */
+    float        shift_step;   /* translational search step                  */
+    float        shift_rng_x;  /* search -rng..+rng in x                     */
+    float        shift_rng_y;  /* search -rng..+rng in y                     */
+    const char  *aln_type;     /* "pre_aln", "multigroup_aln" or "mref_aln"  */
+} AlignConfig;
+
+/* Per-particle parameters; layout of cuda/gpu_aln_common.h:78-85 (24 bytes). */
+typedef struct AlignParam {
+    int   sbj_id;
+    int   ref_id;
+    float shift_x;
+    float shift_y;
+    float angle;
+    bool  mirror;
+} AlignParam;
+
+/* ------------------------------------------------ reference entry points -- */
+
+/* Prints the properties of device `device_idx` (cuda/gpu_aln_common.cu:180-209). */
+void print_gpu_info(unsigned int device_idx);
+
+/* Number of img_dim x img_dim references one reference batch may hold
+ * (cuda/gpu_aln_common.cu:233-237).  This library has no texture-height cap;
+ * it returns the count whose polar spectra stay within 32-bit element
+ * offsets.  Callers pass a second, ignored argument
+ * (bin/sp_isac_applications.py:153-154); the C ABI tolerates it. */
+int get_texture_limit(unsigned int img_dim);
+
+/* Frees every resource of the context; the device id stays fixed
+ * (cuda/gpu_aln_noref.cu:139-180). */
+void gpu_clear(void);
+
+/* Rebuilds the shift grid with x and y range `shift_range`; the number of
+ * shifts must not change (cuda/gpu_aln_noref.cu:117-137). */
+void reset_shifts(float shift_range, float shift_step);
+
+/* Multi-reference alignment set-up (cuda/gpu_aln_noref.cu:311-324).  Returns
+ * aln_cfg->sbj_num host-readable/-writable AlignParam records owned by the
+ * library until gpu_clear; callers write the accumulated shift_x/shift_y
+ * before multi_ref_align (bin/sp_isac2_gpu.py:979-983). */
+AlignParam *multi_ref_align_init(const AlignConfig *aln_cfg, unsigned int cuda_device_id);
+
+/* Aligns every particle with every reference (cuda/gpu_aln_noref.cu:326-554).
+ * sbj_data: sbj_num host pointers to C-contiguous img_dim^2 float32 images;
+ * ref_data: ref_num such pointers.  Writes, for particle p and reference n,
+ *   isac_d[n * isac_d_pitch + p]            = maximum CCF value,
+ *   isac_peak_list[n * 4*sbj_num + 4p + 0..3] = {angle, shift_x, shift_y, mirror}
+ * where shift = accumulated shift + the winning grid shift.  isac_d_pitch is
+ * the caller's global particle count; only columns 0..sbj_num-1 of each row
+ * are written.  cuda_mem_request is the fraction of free device memory the
+ * call may use (0.9 in every caller). */
+void multi_ref_align(const float **sbj_data, const float **ref_data,
+                     float *isac_peak_list, float *isac_d,
+                     unsigned int isac_d_pitch, float cuda_mem_request);
+
+/* Pre-alignment set-up for num_particles particles against one reference
+ * (cuda/gpu_aln_noref.cu:565-609).  aln_cfg->sbj_num is the batch capacity. */
+AlignParam *pre_align_init(unsigned int num_particles, const AlignConfig *aln_cfg,
+                           unsigned int cuda_device_id);
+
+/* 1 if a pre-alignment batch of cfg->sbj_num images fits in request * free
+ * device memory, else 0 (cuda/gpu_aln_noref.cu:611-721).  Declared int so
+ * the value is fully zero-extended for callers that read it as a C int
+ * (bin/sp_isac_applications.py:1458 uses ctypes' default c_int restype). */
+int pre_align_size_check(unsigned int num_particles, const AlignConfig *cfg,
+                         unsigned int cuda_device_id, float request, bool verbose);
+
+/* Copies img_num host images into the "sbj_batch" or "ref_batch" buffer
+ * (cuda/gpu_aln_noref.cu:723-740). */
+void pre_align_fetch(const float **img_data, unsigned int img_num, const char *batch_type);
+
+/* Aligns the fetched subject batch (global particle indices start_idx ..
+ * stop_idx-1) to the fetched reference and updates their AlignParam records
+ * in place (cuda/gpu_aln_noref.cu:742-769, :2463-2520). */
+void pre_align_run(int start_idx, int stop_idx);
+
+/* Reference-free (multigroup) set-up: copies all particles and the initial
+ * class references; sbj_cid_list gives each particle's class and must list
+ * each class as one contiguous run (cuda/gpu_aln_noref.cu:782-846). */
+AlignParam *ref_free_alignment_2D_init(const AlignConfig *aln_cfg, const float **sbj_data_list,
+                                       const float **ref_data_list, const int *sbj_cid_list,
+                                       unsigned int cuda_device_id);
+
+/* 1 if a reference-free run of cfg fits in request * free memory, else 0
+ * (cuda/gpu_aln_noref.cu:848-962).  int return: see pre_align_size_check. */
+int ref_free_alignment_2D_size_check(const AlignConfig *cfg, unsigned int cuda_device_id,
+                                     float request, bool verbose);
+
+/* One reference-free iteration: align each particle to its class reference,
+ * update its AlignParam, re-average the transformed particles into new
+ * class references (cuda/gpu_aln_noref.cu:964-996). */
+void ref_free_alignment_2D(void);
+
+/* Tangent low-pass filter of the current class references
+ * (cuda/gpu_aln_noref.cu:998-1003, :2203-2266). */
+void ref_free_alignment_2D_filter_references(float cutoff_freq, float falloff);
+
+/* ------------------------------------------------------------------ extras */
+
+/* No counterpart in the reference.  Error code of the first failure since
+ * the last gpu_clear_error() (0 = none) and its message. */
+int gpu_last_error(void);
+const char *gpu_last_error_string(void);
+void gpu_clear_error(void);
+
+/* No counterpart in the reference (device-resident form of multi_ref_align,
+ * for callers whose images already live in device memory, e.g. the
+ * benchmark and the Python host layer).  All pointers are device pointers:
+ *   d_sbj [n_sbj][img_dim][img_dim], d_ref [n_ref][img_dim][img_dim] float32,
+ *   d_acc_xy [n_sbj][2] accumulated shifts (NULL = zero),
+ *   d_peak_list [n_ref][4*n_sbj], d_d [n_ref][d_pitch].
+ * `stream` is a hipStream_t (NULL = the library's own stream); the call
+ * returns after the work on it has completed.  Returns 0 or an error code. */
+int isac_mref_align_device(const float *d_sbj, const float *d_ref, const float *d_acc_xy,
+                           unsigned int n_sbj, unsigned int n_ref, unsigned int img_dim,
+                           unsigned int ring_num, unsigned int ring_len,
+                           float shift_step, float shift_rng_x, float shift_rng_y,
+                           float *d_peak_list, float *d_d, unsigned int d_pitch,
+                           void *stream);
+
+/* No counterpart in the reference.  When enabled, the library brackets every
+ * launch of its contraction+psi kernel with HIP events on the stream it
+ * launches on, and accumulates launch count and total milliseconds. */
+void isac_profile_enable(int on);
+void isac_profile_read(int *launches, double *total_ms, int *polar_launches,
+                       double *polar_total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
