@@ -1,0 +1,172 @@
+"""Benchmark of the multi-reference polar alignment hot path on one MI355X.
+
+Workload (default "cfg2"): 20,000 float32 particles of 76x76 px rendered from
+100 seeded blob references (radius <= 34 px) with random psi, mirror and
+sub-pixel shifts in +-3 px at SNR 0.05; search over rings 1..35, shifts +-3
+step 1 (49 shifts), 256 psi samples and both mirrors.  A step is one
+multi-reference alignment of every particle against every reference with the
+images already resident in HBM (isac_mref_align_device); the metric is
+particle x reference alignments per second.
+
+Also reported:
+  roofline      the fused contraction + psi-IFFT kernel (k_ccf_psi): algorithmic
+                flops per launch (pairs x (8 S R K_f + 2 S 2.5 L log2 L)) over its
+                average launch time, timed with HIP events on its stream
+  cpu_baseline  the same algorithm's C restatement (FP32 build, OpenMP) on the
+                host cores, on a time-bounded sample of the same particles; its
+                FP64 build checks the GPU parameters on that sample
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(ROOT, "gpu-isac_amd"), ROOT):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+WORKLOADS = {
+    # name: (P, N, D, radius, snr, max_shift, integer_shifts, R, shift_rng)
+    "cfg2": (20000, 100, 76, 34.0, 0.05, 3.0, False, 35, 3.0),
+    "cfg1": (1000, 8, 64, 28.0, 0.1, 2.0, True, 29, 2.0),
+}
+PEAK_FP32_TFLOPS = 157.3     # MI355X published FP32 peak
+RING_LEN = 256
+
+
+def flops_per_pair(S, R, L=RING_LEN):
+    K = L // 2 + 1
+    return 8.0 * S * R * K + 2.0 * S * 2.5 * L * math.log2(L)
+
+
+def cpu_baseline(sbj_host, ref_host, acc, R, rng, gpu_peak, gpu_d, budget_s, cores):
+    import numpy as np
+    from oracle import coracle
+    P = sbj_host.shape[0]
+    N = ref_host.shape[0]
+    chunk = 16
+    done = 0
+    t0 = time.perf_counter()
+    while done < P and time.perf_counter() - t0 < budget_s:
+        n = min(chunk, P - done)
+        coracle.mref(sbj_host[done:done + n], ref_host, acc[done:done + n], R, rng, rng, 1.0,
+                     nthreads=cores, precision="f32")
+        done += n
+    el = time.perf_counter() - t0
+    out = {"value": done * N / el, "unit": "pairs/s", "cores": cores, "kind": "port",
+           "sample": f"first {done} particles x {N} refs of the same workload "
+                     f"({el:.1f} s, C restatement, FP32, OpenMP)"}
+    # parameter agreement of the GPU run with the FP64 oracle on the first particles
+    m = min(done, 64)
+    ref = coracle.mref(sbj_host[:m], ref_host, acc[:m], R, rng, rng, 1.0, nthreads=cores,
+                       precision="f64")
+    pk = gpu_peak[:, :4 * m]
+    tie = ref["tie"]
+    same = (pk[:, 3::4] == ref["mirror"]) & (np.abs(pk[:, 1::4] - ref["sx"]) < 1e-4) & \
+           (np.abs(pk[:, 2::4] - ref["sy"]) < 1e-4)
+    dpsi = np.abs((pk[:, 0::4] - ref["angle"] + 180.0) % 360.0 - 180.0)
+    drel = np.abs(gpu_d[:, :m] - ref["d"]) / np.abs(ref["d"])
+    out["agreement_vs_fp64"] = {
+        "pairs": int(m * N), "ties": int(tie.sum()),
+        "shift_mirror_exact": float(same.mean()),
+        "shift_mirror_exact_non_tie": float(same[~tie].mean()) if (~tie).any() else 1.0,
+        "psi_within_0.5deg": float((dpsi <= 0.5).mean()),
+        "psi_max_err_deg_non_tie": float(dpsi[~tie].max()) if (~tie).any() else 0.0,
+        "ccf_max_rel_err": float(drel.max())}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
+    ap.add_argument("--particles", type=int, default=None, help="override particle count")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
+              "the path shards over particles with no cross-GPU exchange until the end, "
+              "see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from isac_amd import binding, synth
+
+    P, N, D, radius, snr, max_shift, int_shifts, R, rng = WORKLOADS[args.workload]
+    if args.particles:
+        P = args.particles
+    S = int((2 * rng + 1) ** 2)
+    dev = torch.device("cuda:0")
+    sbj, ref, meta = synth.make_dataset_torch(P, N, D, radius, snr, max_shift, int_shifts, dev)
+    acc = torch.zeros((P, 2), dtype=torch.float32, device=dev)
+    peak = torch.empty((N, 4 * P), dtype=torch.float32, device=dev)
+    dval = torch.empty((N, P), dtype=torch.float32, device=dev)
+    lib = binding.load()
+    torch.cuda.synchronize()
+
+    def step():
+        binding.mref_align_device(sbj, ref, R, 1.0, rng, rng, acc_xy=acc, out=(peak, dval))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    lib.isac_profile_enable(1)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    import ctypes
+    n_ccf, ms_ccf = ctypes.c_int(0), ctypes.c_double(0.0)
+    n_pol, ms_pol = ctypes.c_int(0), ctypes.c_double(0.0)
+    lib.isac_profile_read(ctypes.byref(n_ccf), ctypes.byref(ms_ccf), ctypes.byref(n_pol),
+                          ctypes.byref(ms_pol))
+    lib.isac_profile_enable(0)
+    binding.check(lib)
+
+    ms_step = el / args.steps * 1e3
+    value = P * N / (el / args.steps)
+    launches = max(n_ccf.value, 1)
+    avg_launch_ms = ms_ccf.value / launches
+    pairs_per_launch = P * N * args.steps / launches
+    fl = flops_per_pair(S, R) * pairs_per_launch
+    achieved = fl / (avg_launch_ms * 1e-3) / 1e12
+    res = {
+        "metric": "particle x reference alignments/sec (76 px, 49 shifts, 256 psi, 2 mirrors)",
+        "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded blob references, rendered particles + Gaussian noise)",
+        "config": {"workload": f"{args.workload}: {P} particles x {N} refs, {D}x{D} px, "
+                               f"rings 1..{R}, shifts +-{int(rng)} step 1 ({S}), psi 256 x 2 mirrors",
+                   "particles": P, "refs": N, "img_dim": D, "rings": R, "shifts": S},
+        "roofline": {"bound": "mfma", "kernel": "k_ccf_psi (ring contraction + psi IFFT + search)",
+                     "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "avg_launch_ms": avg_launch_ms, "launches": launches,
+                     "flop_per_pair": flops_per_pair(S, R),
+                     "polar_kernel_ms_per_step": ms_pol.value / args.steps},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        cores = min(16, len(os.sched_getaffinity(0)))
+        m = min(P, 4096)
+        sbj_h = sbj[:m].cpu().numpy()
+        res["cpu_baseline"] = cpu_baseline(sbj_h, ref.cpu().numpy(), np.zeros((m, 2), np.float32),
+                                           R, rng, peak.cpu().numpy(), dval.cpu().numpy(),
+                                           args.cpu_seconds, cores)
+        res["cpu_baseline"]["gpu_over_cpu"] = value / res["cpu_baseline"]["value"]
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

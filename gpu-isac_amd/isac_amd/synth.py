@@ -151,3 +151,67 @@ def config1(n_particles: int = 1000, snr=0.1) -> Dataset:
 def config2(n_particles: int = 20000, snr=0.05, n_refs: int = 100) -> Dataset:
     """20,000 x 76^2 particles, 100 refs (r <= 34), sub-pixel shifts +-3, SNR 0.05."""
     return make_dataset(n_particles, n_refs, 76, 34.0, snr, 3.0, False)
+
+
+def make_dataset_torch(n_particles: int, n_refs: int, D: int, radius: float, snr,
+                       max_shift: float, integer_shifts: bool, device="cuda",
+                       seed_refs: int = 1, seed_particles: int = 2, seed_noise: int = 3,
+                       chunk: int = 1024):
+    """Same model and parameter draws as make_dataset, rendered with torch on
+    ``device`` in float32 (fast for large particle counts; the noise stream
+    differs from the numpy one).  Returns (sbj, ref) device tensors plus the
+    generating parameters as numpy arrays."""
+    import torch
+    blobs = make_blobs(n_refs, radius, seed_refs)
+    ref = render_refs(blobs, D)
+    rng = np.random.default_rng(seed_particles)
+    ref_idx = rng.integers(0, n_refs, n_particles)
+    phi = rng.uniform(0.0, 360.0, n_particles)
+    if integer_shifts:
+        m = int(round(max_shift))
+        shift = rng.integers(-m, m + 1, (n_particles, 2)).astype(np.float64)
+    else:
+        shift = rng.uniform(-max_shift, max_shift, (n_particles, 2))
+    mirror = rng.integers(0, 2, n_particles)
+    # padded blob table [n_refs, B, 7]: cx, cy, 1/s1^2, 1/s2^2, cos th, sin th, amp
+    B = int(np.bincount(blobs.ref, minlength=n_refs).max())
+    tab = np.zeros((n_refs, B, 7), np.float32)
+    cnt = np.zeros(n_refs, np.int64)
+    for j in range(blobs.ref.shape[0]):
+        r = blobs.ref[j]
+        tab[r, cnt[r]] = (blobs.cx[j], blobs.cy[j], 1.0 / blobs.s1[j] ** 2, 1.0 / blobs.s2[j] ** 2,
+                          np.cos(blobs.th[j]), np.sin(blobs.th[j]), blobs.amp[j])
+        cnt[r] += 1
+    dev = torch.device(device)
+    tab_t = torch.from_numpy(tab).to(dev)
+    c = D // 2
+    ax = torch.arange(D, device=dev, dtype=torch.float32) - c
+    yy, xx = torch.meshgrid(ax, ax, indexing="ij")
+    inside = (xx * xx + yy * yy) <= radius * radius
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed_noise)
+    sbj = torch.empty((n_particles, D, D), dtype=torch.float32, device=dev)
+    for p0 in range(0, n_particles, chunk):
+        p1 = min(n_particles, p0 + chunk)
+        ph = torch.from_numpy(np.radians(phi[p0:p1])).float().to(dev).view(-1, 1, 1)
+        sh = torch.from_numpy(shift[p0:p1]).float().to(dev)
+        qx = xx[None] - sh[:, 0].view(-1, 1, 1)
+        qy = yy[None] - sh[:, 1].view(-1, 1, 1)
+        cp, sp = torch.cos(ph), torch.sin(ph)
+        rx = qx * cp + qy * sp
+        ry = -qx * sp + qy * cp
+        mir = torch.from_numpy(mirror[p0:p1]).to(dev).view(-1, 1, 1)
+        rx = torch.where(mir == 1, -rx, rx)
+        t = tab_t[torch.from_numpy(ref_idx[p0:p1]).to(dev)]          # (b, B, 7)
+        dx = rx[:, None] - t[:, :, 0, None, None]
+        dy = ry[:, None] - t[:, :, 1, None, None]
+        u = dx * t[:, :, 4, None, None] + dy * t[:, :, 5, None, None]
+        v = -dx * t[:, :, 5, None, None] + dy * t[:, :, 4, None, None]
+        img = (t[:, :, 6, None, None] * torch.exp(-0.5 * (u * u * t[:, :, 2, None, None]
+                                                          + v * v * t[:, :, 3, None, None]))).sum(1)
+        if snr is not None and np.isfinite(snr):
+            var = img[:, inside].var(dim=1, unbiased=False)
+            sig = torch.sqrt(var / float(snr)).view(-1, 1, 1)
+            img = img + sig * torch.randn(img.shape, generator=gen, device=dev)
+        sbj[p0:p1] = img
+    return sbj, torch.from_numpy(ref).to(dev), Dataset(None, ref, ref_idx, phi, shift, mirror)

@@ -1,0 +1,85 @@
+"""C-ABI checks that need no GPU: exports, struct layout, host-side error paths."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from isac_amd import binding
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "gpu_aln_pack.h")
+
+
+def header_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\([^;{]*\)\s*;", txt)
+    return [n for n in names if n not in ("sizeof",)]
+
+
+def test_header_declares_reference_symbols():
+    names = header_functions()
+    for s in binding.REFERENCE_SYMBOLS + binding.EXTRA_SYMBOLS:
+        assert s in names, s
+    assert len(set(names)) == len(binding.REFERENCE_SYMBOLS) + len(binding.EXTRA_SYMBOLS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(binding.LIB_PATH)
+    for s in header_functions():
+        assert hasattr(lib, s), f"{s} not exported"
+
+
+def test_struct_layout_matches_reference():
+    # cuda/gpu_aln_common.h:62-85 on LP64
+    assert ctypes.sizeof(binding.AlignConfig) == 40
+    assert binding.AlignConfig.aln_type.offset == 32
+    assert binding.AlignConfig.shift_step.offset == 20
+    assert ctypes.sizeof(binding.AlignParam) == 24
+    assert binding.AlignParam.mirror.offset == 20
+    assert binding.AlignParam.shift_x.offset == 8
+
+
+def test_texture_limit_positive():
+    lib = binding.load()
+    assert lib.get_texture_limit(76) > 1000
+    assert lib.get_texture_limit(0) == 0
+
+
+def test_calls_without_init_report_errors():
+    lib = binding.load()
+    lib.gpu_clear_error()
+    lib.pre_align_run(0, 10)
+    assert lib.gpu_last_error() == 3
+    with pytest.raises(binding.GpuAlignError):
+        binding.check(lib)
+    assert lib.gpu_last_error() == 0
+    lib.reset_shifts(2.0, 1.0)
+    assert lib.gpu_last_error() == 3
+    lib.gpu_clear_error()
+    pk = np.zeros(4, np.float32)
+    d = np.zeros(1, np.float32)
+    lib.multi_ref_align(None, None, pk.ctypes.data_as(binding.float_ptr),
+                        d.ctypes.data_as(binding.float_ptr), 1, 0.9)
+    assert lib.gpu_last_error() == 3
+    lib.gpu_clear_error()
+    lib.ref_free_alignment_2D()
+    assert lib.gpu_last_error() == 3
+    lib.gpu_clear_error()
+    lib.pre_align_fetch(None, 0, b"sbj_batch")
+    assert lib.gpu_last_error() == 3
+    lib.gpu_clear_error()
+
+
+def test_driver_post_rotation_matches_oracle():
+    from oracle import isac_oracle as O
+    rng = np.random.default_rng(0)
+    pk = rng.uniform(-3, 3, (3, 4 * 5)).astype(np.float32)
+    pk[:, 0::4] = rng.uniform(0, 360, (3, 5))
+    out = binding.post_rotate(pk)
+    sx, sy = O.driver_post_rotation(pk[:, 0::4].astype(np.float64), pk[:, 1::4].astype(np.float64),
+                                    pk[:, 2::4].astype(np.float64))
+    assert np.allclose(out[:, 1::4], sx, atol=1e-5) and np.allclose(out[:, 2::4], sy, atol=1e-5)
+    assert np.array_equal(out[:, 0::4], pk[:, 0::4]) and np.array_equal(out[:, 3::4], pk[:, 3::4])

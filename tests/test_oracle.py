@@ -1,0 +1,325 @@
+"""Pins of the FP64 oracle (CPU).
+
+Each test pins one oracle function against something that does not restate
+its formula: a closed form, a brute-force sum on a tiny input, an independent
+library routine, or an invariant of the mathematics.  A plausible slip (sign,
+index, dropped term, transposed operand) fails at least one of these.
+"""
+import numpy as np
+import pytest
+
+from isac_amd import synth
+from oracle import coracle
+from oracle import isac_oracle as O
+
+
+# --------------------------------------------------------------- P1 grids
+
+def test_polar_template_closed_form():
+    ox, oy = O.polar_template(3, 8)
+    # ring i has radius i+1; j=0 on +x, j=L/4 on +y, j=L/2 on -x
+    assert np.allclose([ox[0, 0], oy[0, 0]], [1, 0])
+    assert np.allclose([ox[1, 2], oy[1, 2]], [0, 2])
+    assert np.allclose([ox[2, 4], oy[2, 4]], [-3, 0])
+    assert np.allclose(np.hypot(ox, oy), np.arange(1, 4)[:, None] * np.ones((1, 8)))
+
+
+def test_polar_template_matches_float_formula():
+    # the library computes the template in float like cuda/gpu_aln_common.cu:56-58
+    R, L = 35, 256
+    ox, oy = O.polar_template(R, L)
+    j = np.arange(L, dtype=np.float32)
+    ang = (j / np.float32(L)).astype(np.float64) * 3.14159265358979 * 2
+    ang = ang.astype(np.float32)
+    fx = np.cos(ang)[None, :] * np.arange(1, R + 1, dtype=np.float32)[:, None]
+    assert np.abs(fx - ox).max() < 4e-5
+
+
+def test_shift_grid_order_and_count():
+    g = O.shift_grid(2.0, 1.0, 1.0)
+    assert g.shape == (15, 2) and O.shift_count(2.0, 1.0, 1.0) == 15
+    assert tuple(g[0]) == (-2.0, -1.0) and tuple(g[1]) == (-2.0, 0.0) and tuple(g[3]) == (-1.0, -1.0)
+    assert tuple(g[-1]) == (2.0, 1.0)
+    # x-major: index = ix * (2Y/h+1) + iy
+    ix, iy = 3, 2
+    assert tuple(g[ix * 3 + iy]) == (-2.0 + ix, -1.0 + iy)
+    assert O.shift_grid(3, 3, 1).shape[0] == 49 and O.shift_grid(0, 0, 1).shape[0] == 1
+
+
+# ----------------------------------------------------------- sampling pins
+
+def test_bilinear_closed_form():
+    img = np.arange(20, dtype=np.float64).reshape(4, 5)   # img[y, x] = 5y + x (linear)
+    x = np.array([0.0, 1.25, 3.5, 2.0])
+    y = np.array([0.0, 2.5, 1.75, 3.0])
+    assert np.allclose(O.bilinear_clamped(img, x, y), 5 * y + x)   # exact on linear images
+    # clamp to edge
+    assert O.bilinear_clamped(img, np.array([-3.0]), np.array([-2.0]))[0] == img[0, 0]
+    assert O.bilinear_clamped(img, np.array([9.5]), np.array([3.2]))[0] == img[3, 4]
+
+
+def test_normalize_moments():
+    rng = np.random.default_rng(0)
+    p = rng.normal(3.0, 2.0, (7, 32))
+    q = O.normalize_polar(p)
+    w = np.arange(1, 8)[:, None]
+    nn = w.sum() * 32
+    mu = (w * q).sum() / nn
+    var = (w * q * q).sum() / nn - mu * mu
+    # weighted mean 0, weighted variance (sd/(sd+1e-7))^2
+    pm = (w * p).sum() / nn
+    sd = np.sqrt((w * p * p).sum() / nn - pm * pm)
+    assert abs(mu) < 1e-12
+    assert abs(var - (sd / (sd + 1e-7)) ** 2) < 1e-12
+
+
+# ------------------------------------------------------------ P2 FFT pins
+
+def _dft(x, sign):
+    L = x.shape[-1]
+    j = np.arange(L)
+    return (x[..., None, :] * np.exp(sign * 2j * np.pi * np.outer(j, j) / L)).sum(-1)
+
+
+def test_ring_spectra_brute_force():
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=(3, 16))
+    X = O.ring_spectra(x)
+    assert np.allclose(X, _dft(x, -1)[..., :9])       # unnormalised, e^{-i}
+
+
+def test_psi_ccf_brute_force():
+    rng = np.random.default_rng(2)
+    L = 16
+    c = rng.normal(size=L // 2 + 1) + 1j * rng.normal(size=L // 2 + 1)
+    c[0] = c[0].real
+    c[-1] = c[-1].real
+    full = np.concatenate([c, np.conj(c[1:-1][::-1])])
+    ref = _dft(full, +1).real                           # unnormalised C2R, e^{+i}
+    assert np.allclose(O.psi_ccf(c, L), ref)
+    t = np.arange(L)
+    closed = c[0].real + c[-1].real * (-1.0) ** t + 2 * sum(
+        (c[k] * np.exp(2j * np.pi * k * t / L)).real for k in range(1, L // 2))
+    assert np.allclose(O.psi_ccf(c, L), closed)
+
+
+# ----------------------------------------------- P3 correlation theorem
+
+def test_ccf_correlation_theorem():
+    rng = np.random.default_rng(3)
+    R, L = 3, 16
+    r = rng.normal(size=(R, L))
+    s = rng.normal(size=(R, L))
+    c0, c1 = O.ccf_spectra(O.ring_spectra(r), O.ring_spectra(s))
+    z0, z1 = O.psi_ccf(c0, L), O.psi_ccf(c1, L)
+    w = np.arange(1, R + 1)
+    b0 = np.array([L * sum(w[i] * sum(r[i, (j + t) % L] * s[i, j] for j in range(L)) for i in range(R))
+                   for t in range(L)])
+    b1 = np.array([L * sum(w[i] * sum(r[i, j] * s[i, (-j - t) % L] for j in range(L)) for i in range(R))
+                   for t in range(L)])
+    assert np.allclose(z0, b0) and np.allclose(z1, b1)
+
+
+# ------------------------------------------------------- P6 parabola
+
+@pytest.mark.parametrize("delta", [-0.5, -0.31, 0.0, 0.17, 0.5])
+def test_parabola_exact(delta):
+    q = np.arange(-3, 4)
+    x = -(q - delta) ** 2 + 5.0
+    assert abs(O.parabola_delta(x) - delta) < 1e-12
+
+
+def test_parabola_flat_is_zero():
+    assert O.parabola_delta(np.ones(7)) == 0.0
+
+
+def test_angle_decode_conventions():
+    # multi-ref: wrap >= 360 for both mirrors; single-ref: only inside the mirror branch
+    assert O.angle_multiref(0, -0.5, 0) == pytest.approx(360.0 / 256 * 0.5)
+    assert O.angle_multiref(10, 0.0, 1) == pytest.approx(360 - 10 * 360 / 256 + 180 - 360)
+    assert O.angle_single_ref(0, -0.5, 0) == pytest.approx(360 + 360.0 / 256 * 0.5)   # not wrapped
+    assert O.angle_single_ref(0, 0.0, 1) == pytest.approx(180.0)
+
+
+# -------------------------------------- P4/P5 rotation, shift and mirror
+
+def test_recovers_rotation_shift_mirror_noise_free():
+    ds = synth.make_dataset(16, 3, 64, 28.0, None, 2, True)
+    res = O.multi_ref_align(ds.sbj, ds.ref, np.zeros((16, 2)), 29, 2, 2, 1)
+    for p in range(16):
+        n = ds.ref_idx[p]
+        assert res["mirror"][n, p] == ds.mirror[p]
+        assert res["sx"][n, p] == ds.shift[p, 0] and res["sy"][n, p] == ds.shift[p, 1]
+        da = abs((res["angle"][n, p] - ds.phi[p] + 180) % 360 - 180)
+        assert da < 0.1, (p, res["angle"][n, p], ds.phi[p])
+        assert res["d"][:, p].argmax() == n            # generating reference scores highest
+
+
+def test_exact_polar_rotation_gives_integer_peak():
+    # a particle equal to the reference rotated by k psi steps about the centre,
+    # built by rotating the image with an exact 90 degree turn (k = 64)
+    ds = synth.make_dataset(1, 1, 64, 28.0, None, 0, True)
+    ref = ds.ref
+    c = 32
+    rot = np.empty_like(ref[0])
+    yy, xx = np.mgrid[0:64, 0:64]
+    # particle(x) = ref(R_{-90}(x - c) + c): R_{-90}(dx, dy) = (dy, -dx)
+    sx = (yy - c) + c
+    sy = -(xx - c) + c
+    ok = (sx >= 0) & (sx < 64) & (sy >= 0) & (sy < 64)
+    rot[ok] = ref[0][sy[ok], sx[ok]]
+    rot[~ok] = 0
+    res = O.multi_ref_align(rot[None], ref, np.zeros((1, 2)), 29, 0, 0, 1)
+    assert res["t"][0, 0] == 256 - 64 and res["mirror"][0, 0] == 0
+    assert abs(res["delta"][0, 0]) < 1e-6              # symmetric peak: parabola offset 0
+    assert res["angle"][0, 0] == pytest.approx(90.0, abs=1e-5)
+
+
+def test_pure_mirror_selects_mirror():
+    ds = synth.make_dataset(1, 1, 64, 28.0, None, 0, True)
+    ref = ds.ref
+    mir = np.empty_like(ref)
+    mir[0, :, 1:] = ref[0, :, 1:][:, ::-1]    # x -> 64 - x about centre 32
+    mir[0, :, 0] = ref[0, :, 0]
+    res = O.multi_ref_align(mir, ref, np.zeros((1, 2)), 29, 0, 0, 1)
+    assert res["mirror"][0, 0] == 1 and res["t"][0, 0] == 128
+    assert abs((res["angle"][0, 0] + 180) % 360 - 180) < 1e-4
+
+
+# --------------------------------------------------------- P8 tie-break
+
+def test_first_max_wins_order():
+    # the oracle's decode scans (m, s, t) flattened: the first maximum wins
+    tab = np.zeros((1, 1, 2, 3, 8))
+    tab[0, 0, 1, 0, 2] = 5.0     # mirror 1, shift 0
+    tab[0, 0, 0, 2, 7] = 5.0     # mirror 0, shift 2  (earlier in (m, s, t) order)
+    flat = tab.reshape(1, 1, -1)
+    idx = np.argmax(flat, axis=-1)[0, 0]
+    assert idx == (0 * 3 + 2) * 8 + 7
+
+
+def test_c_oracle_matches_numpy_oracle():
+    ds = synth.make_dataset(10, 4, 48, 20.0, 0.2, 2, True)
+    acc = np.random.default_rng(4).uniform(-1, 1, (10, 2)).astype(np.float32)
+    a = O.multi_ref_align(ds.sbj, ds.ref, acc, 20, 2, 2, 1)
+    b = coracle.mref(ds.sbj, ds.ref, acc, 20, 2, 2, 1)
+    for k in ("mirror", "shift_idx", "t"):
+        assert np.array_equal(np.asarray(a[k]), np.asarray(b[k]))
+    assert np.allclose(a["d"], b["d"], rtol=1e-6)
+    assert np.allclose(a["delta"], b["delta"], atol=1e-9)
+    assert np.allclose(a["sx"], b["sx"], atol=1e-6) and np.allclose(a["sy"], b["sy"], atol=1e-6)
+
+
+# -------------------------------------------- P9 / P10 outputs, conversion
+
+def test_peak_list_layout():
+    res = {k: np.arange(6, dtype=float).reshape(2, 3) + 10 * i
+           for i, k in enumerate(("angle", "sx", "sy", "mirror"))}
+    res["d"] = np.zeros((2, 3))
+    pk = O.peak_list_layout(res, 3)
+    assert pk.shape == (2, 12)
+    assert pk[1, 4 * 2 + 0] == res["angle"][1, 2] and pk[1, 4 * 2 + 3] == res["mirror"][1, 2]
+    assert pk[0, 4 * 1 + 1] == res["sx"][0, 1] and pk[0, 4 * 1 + 2] == res["sy"][0, 1]
+
+
+def test_driver_post_rotation_closed_form():
+    sx, sy = O.driver_post_rotation(np.array([90.0]), np.array([1.0]), np.array([0.0]))
+    assert np.allclose([sx[0], sy[0]], [0.0, 1.0])
+    sx, sy = O.driver_post_rotation(np.array([0.0]), np.array([2.0]), np.array([3.0]))
+    assert np.allclose([sx[0], sy[0]], [-2.0, -3.0])
+    a = np.array([33.0])
+    sx, sy = O.driver_post_rotation(a, np.array([1.5]), np.array([-0.5]))
+    assert np.hypot(sx, sy)[0] == pytest.approx(np.hypot(1.5, -0.5))
+
+
+# ------------------------------------------------- single-ref decode (S13)
+
+def test_single_ref_decode_init_zero_and_clamp():
+    S, L = 4, 256
+    tab = -np.ones((2, 2, S, L))                        # all negative -> index 0
+    tab[1, 1, 3, 10] = 2.0                              # particle 1: mirror 1, shift 3, t 10
+    shifts = np.array([[-1, -1], [-1, 0], [0, 0], [5, 5]], np.float32)
+    acc = np.array([[0.0, 0.0], [9.0, -9.0]])
+    prm = O.decode_single_ref(tab, acc, shifts, 32, 20, L)
+    assert prm["mirror"][0] == 0 and prm["shift_idx"][0] == 0 and prm["t"][0] == 0
+    assert prm["shift_x"][0] == -1 and prm["shift_y"][0] == -1
+    lim = 32 - 20 - 2
+    assert prm["mirror"][1] == 1 and prm["t"][1] == 10
+    assert prm["shift_x"][1] == lim and prm["shift_y"][1] == -4.0      # x clamped, y not
+    # isolated spike: c2 = -96, c3 = -12 -> delta = 0; angle = 360 - 10*1.40625 + 180 - 360
+    assert prm["delta"][1] == 0.0
+    assert prm["angle"][1] == pytest.approx(165.9375, abs=1e-12)
+
+
+# ------------------------------------ transform / average / tangent filter
+
+def test_transform_identity_and_quarter_turn():
+    rng = np.random.default_rng(5)
+    img = rng.normal(size=(1, 16, 16))
+    out = O.transform_images(img, [0.0], [0.0], [0.0], [0])
+    assert np.allclose(out, img)
+    out = O.transform_images(img, [90.0], [0.0], [0.0], [0])
+    c = 8
+    # out[y, x] = img[x, 2c - y] where in range
+    for y in range(1, 16):
+        for x in range(16):
+            assert out[0, y, x] == pytest.approx(img[0, x, 2 * c - y], abs=1e-9)
+    out = O.transform_images(img, [0.0], [1.0], [-2.0], [1])   # mirror x -> 16 - x, shift
+    for y in range(2, 16):
+        for x in range(2, 16):
+            assert out[0, y, x] == pytest.approx(img[0, y - 2, 16 - x + 1], abs=1e-9)
+
+
+def test_class_averages():
+    imgs = np.arange(5 * 4, dtype=float).reshape(5, 2, 2)
+    av = O.class_averages(imgs, [0, 2, 5])
+    assert np.allclose(av[0], imgs[:2].mean(0)) and np.allclose(av[1], imgs[2:].mean(0))
+
+
+def test_tangent_filter_on_single_frequency():
+    D = 32
+    y, x = np.mgrid[0:D, 0:D]
+    kx, ky = 3, 5
+    img = np.cos(2 * np.pi * (kx * x + ky * y) / D)
+    refs = np.stack([img, img])
+    cut, fall = 0.12, 0.3
+    out = O.tangent_filter(refs, cut, fall)
+    xx = kx / (D // 2 + 1) * 0.5
+    yy = min(ky, D - ky) / (D / 2) * 0.5
+    dd = np.hypot(xx, yy)
+    cc = np.pi / (2 * fall * cut)
+    H = 0.5 * (np.tanh(cc * (dd + cut)) - np.tanh(cc * (dd - cut)))
+    assert np.allclose(out[1], H * img * D * D, atol=1e-8)          # gain D^2 (unnormalised)
+    assert np.allclose(out[0], H * img / (D * D), atol=1e-12)       # first image gain 1/D^2
+
+
+def test_ref_free_iteration_converges_noise_free():
+    # class members are rotated/mirrored copies of one reference; starting
+    # from their plain mean, a few iterations recover a sharp average
+    D, R = 32, 12
+    blobs = synth.make_blobs(1, 12.0, 9)
+    ref = synth.render_refs(blobs, D)
+    rng = np.random.default_rng(6)
+    P = 12
+    phi = rng.uniform(0, 360, P)
+    mir = rng.integers(0, 2, P)
+    sbj = synth.render_particles(blobs, D, np.zeros(P, int), phi, np.zeros((P, 2)), mir, None, 12.0, 0)
+    cid = np.zeros(P, int)
+    avg = sbj.mean(0, keepdims=True).astype(np.float64)
+    acc = np.zeros((P, 2))
+    shifts = O.shift_grid(1, 1, 1)
+
+    def best_corr(a):
+        best = -1
+        for m in (0, 1):
+            for ang in np.arange(0, 360, 2.0):
+                t = O.transform_images(ref.astype(np.float64), [ang], [0.0], [0.0], [m])[0]
+                best = max(best, np.corrcoef(t.ravel(), a.ravel())[0, 1])
+        return best
+
+    c_start = best_corr(avg[0])
+    for _ in range(3):
+        prm, avg = O.ref_free_iteration(sbj, avg, cid, acc, R, shifts, D)
+        acc = np.stack([prm["shift_x"], prm["shift_y"]], 1)
+    c_end = best_corr(avg[0])
+    assert c_end > 0.97 and c_end > c_start
