@@ -127,7 +127,8 @@ __device__ __forceinline__ void wave_argmax(float &v, int &t) {
 }
 
 // ------------------------------------------------------------------ stage A
-// Block = 4 wavefronts = 4 grid shifts of one image; the image is staged in LDS.
+// Block = 4 wavefronts on one image (staged once in LDS); wavefront w handles
+// grid shifts w, w+4, w+8, ...
 // A wavefront samples two rings at a time (ring a real part, ring b imaginary
 // part), runs one 256-point FFT and unpacks both real spectra.  The normalisation
 // sums (radius-weighted sum and sum of squares) accumulate in double.
@@ -145,13 +146,12 @@ __global__ __launch_bounds__(256) void k_polar_spectra(
     for (int i = tid; i < D * D; i += 256) img[i] = src[i];
     tw[tid] = twg[tid];
     __syncthreads();
-    const int s = blockIdx.y * 4 + wave;
-    if (s >= S) return;
     float ax = 0.0f, ay = 0.0f;
     if (acc.base) {
         ax = acc.base[(size_t)p * acc.stride];
         ay = acc.base[(size_t)p * acc.stride + 1];
     }
+    for (int s = wave; s < S; s += 4) {
     const int2 si = shift_i[s];
     const float cx = (float)(D / 2 + si.x) + ax;
     const float cy = (float)(D / 2 + si.y) + ay;
@@ -211,6 +211,7 @@ __global__ __launch_bounds__(256) void k_polar_spectra(
         const double sd = sqrt(fabs(sQ / nn - mu * mu));
         norm[(size_t)p * S + s] = f2((float)(mu * kRingLen), (float)(1.0 / (sd + 1e-7)));
     }
+    }  // shifts
 }
 
 __global__ __launch_bounds__(128) void k_ref_finalize(float2 *__restrict__ rspec,
@@ -237,187 +238,289 @@ __global__ __launch_bounds__(128) void k_ref_finalize(float2 *__restrict__ rspec
 }
 
 // ------------------------------------------------------------------ stage B
-// One wavefront per work item.  Multi-reference: item = (particle, group of
-// NREF references); single-reference: item = particle.  The wavefront walks the
-// grid shifts four at a time: lane l accumulates, over rings, the four real
-// products P1..P4 of spectrum slots l and l+64 for 4 shifts x NREF references
-// (C0 = R conj S = (P1+P2, P4-P3), C1 = conj(R S) = (P1-P2, -(P3+P4))).  Each
-// (shift, reference) pair then becomes one 256-point inverse FFT of
-// Z = C0 + i C1 (Hermitian-extended), so z[t] = c0[t] + i c1[t] gives the psi
-// CCF of both mirrors at once.  The running first-max over (mirror, shift, psi)
-// is kept per reference and mirror, with the seven samples around it.
-template <int NREF, bool SINGLE>
-__global__ __launch_bounds__(256) void k_ccf_psi(
+// Psi stage of one (shift row, reference) pair.  Q / P hold the four real ring
+// products P1..P4 (sr*rr, si*ri, si*rr, sr*ri summed over rings with the ring
+// weight folded into the reference) of spectrum slots `lane` and `lane+64`:
+// C0 = R conj S = (P1+P2, P4-P3), C1 = conj(R S) = (P1-P2, -(P3+P4)).  The pair
+// becomes one 256-point inverse FFT of Z = C0 + i C1 (Hermitian-extended), so
+// z[t] = c0[t] + i c1[t] holds the psi CCF of both mirrors.  The running
+// first-max over (mirror, shift, psi) of this reference (bv, bf) and the seven
+// samples around it (nb, LDS) are updated; the 6-step wave argmax only runs
+// when some lane beats the running maximum (rows arrive in increasing shift
+// order, so an equal value can never displace the earlier maximum).
+template <bool SINGLE>
+__device__ __forceinline__ void psi_update(const float (&Q)[4], const float (&P)[4], float2 nm,
+                                           float rs0, int s, int lane, float2 *b0, float2 *b1,
+                                           const float2 *tw, float *zb, float (*nb)[8],
+                                           float (&bv)[2], int (&bf)[2]) {
+    const int pl = (64 - lane) & 63;
+    const float gs = nm.y;
+    float A = P[0] + P[1], B = P[3] - P[2], Cc = P[0] - P[1], Dd = -(P[2] + P[3]);
+    const float2 zhi = f2(gs * (A - Dd), gs * (B + Cc));
+    const float2 yhi = f2(gs * (A + Dd), gs * (Cc - B));
+    A = Q[0] + Q[1];
+    B = Q[3] - Q[2];
+    Cc = Q[0] - Q[1];
+    Dd = -(Q[2] + Q[3]);
+    float2 zlo = f2(gs * (A - Dd), gs * (B + Cc));
+    const float2 ylo = f2(gs * (A + Dd), gs * (Cc - B));
+    float2 nyq = f2(0.0f, 0.0f);
+    if (lane == 0) {  // packed slot 0: DC (with the mean correction) and Nyquist
+        const float c0 = gs * (Q[0] - nm.x * rs0);
+        const float cn = gs * Q[1];
+        zlo = f2(c0, c0);
+        nyq = f2(cn, cn);
+    }
+    const float2 syhi = shfl2(yhi, pl), sylo = shfl2(ylo, pl);
+    float2 v[4];
+    v[0] = zlo;
+    v[1] = zhi;
+    v[2] = lane ? syhi : nyq;
+    v[3] = lane ? sylo : yhi;
+    fft256_wave<true>(v, b0, b1, tw, lane);
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        float bvl = m ? v[0].y : v[0].x;
+        int bt = lane;
+#pragma unroll
+        for (int q = 1; q < 4; ++q) {
+            const float x = m ? v[q].y : v[q].x;
+            if (x > bvl) {
+                bvl = x;
+                bt = lane + 64 * q;
+            }
+        }
+        const bool sentinel = SINGLE && m == 0 && s == 0 && bf[0] == 0 && bv[0] == 0.0f;
+        if (!__any(bvl > bv[m]) && !sentinel) continue;
+        wave_argmax(bvl, bt);
+        const int flat = s * kRingLen + bt;
+        const bool upd = bvl > bv[m] || (bvl == bv[m] && flat < bf[m]);
+        if (upd || sentinel) {
+            const int tc = upd ? bt : 0;
+            if (upd) {
+                bv[m] = bvl;
+                bf[m] = flat;
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (int q = 0; q < 4; ++q) zb[lane + 64 * q] = m ? v[q].y : v[q].x;
+            wave_lds_sync();
+            if (lane < 7) nb[m][lane] = zb[(tc + lane - 3) & 255];
+            wave_lds_sync();
+        }
+    }
+}
+
+__device__ __forceinline__ void write_rec(PeakRec *o, const float (&bv)[2], const int (&bf)[2],
+                                          const float (*nb)[8], int lane) {
+    const int m = (bv[1] > bv[0]) ? 1 : 0;
+    const int f = bf[m];
+    if (lane == 0) {
+        o->val = bv[m];
+        o->m = m;
+        o->s = f / kRingLen;
+        o->t = f % kRingLen;
+        o->pad = 0;
+    }
+    if (lane < 7) o->x[lane] = nb[m][lane];
+}
+
+// Single reference per particle (pre-alignment / reference-free): one wavefront
+// per particle walks the shifts four at a time (4 rows x 1 reference).  The
+// search starts from value 0 at flat index 0, like the reference's row argmax
+// (cuda/gpu_aln_noref.cu:1638-1664).
+__global__ __launch_bounds__(256) void k_ccf_single(
     const float2 *__restrict__ spec, const float2 *__restrict__ norm,
     const float2 *__restrict__ rspec, const float *__restrict__ rsum0,
-    const int *__restrict__ ref_of, int ref_stride, int n_img, int S, int R, int N, int G,
+    const int *__restrict__ ref_of, int ref_stride, int n_img, int S, int R, int N,
     const float2 *__restrict__ twg, PeakRec *__restrict__ rec) {
     __shared__ float2 tw[256];
     __shared__ float2 fbuf[4][512];
     __shared__ float zbuf[4][256];
-    __shared__ float nbuf[4][NREF][2][8];
+    __shared__ float nbuf[4][2][8];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     tw[tid] = twg[tid];
     __syncthreads();
-    const int wid = xcd_remap(blockIdx.x, gridDim.x) * 4 + wave;
-    const int p = SINGLE ? wid : wid / G;
-    const int g = SINGLE ? 0 : wid % G;
+    const int p = blockIdx.x * 4 + wave;
     if (p >= n_img) return;
-
-    int nref[NREF];
-    bool nval[NREF];
-#pragma unroll
-    for (int c = 0; c < NREF; ++c) {
-        if (SINGLE) {  // the host validates ref ids; clamp keeps every read in bounds
-            nref[c] = min(max(ref_of[(size_t)p * ref_stride], 0), N - 1);
-            nval[c] = true;
-        } else {
-            const int n = g * NREF + c;
-            nval[c] = n < N;
-            nref[c] = nval[c] ? n : N - 1;
-        }
-    }
-    float bv[NREF][2];
-    int bf[NREF][2];
-    float rs0[NREF];
-#pragma unroll
-    for (int c = 0; c < NREF; ++c) {
-        rs0[c] = rsum0[nref[c]];
-#pragma unroll
-        for (int m = 0; m < 2; ++m) {
-            bv[c][m] = SINGLE ? 0.0f : -3.40282e+38f;
-            bf[c][m] = SINGLE ? 0 : INT_MAX;
-        }
-    }
-    float2 *b0 = fbuf[wave];
-    float2 *b1 = b0 + 256;
-    float *zb = zbuf[wave];
-    const int pl = (64 - lane) & 63;
+    // the host validates ref ids; the clamp keeps every read in bounds
+    const int n = min(max(ref_of[(size_t)p * ref_stride], 0), N - 1);
+    float bv[2] = {0.0f, 0.0f};
+    int bf[2] = {0, 0};
+    const float rs0 = rsum0[n];
+    const float2 *rb0 = rspec + (size_t)n * R * kSlots;
     const size_t img_stride = (size_t)S * R * kSlots;
-
     for (int s0 = 0; s0 < S; s0 += 4) {
-        float acc[4][NREF][2][4];
+        float acc[4][2][4];
 #pragma unroll
         for (int r = 0; r < 4; ++r)
 #pragma unroll
-            for (int c = 0; c < NREF; ++c)
+            for (int h = 0; h < 2; ++h)
 #pragma unroll
-                for (int h = 0; h < 2; ++h)
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) acc[r][c][h][e] = 0.0f;
+                for (int e = 0; e < 4; ++e) acc[r][h][e] = 0.0f;
         const float2 *arow[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r)
             arow[r] = spec + (size_t)p * img_stride + (size_t)min(s0 + r, S - 1) * R * kSlots;
         for (int i = 0; i < R; ++i) {
-            float2 a[4][2], b[NREF][2];
+            const float2 b0v = rb0[i * kSlots + lane], b1v = rb0[i * kSlots + lane + 64];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                a[r][0] = arow[r][i * kSlots + lane];
-                a[r][1] = arow[r][i * kSlots + lane + 64];
+                const float2 a0 = arow[r][i * kSlots + lane], a1 = arow[r][i * kSlots + lane + 64];
+                acc[r][0][0] += b0v.x * a0.x;
+                acc[r][0][1] += b0v.y * a0.y;
+                acc[r][0][2] += b0v.x * a0.y;
+                acc[r][0][3] += b0v.y * a0.x;
+                acc[r][1][0] += b1v.x * a1.x;
+                acc[r][1][1] += b1v.y * a1.y;
+                acc[r][1][2] += b1v.x * a1.y;
+                acc[r][1][3] += b1v.y * a1.x;
             }
-#pragma unroll
-            for (int c = 0; c < NREF; ++c) {
-                const float2 *rb = rspec + ((size_t)nref[c] * R + i) * kSlots;
-                b[c][0] = rb[lane];
-                b[c][1] = rb[lane + 64];
-            }
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-#pragma unroll
-                for (int c = 0; c < NREF; ++c)
-#pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        acc[r][c][h][0] += b[c][h].x * a[r][h].x;
-                        acc[r][c][h][1] += b[c][h].y * a[r][h].y;
-                        acc[r][c][h][2] += b[c][h].x * a[r][h].y;
-                        acc[r][c][h][3] += b[c][h].y * a[r][h].x;
-                    }
         }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int s = s0 + r;
+            if (s >= S) break;
+            psi_update<true>(acc[r][0], acc[r][1], norm[(size_t)p * S + s], rs0, s, lane,
+                             fbuf[wave], fbuf[wave] + 256, tw, zbuf[wave], nbuf[wave], bv, bf);
+        }
+    }
+    write_rec(rec + p, bv, bf, nbuf[wave], lane);
+}
+
+// Multi-reference: one workgroup = one particle x NW groups of 4 references (one
+// group per wavefront).  The workgroup walks the shifts four at a time; the four
+// shift rows' ring spectra are staged cooperatively in LDS in double-buffered
+// chunks of kRingChunk rings, so only the references' spectra come through the
+// vector L1.  Lane l accumulates, over rings, P1..P4 of slots l and l+64 for
+// 4 rows x 4 references (128 accumulators), then each (row, reference) pair goes
+// through psi_update.
+constexpr int kRingChunk = 4;
+
+template <int NW>
+__global__ __launch_bounds__(NW * 64) void k_ccf_mref(
+    const float2 *__restrict__ spec, const float2 *__restrict__ norm,
+    const float2 *__restrict__ rspec, const float *__restrict__ rsum0, int n_img, int S, int R,
+    int N, int G, int GB, const float2 *__restrict__ twg, PeakRec *__restrict__ rec) {
+    constexpr int NT = NW * 64;
+    constexpr int CH4 = 4 * kRingChunk * kSlots / 2;   // float4 per staged chunk
+    __shared__ float2 tw[256];
+    __shared__ float4 abuf4[2][CH4];                    // [buf][ring][row][slot] float2 pairs
+    __shared__ float2 fbuf[NW][512];
+    __shared__ float zbuf[NW][256];
+    __shared__ float nbuf[NW][4][2][8];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < 256; i += NT) tw[i] = twg[i];
+    __syncthreads();
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int p = lb / GB;
+    if (p >= n_img) return;  // uniform over the workgroup
+    const int g = (lb % GB) * NW + wave;
+    const bool active = g < G;
+    int nref[4];
+    bool nval[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int n = g * 4 + c;
+        nval[c] = active && n < N;
+        nref[c] = min(n, N - 1);
+    }
+    float bv[4][2];
+    int bf[4][2];
+    float rs0[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        rs0[c] = rsum0[nref[c]];
+        bv[c][0] = bv[c][1] = -3.40282e+38f;
+        bf[c][0] = bf[c][1] = INT_MAX;
+    }
+    const size_t img_stride = (size_t)S * R * kSlots;
+    const int nchunk = (R + kRingChunk - 1) / kRingChunk;
+    const float2 *abuf = reinterpret_cast<const float2 *>(&abuf4[0][0]);
+
+    for (int s0 = 0; s0 < S; s0 += 4) {
+        const float4 *rows4[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            rows4[r] = reinterpret_cast<const float4 *>(spec + (size_t)p * img_stride +
+                                                        (size_t)min(s0 + r, S - 1) * R * kSlots);
+        // Stage one chunk (4 rows x kRingChunk rings, 1 KB per row-ring) with direct
+        // global->LDS loads: one wavefront instruction moves one row-ring, each lane
+        // 16 bytes to LDS base + 16*lane.
+        auto load_chunk = [&](int ch, int buf) {
+            for (int rr = wave; rr < 4 * kRingChunk; rr += NW) {
+                const int ii = rr >> 2, r = rr & 3;
+                const int ring = ch * kRingChunk + ii;
+                if (ring >= R) continue;
+                __builtin_amdgcn_global_load_lds(
+                    (const void *)(rows4[r] + ring * (kSlots / 2) + lane),
+                    (__attribute__((address_space(3))) void *)(&abuf4[buf][(ii * 4 + r) * (kSlots / 2)]),
+                    16, 0, 0);
+            }
+        };
+        float acc[4][4][2][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) acc[r][c][h][e] = 0.0f;
+        load_chunk(0, 0);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        for (int ch = 0; ch < nchunk; ++ch) {
+            const int buf = ch & 1;
+            if (ch + 1 < nchunk) load_chunk(ch + 1, buf ^ 1);
+            const int i1 = min(R, (ch + 1) * kRingChunk);
+            for (int i = ch * kRingChunk; i < i1; ++i) {
+                const int ii = i - ch * kRingChunk;
+                float2 a[4][2], b[4][2];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const float2 *ar = abuf + (size_t)buf * (2 * CH4) + (ii * 4 + r) * kSlots;
+                    a[r][0] = ar[lane];
+                    a[r][1] = ar[lane + 64];
+                }
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const float2 *rb = rspec + ((size_t)nref[c] * R + i) * kSlots;
+                    b[c][0] = rb[lane];
+                    b[c][1] = rb[lane + 64];
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            acc[r][c][h][0] += b[c][h].x * a[r][h].x;
+                            acc[r][c][h][1] += b[c][h].y * a[r][h].y;
+                            acc[r][c][h][2] += b[c][h].x * a[r][h].y;
+                            acc[r][c][h][3] += b[c][h].y * a[r][h].x;
+                        }
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+        }
+        if (!active) continue;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int s = s0 + r;
             if (s >= S) break;
             const float2 nm = norm[(size_t)p * S + s];
 #pragma unroll
-            for (int c = 0; c < NREF; ++c) {
+            for (int c = 0; c < 4; ++c) {
                 if (!nval[c]) continue;
-                const float gs = nm.y;
-                // slot lane+64 (always a plain frequency)
-                const float *P = acc[r][c][1];
-                float A = P[0] + P[1], B = P[3] - P[2], Cc = P[0] - P[1], Dd = -(P[2] + P[3]);
-                const float2 zhi = f2(gs * (A - Dd), gs * (B + Cc));
-                const float2 yhi = f2(gs * (A + Dd), gs * (Cc - B));
-                const float *Q = acc[r][c][0];
-                A = Q[0] + Q[1];
-                B = Q[3] - Q[2];
-                Cc = Q[0] - Q[1];
-                Dd = -(Q[2] + Q[3]);
-                float2 zlo = f2(gs * (A - Dd), gs * (B + Cc));
-                const float2 ylo = f2(gs * (A + Dd), gs * (Cc - B));
-                float2 nyq = f2(0.0f, 0.0f);
-                if (lane == 0) {  // packed slot 0: DC (with the mean correction) and Nyquist
-                    const float c0 = gs * (Q[0] - nm.x * rs0[c]);
-                    const float cn = gs * Q[1];
-                    zlo = f2(c0, c0);
-                    nyq = f2(cn, cn);
-                }
-                const float2 syhi = shfl2(yhi, pl), sylo = shfl2(ylo, pl);
-                float2 v[4];
-                v[0] = zlo;
-                v[1] = zhi;
-                v[2] = lane ? syhi : nyq;
-                v[3] = lane ? sylo : yhi;
-                fft256_wave<true>(v, b0, b1, tw, lane);
-#pragma unroll
-                for (int m = 0; m < 2; ++m) {
-                    float bvl = m ? v[0].y : v[0].x;
-                    int bt = lane;
-#pragma unroll
-                    for (int q = 1; q < 4; ++q) {
-                        const float x = m ? v[q].y : v[q].x;
-                        if (x > bvl) {
-                            bvl = x;
-                            bt = lane + 64 * q;
-                        }
-                    }
-                    wave_argmax(bvl, bt);
-                    const int flat = s * kRingLen + bt;
-                    const bool upd = bvl > bv[c][m] || (bvl == bv[c][m] && flat < bf[c][m]);
-                    const bool sentinel = SINGLE && m == 0 && s == 0 && !upd &&
-                                          bf[c][0] == 0 && bv[c][0] == 0.0f;
-                    if (upd || sentinel) {
-                        const int tc = upd ? bt : 0;
-                        if (upd) {
-                            bv[c][m] = bvl;
-                            bf[c][m] = flat;
-                        }
-                        wave_lds_sync();
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) zb[lane + 64 * q] = m ? v[q].y : v[q].x;
-                        wave_lds_sync();
-                        if (lane < 7) nbuf[wave][c][m][lane] = zb[(tc + lane - 3) & 255];
-                        wave_lds_sync();
-                    }
-                }
+                psi_update<false>(acc[r][c][0], acc[r][c][1], nm, rs0[c], s, lane, fbuf[wave],
+                                  fbuf[wave] + 256, tw, zbuf[wave], nbuf[wave][c], bv[c], bf[c]);
             }
         }
     }
 #pragma unroll
-    for (int c = 0; c < NREF; ++c) {
-        if (!nval[c]) continue;
-        const int m = (bv[c][1] > bv[c][0]) ? 1 : 0;
-        PeakRec *o = SINGLE ? rec + p : rec + (size_t)p * N + nref[c];
-        const int f = bf[c][m];
-        if (lane == 0) {
-            o->val = bv[c][m];
-            o->m = m;
-            o->s = f / kRingLen;
-            o->t = f % kRingLen;
-            o->pad = 0;
-        }
-        if (lane < 7) o->x[lane] = nbuf[wave][c][m][lane];
-    }
+    for (int c = 0; c < 4; ++c)
+        if (nval[c]) write_rec(rec + (size_t)p * N + nref[c], bv[c], bf[c], nbuf[wave][c], lane);
 }
 
 // ------------------------------------------------------------------ decode
@@ -616,7 +719,7 @@ hipError_t launch_polar_spectra(const float *imgs, int n_img, int img_dim, Shift
         if (e != hipSuccess) return e;
         lds_raised = true;
     }
-    dim3 grid(n_img, (n_shift + 3) / 4);
+    dim3 grid(n_img);
     hipLaunchKernelGGL(k_polar_spectra, grid, dim3(256), lds, st, imgs, img_dim, acc, shift_i,
                        n_shift, tmpl, n_ring, twiddle, spec, norm);
     return hipGetLastError();
@@ -629,16 +732,26 @@ hipError_t launch_ref_finalize(float2 *rspec, const float2 *rnorm, int n_ref, in
     return hipGetLastError();
 }
 
+template <int NW>
+static hipError_t launch_mref_nw(const float2 *spec, const float2 *norm, const float2 *rspec,
+                                 const float *rsum0, int n_img, int S, int R, int N, int G,
+                                 const float2 *tw, PeakRec *rec, hipStream_t st) {
+    const int GB = (G + NW - 1) / NW;
+    const long long blocks = (long long)n_img * GB;
+    hipLaunchKernelGGL((k_ccf_mref<NW>), dim3((unsigned)blocks), dim3(NW * 64), 0, st, spec, norm,
+                       rspec, rsum0, n_img, S, R, N, G, GB, tw, rec);
+    return hipGetLastError();
+}
+
 hipError_t launch_ccf_mref(const float2 *spec, const float2 *norm, const float2 *rspec,
                            const float *rsum0, int n_img, int n_shift, int n_ring, int n_ref,
                            const float2 *twiddle, PeakRec *rec, hipStream_t st) {
     if (n_img <= 0 || n_ref <= 0) return hipSuccess;
+    // Four waves (16 references) per workgroup: at ~250 VGPRs two waves fit per
+    // SIMD, so two workgroups share a CU; measured faster than 2, 3, 5 or 8 waves.
     const int G = (n_ref + 3) / 4;
-    const long long items = (long long)n_img * G;
-    const int blocks = (int)((items + 3) / 4);
-    hipLaunchKernelGGL((k_ccf_psi<4, false>), dim3(blocks), dim3(256), 0, st, spec, norm, rspec,
-                       rsum0, (const int *)nullptr, 0, n_img, n_shift, n_ring, n_ref, G, twiddle, rec);
-    return hipGetLastError();
+    return launch_mref_nw<4>(spec, norm, rspec, rsum0, n_img, n_shift, n_ring, n_ref, G, twiddle,
+                             rec, st);
 }
 
 hipError_t launch_ccf_single(const float2 *spec, const float2 *norm, const float2 *rspec,
@@ -647,8 +760,8 @@ hipError_t launch_ccf_single(const float2 *spec, const float2 *norm, const float
                              PeakRec *rec, hipStream_t st) {
     if (n_img <= 0) return hipSuccess;
     const int blocks = (n_img + 3) / 4;
-    hipLaunchKernelGGL((k_ccf_psi<1, true>), dim3(blocks), dim3(256), 0, st, spec, norm, rspec,
-                       rsum0, ref_of, ref_stride, n_img, n_shift, n_ring, n_ref, 1, twiddle, rec);
+    hipLaunchKernelGGL(k_ccf_single, dim3(blocks), dim3(256), 0, st, spec, norm, rspec, rsum0,
+                       ref_of, ref_stride, n_img, n_shift, n_ring, n_ref, twiddle, rec);
     return hipGetLastError();
 }
 
