@@ -97,10 +97,12 @@ struct Workspace {
     DevBuf tmpl, twiddle, zero_shift, shifts_i, shifts_f;
     DevBuf spec, norm, rspec, rnorm, rsum0, rec;
     DevBuf peak, dvals, acc;
+    DevBuf mix_p, mix_pn, mix_r, mix_rn;   // 16-bit MFMA operands and row norms
     std::vector<float2> shifts_uploaded;
     void release() {
         DevBuf *all[] = {&tmpl, &twiddle, &zero_shift, &shifts_i, &shifts_f, &spec, &norm,
-                         &rspec, &rnorm, &rsum0, &rec, &peak, &dvals, &acc};
+                         &rspec, &rnorm, &rsum0, &rec, &peak, &dvals, &acc,
+                         &mix_p, &mix_pn, &mix_r, &mix_rn};
         for (DevBuf *b : all) b->release();
         tmpl_rings = 0;
         shifts_uploaded.clear();
@@ -312,7 +314,56 @@ int run_mref(const float *d_sbj, const float *d_ref, ShiftView acc, int P, int N
                                          ws.twiddle.as<float2>(), ws.rec.as<PeakRec>(), st));
             if (g.prof) { g.prof_ccf_ms += ps.finish(); g.prof_ccf_n++; }
         }
-        HIP_OK(isac::launch_decode_mref(ws.rec.as<PeakRec>(), n, N, a, ws.shifts_f.as<float2>(),
+        HIP_OK(isac::launch_decode_mref(ws.rec.as<PeakRec>(), n, N, a, ws.shifts_f.as<float2>(), S,
+                                        d_peak, peak_row, p0, d_d, d_pitch, st));
+    }
+    HIP_OK(hipStreamSynchronize(st));
+    return 0;
+}
+
+// Device-resident mixed-precision multi-reference pipeline: FP16/BF16 MFMA ring
+// contraction and FP16 MFMA psi transform; writes the approximate d / peak_list.
+int run_mref_mixed(const float *d_sbj, const float *d_ref, ShiftView acc, int P, int N, int D,
+                   int R, const std::vector<float2> &shifts, bool bf16, float *d_peak,
+                   int peak_row, float *d_d, int d_pitch, float mem_frac, hipStream_t st) {
+    const int S = (int)shifts.size();
+    if (int e = check_shape(D, R, kRingLen, shifts.size())) return e;
+    if (P <= 0 || N <= 0) return 0;
+    if (int e = ensure_template(R)) return e;
+    if (int e = ensure_shifts(shifts)) return e;
+    if (int e = ref_spectra(d_ref, N, D, R, st)) return e;
+    const int Q = (R + 3) / 4;
+    const size_t op_row = (size_t)Q * kSlots * 2 * 8;   // bytes of 16-bit operands per row
+    HIP_OK(ws.mix_r.reserve((size_t)N * op_row));
+    HIP_OK(ws.mix_rn.reserve((size_t)N * sizeof(float)));
+    HIP_OK(isac::launch_spec_to_mixed(ws.rspec.as<float2>(), nullptr, N, R, 1, bf16, ws.mix_r.p,
+                                      ws.mix_rn.as<float>(), st));
+    const size_t per = (size_t)S * R * kSlots * sizeof(float2) + (size_t)S * sizeof(float2) +
+                       (size_t)S * (op_row + sizeof(float)) + (size_t)N * sizeof(PeakRec);
+    const int pb = batch_for(per, 0, mem_frac, P);
+    HIP_OK(ws.spec.reserve((size_t)pb * S * R * kSlots * sizeof(float2)));
+    HIP_OK(ws.norm.reserve((size_t)pb * S * sizeof(float2)));
+    HIP_OK(ws.mix_p.reserve((size_t)pb * S * op_row));
+    HIP_OK(ws.mix_pn.reserve((size_t)pb * S * sizeof(float)));
+    HIP_OK(ws.rec.reserve((size_t)pb * N * sizeof(PeakRec)));
+    for (int p0 = 0; p0 < P; p0 += pb) {
+        const int n = std::min(pb, P - p0);
+        ShiftView a = acc;
+        if (a.base) a.base += (size_t)p0 * a.stride;
+        HIP_OK(isac::launch_polar_spectra(d_sbj + (size_t)p0 * D * D, n, D, a,
+                                          ws.shifts_i.as<int2>(), S, ws.tmpl.as<float2>(), R,
+                                          ws.twiddle.as<float2>(), ws.spec.as<float2>(),
+                                          ws.norm.as<float2>(), st));
+        HIP_OK(isac::launch_spec_to_mixed(ws.spec.as<float2>(), ws.norm.as<float2>(), n * S, R, 0,
+                                          bf16, ws.mix_p.p, ws.mix_pn.as<float>(), st));
+        {
+            ProfScope ps(g.prof, st);
+            HIP_OK(isac::launch_mref_mixed(ws.mix_p.p, ws.mix_pn.as<float>(), ws.mix_r.p,
+                                           ws.mix_rn.as<float>(), n, S, R, N, bf16,
+                                           ws.rec.as<PeakRec>(), st));
+            if (g.prof) { g.prof_ccf_ms += ps.finish(); g.prof_ccf_n++; }
+        }
+        HIP_OK(isac::launch_decode_mref(ws.rec.as<PeakRec>(), n, N, a, ws.shifts_f.as<float2>(), S,
                                         d_peak, peak_row, p0, d_d, d_pitch, st));
     }
     HIP_OK(hipStreamSynchronize(st));
@@ -353,7 +404,7 @@ int run_single(const float *d_sbj, DevParam *prm, const int *ref_of, int ref_str
                                        ref_of + (size_t)p0 * ref_stride, ref_stride, n, S, R, N,
                                        ws.twiddle.as<float2>(), ws.rec.as<PeakRec>(), st));
         HIP_OK(isac::launch_decode_single(ws.rec.as<PeakRec>(), n, prm + p0,
-                                          ws.shifts_f.as<float2>(), shift_limit, st));
+                                          ws.shifts_f.as<float2>(), S, shift_limit, st));
     }
     HIP_OK(hipStreamSynchronize(st));
     return 0;
@@ -601,6 +652,25 @@ int isac_mref_align_device(const float *d_sbj, const float *d_ref, const float *
     ShiftView acc{d_acc_xy, 2};
     return run_mref(d_sbj, d_ref, acc, (int)n_sbj, (int)n_ref, (int)img_dim, (int)ring_num, sh,
                     d_peak_list, 4 * (int)n_sbj, d_d, (int)d_pitch, 0.9f, st);
+}
+
+int isac_mref_align_mixed_device(const float *d_sbj, const float *d_ref, const float *d_acc_xy,
+                                 unsigned int n_sbj, unsigned int n_ref, unsigned int img_dim,
+                                 unsigned int ring_num, unsigned int ring_len, float shift_step,
+                                 float shift_rng_x, float shift_rng_y, int bf16, float *d_peak_list,
+                                 float *d_d, unsigned int d_pitch, void *stream) {
+    if (int e = ensure_device_ws(-1)) return e;
+    std::vector<float2> sh = make_shift_grid(shift_rng_x, shift_rng_y, shift_step);
+    if (int e = check_shape((int)img_dim, (int)ring_num, (int)ring_len, sh.size())) return e;
+    if (n_sbj == 0 || n_ref == 0) return 0;
+    if (!d_sbj || !d_ref || !d_peak_list || !d_d || d_pitch < n_sbj) {
+        set_error(kErrArg, "isac_mref_align_mixed_device: bad pointers or d_pitch < n_sbj");
+        return kErrArg;
+    }
+    hipStream_t st = stream ? (hipStream_t)stream : ws.stream;
+    ShiftView acc{d_acc_xy, 2};
+    return run_mref_mixed(d_sbj, d_ref, acc, (int)n_sbj, (int)n_ref, (int)img_dim, (int)ring_num,
+                          sh, bf16 != 0, d_peak_list, 4 * (int)n_sbj, d_d, (int)d_pitch, 0.9f, st);
 }
 
 void isac_profile_enable(int on) {

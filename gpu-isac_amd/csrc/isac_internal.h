@@ -70,12 +70,13 @@ hipError_t launch_ccf_single(const float2 *spec, const float2 *norm, const float
 
 // Multi-reference decode into the caller's d / peak_list layout.
 hipError_t launch_decode_mref(const PeakRec *rec, int n_img, int n_ref, ShiftView acc,
-                              const float2 *shifts_f, float *peak, int peak_row, int p_off,
-                              float *d, int d_pitch, hipStream_t st);
+                              const float2 *shifts_f, int n_shift, float *peak, int peak_row,
+                              int p_off, float *d, int d_pitch, hipStream_t st);
 
 // Pre-align / reference-free decode: update AlignParam records in place.
 hipError_t launch_decode_single(const PeakRec *rec, int cnt, DevParam *prm,
-                                const float2 *shifts_f, float shift_limit, hipStream_t st);
+                                const float2 *shifts_f, int n_shift, float shift_limit,
+                                hipStream_t st);
 
 // Apply found parameters to images: out[i] = T_i(imgs[i]).
 hipError_t launch_transform(const float *imgs, int n, int img_dim, const DevParam *prm,
@@ -89,5 +90,15 @@ hipError_t launch_class_average(const float *tr, int img_dim, const int *cid_idx
 // tw_d holds img_dim twiddles e^{-2 pi i m / img_dim}.
 hipError_t launch_tanl_filter(float *refs, int n, int img_dim, float cutoff, float falloff,
                               float2 *work, const float2 *tw_d, hipStream_t st);
+
+// Mixed-precision (MFMA) multi-reference search: FP32 spectra -> 16-bit MFMA
+// operands [row][ceil(R/4)][kSlots][re,im][4 rings] (+ Frobenius norm per row).
+hipError_t launch_spec_to_mixed(const float2 *spec, const float2 *norm, int n_rows, int n_ring,
+                                int is_ref, bool bf16, void *out, float *row_norm, hipStream_t st);
+
+// Approximate peaks of every (particle, reference) pair, rec[p*n_ref + n].
+hipError_t launch_mref_mixed(const void *pop, const float *pnorm, const void *rop,
+                             const float *rnorm, int n_img, int n_shift, int n_ring, int n_ref,
+                             bool bf16, PeakRec *rec, hipStream_t st);
 
 }  // namespace isac

@@ -314,7 +314,7 @@ __device__ __forceinline__ void psi_update(const float (&Q)[4], const float (&P)
 __device__ __forceinline__ void write_rec(PeakRec *o, const float (&bv)[2], const int (&bf)[2],
                                           const float (*nb)[8], int lane) {
     const int m = (bv[1] > bv[0]) ? 1 : 0;
-    const int f = bf[m];
+    const int f = (bf[m] == INT_MAX) ? 0 : bf[m];   // never updated (all values NaN)
     if (lane == 0) {
         o->val = bv[m];
         o->m = m;
@@ -527,7 +527,7 @@ __global__ __launch_bounds__(NW * 64) void k_ccf_mref(
 // Multi-reference decode (reference wizard, cuda/gpu_aln_noref.cu:1758-1793, with
 // its float/double mix) and output scatter (:517-540).
 __global__ void k_decode_mref(const PeakRec *__restrict__ rec, int n_img, int N, ShiftView acc,
-                              const float2 *__restrict__ shifts_f, float *__restrict__ peak,
+                              const float2 *__restrict__ shifts_f, int n_shift, float *__restrict__ peak,
                               int peak_row, int p_off, float *__restrict__ d, int d_pitch) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n_img * N) return;
@@ -548,7 +548,7 @@ __global__ void k_decode_mref(const PeakRec *__restrict__ rec, int n_img, int N,
         ax = acc.base[(size_t)p * acc.stride];
         ay = acc.base[(size_t)p * acc.stride + 1];
     }
-    const float2 sh = shifts_f[r.s];
+    const float2 sh = shifts_f[min(max(r.s, 0), n_shift - 1)];
     const size_t pp = (size_t)p_off + p;
     d[(size_t)n * d_pitch + pp] = r.val;
     float *pk = peak + (size_t)n * peak_row + 4 * pp;
@@ -561,7 +561,7 @@ __global__ void k_decode_mref(const PeakRec *__restrict__ rec, int n_img, int N,
 // Pre-align / reference-free decode (cuda/gpu_aln_noref.cu:2486-2519, FP64
 // parabola :2558-2581).
 __global__ void k_decode_single(const PeakRec *__restrict__ rec, int cnt, DevParam *__restrict__ prm,
-                                const float2 *__restrict__ shifts_f, float lim) {
+                                const float2 *__restrict__ shifts_f, int n_shift, float lim) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= cnt) return;
     const PeakRec r = rec[i];
@@ -575,7 +575,7 @@ __global__ void k_decode_single(const PeakRec *__restrict__ rec, int cnt, DevPar
     double interp = step * (double)r.t;
     if (c3 != 0.0) interp = interp + step * (c2 / (2.0 * c3) - 4.0);
     q.mirror = (r.m == 1);
-    const float2 sh = shifts_f[r.s];
+    const float2 sh = shifts_f[min(max(r.s, 0), n_shift - 1)];
     q.shift_x += sh.x;
     q.shift_y += sh.y;
     q.shift_x = fminf(fmaxf(q.shift_x, -lim), lim);
@@ -766,20 +766,21 @@ hipError_t launch_ccf_single(const float2 *spec, const float2 *norm, const float
 }
 
 hipError_t launch_decode_mref(const PeakRec *rec, int n_img, int n_ref, ShiftView acc,
-                              const float2 *shifts_f, float *peak, int peak_row, int p_off,
-                              float *d, int d_pitch, hipStream_t st) {
+                              const float2 *shifts_f, int n_shift, float *peak, int peak_row,
+                              int p_off, float *d, int d_pitch, hipStream_t st) {
     const long long n = (long long)n_img * n_ref;
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_decode_mref, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, rec,
-                       n_img, n_ref, acc, shifts_f, peak, peak_row, p_off, d, d_pitch);
+                       n_img, n_ref, acc, shifts_f, n_shift, peak, peak_row, p_off, d, d_pitch);
     return hipGetLastError();
 }
 
 hipError_t launch_decode_single(const PeakRec *rec, int cnt, DevParam *prm,
-                                const float2 *shifts_f, float shift_limit, hipStream_t st) {
+                                const float2 *shifts_f, int n_shift, float shift_limit,
+                                hipStream_t st) {
     if (cnt <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_decode_single, dim3((cnt + 255) / 256), dim3(256), 0, st, rec, cnt, prm,
-                       shifts_f, shift_limit);
+                       shifts_f, n_shift, shift_limit);
     return hipGetLastError();
 }
 

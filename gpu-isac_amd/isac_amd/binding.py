@@ -31,7 +31,7 @@ REFERENCE_SYMBOLS = (
 )
 EXTRA_SYMBOLS = (
     "gpu_last_error", "gpu_last_error_string", "gpu_clear_error", "isac_mref_align_device",
-    "isac_profile_enable", "isac_profile_read",
+    "isac_mref_align_mixed_device", "isac_profile_enable", "isac_profile_read",
 )
 
 
@@ -112,6 +112,9 @@ def load(path: Optional[str] = None):
     lib.isac_mref_align_device.argtypes = [vp, vp, vp, c_uint, c_uint, c_uint, c_uint, c_uint,
                                            c_float, c_float, c_float, vp, vp, c_uint, vp]
     lib.isac_mref_align_device.restype = c_int
+    lib.isac_mref_align_mixed_device.argtypes = [vp, vp, vp, c_uint, c_uint, c_uint, c_uint, c_uint,
+                                                 c_float, c_float, c_float, c_int, vp, vp, c_uint, vp]
+    lib.isac_mref_align_mixed_device.restype = c_int
     lib.isac_profile_enable.argtypes = [c_int]
     lib.isac_profile_enable.restype = None
     lib.isac_profile_read.argtypes = [ctypes.POINTER(c_int), ctypes.POINTER(ctypes.c_double),
@@ -325,4 +328,33 @@ def mref_align_device(sbj, ref, ring_num: int, shift_step: float, shift_rng_x: f
     if rc:
         check(lib)
         raise GpuAlignError(f"isac_mref_align_device returned {rc}")
+    return peak, d
+
+
+def mref_align_mixed_device(sbj, ref, ring_num: int, shift_step: float, shift_rng_x: float,
+                            shift_rng_y: float, precision: str = "fp16", acc_xy=None,
+                            stream=None, out=None):
+    """Mixed-precision (MFMA) multi-reference alignment on torch tensors.
+
+    precision "fp16" or "bf16" selects the ring-contraction operand type; the
+    psi transform runs on FP16 MFMA.  Returns approximate (peak_list, d)."""
+    import torch
+    lib = load()
+    P, D = sbj.shape[0], sbj.shape[-1]
+    N = ref.shape[0]
+    assert sbj.is_cuda and ref.is_cuda and sbj.is_contiguous() and ref.is_contiguous()
+    if out is None:
+        peak = torch.empty((N, 4 * P), dtype=torch.float32, device=sbj.device)
+        d = torch.empty((N, P), dtype=torch.float32, device=sbj.device)
+    else:
+        peak, d = out
+    acc_ptr = acc_xy.data_ptr() if acc_xy is not None else None
+    st = (stream or torch.cuda.current_stream()).cuda_stream
+    rc = lib.isac_mref_align_mixed_device(sbj.data_ptr(), ref.data_ptr(), acc_ptr, P, N, D, ring_num,
+                                          RING_LEN, shift_step, shift_rng_x, shift_rng_y,
+                                          1 if precision == "bf16" else 0, peak.data_ptr(),
+                                          d.data_ptr(), P, st)
+    if rc:
+        check(lib)
+        raise GpuAlignError(f"isac_mref_align_mixed_device returned {rc}")
     return peak, d

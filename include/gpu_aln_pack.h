@@ -158,6 +158,18 @@ int isac_mref_align_device(const float *d_sbj, const float *d_ref, const float *
                            float *d_peak_list, float *d_d, unsigned int d_pitch,
                            void *stream);
 
+/* No counterpart in the reference (mixed-precision form of
+ * isac_mref_align_device): the ring contraction runs on FP16 (bf16 = 0) or BF16
+ * (bf16 = 1) MFMA with FP32 accumulation and the psi transform on FP16 MFMA.
+ * Writes approximate d / peak_list for every pair (same layout as
+ * isac_mref_align_device); FP32 re-scoring of candidates is separate. */
+int isac_mref_align_mixed_device(const float *d_sbj, const float *d_ref, const float *d_acc_xy,
+                                 unsigned int n_sbj, unsigned int n_ref, unsigned int img_dim,
+                                 unsigned int ring_num, unsigned int ring_len,
+                                 float shift_step, float shift_rng_x, float shift_rng_y, int bf16,
+                                 float *d_peak_list, float *d_d, unsigned int d_pitch,
+                                 void *stream);
+
 /* No counterpart in the reference.  When enabled, the library brackets every
  * launch of its contraction+psi kernel with HIP events on the stream it
  * launches on, and accumulates launch count and total milliseconds. */
