@@ -98,11 +98,13 @@ struct Workspace {
     DevBuf spec, norm, rspec, rnorm, rsum0, rec;
     DevBuf peak, dvals, acc;
     DevBuf mix_p, mix_pn, mix_r, mix_rn;   // 16-bit MFMA operands and row norms
+    DevBuf cand, rec_c, best, second, tie, val64;   // candidate re-scoring
     std::vector<float2> shifts_uploaded;
     void release() {
         DevBuf *all[] = {&tmpl, &twiddle, &zero_shift, &shifts_i, &shifts_f, &spec, &norm,
                          &rspec, &rnorm, &rsum0, &rec, &peak, &dvals, &acc,
-                         &mix_p, &mix_pn, &mix_r, &mix_rn};
+                         &mix_p, &mix_pn, &mix_r, &mix_rn, &cand, &rec_c, &best, &second,
+                         &tie, &val64};
         for (DevBuf *b : all) b->release();
         tmpl_rings = 0;
         shifts_uploaded.clear();
@@ -323,9 +325,20 @@ int run_mref(const float *d_sbj, const float *d_ref, ShiftView acc, int P, int N
 
 // Device-resident mixed-precision multi-reference pipeline: FP16/BF16 MFMA ring
 // contraction and FP16 MFMA psi transform; writes the approximate d / peak_list.
+struct BestOut {
+    int *ref = nullptr;      // [P] best reference (exact FP32 re-scoring of the top candidates)
+    float *val = nullptr;    // [P] its peak value
+    float *peak = nullptr;   // [P][4] angle, sx, sy, mirror
+    int *tie = nullptr;      // [P] 1 when the runner-up was within tie_rel (re-scored in FP64)
+    float tie_rel = 1e-5f;
+};
+
+constexpr int kCandidates = 8;
+
 int run_mref_mixed(const float *d_sbj, const float *d_ref, ShiftView acc, int P, int N, int D,
                    int R, const std::vector<float2> &shifts, bool bf16, float *d_peak,
-                   int peak_row, float *d_d, int d_pitch, float mem_frac, hipStream_t st) {
+                   int peak_row, float *d_d, int d_pitch, float mem_frac, hipStream_t st,
+                   const BestOut *bo = nullptr) {
     const int S = (int)shifts.size();
     if (int e = check_shape(D, R, kRingLen, shifts.size())) return e;
     if (P <= 0 || N <= 0) return 0;
@@ -339,8 +352,17 @@ int run_mref_mixed(const float *d_sbj, const float *d_ref, ShiftView acc, int P,
     HIP_OK(isac::launch_spec_to_mixed(ws.rspec.as<float2>(), nullptr, N, R, 1, bf16, ws.mix_r.p,
                                       ws.mix_rn.as<float>(), st));
     const size_t per = (size_t)S * R * kSlots * sizeof(float2) + (size_t)S * sizeof(float2) +
-                       (size_t)S * (op_row + sizeof(float)) + (size_t)N * sizeof(PeakRec);
+                       (size_t)S * (op_row + sizeof(float)) + (size_t)N * sizeof(PeakRec) +
+                       kCandidates * (sizeof(PeakRec) + sizeof(int)) + 16 * sizeof(int);
     const int pb = batch_for(per, 0, mem_frac, P);
+    if (bo) {
+        HIP_OK(ws.cand.reserve((size_t)pb * kCandidates * sizeof(int)));
+        HIP_OK(ws.rec_c.reserve((size_t)pb * kCandidates * sizeof(PeakRec)));
+        HIP_OK(ws.best.reserve((size_t)pb * sizeof(int)));
+        HIP_OK(ws.second.reserve((size_t)pb * sizeof(int)));
+        HIP_OK(ws.tie.reserve((size_t)pb * sizeof(int)));
+        HIP_OK(ws.val64.reserve((size_t)pb * 2 * sizeof(double)));
+    }
     HIP_OK(ws.spec.reserve((size_t)pb * S * R * kSlots * sizeof(float2)));
     HIP_OK(ws.norm.reserve((size_t)pb * S * sizeof(float2)));
     HIP_OK(ws.mix_p.reserve((size_t)pb * S * op_row));
@@ -365,6 +387,26 @@ int run_mref_mixed(const float *d_sbj, const float *d_ref, ShiftView acc, int P,
         }
         HIP_OK(isac::launch_decode_mref(ws.rec.as<PeakRec>(), n, N, a, ws.shifts_f.as<float2>(), S,
                                         d_peak, peak_row, p0, d_d, d_pitch, st));
+        if (bo) {
+            const int K = kCandidates;
+            HIP_OK(isac::launch_topk(ws.rec.as<PeakRec>(), n, N, K, ws.cand.as<int>(), st));
+            HIP_OK(isac::launch_ccf_candidates(ws.spec.as<float2>(), ws.norm.as<float2>(),
+                                               ws.rspec.as<float2>(), ws.rsum0.as<float>(),
+                                               ws.cand.as<int>(), K, n, S, R, N,
+                                               ws.twiddle.as<float2>(), ws.rec_c.as<PeakRec>(), st));
+            HIP_OK(isac::launch_best_candidate(ws.rec_c.as<PeakRec>(), ws.cand.as<int>(), n, K,
+                                               bo->tie_rel, ws.best.as<int>(), ws.second.as<int>(),
+                                               ws.tie.as<int>(), st));
+            HIP_OK(isac::launch_rescore_f64(ws.spec.as<float2>(), ws.norm.as<float2>(),
+                                            ws.rspec.as<float2>(), ws.rec_c.as<PeakRec>(),
+                                            ws.cand.as<int>(), ws.best.as<int>(),
+                                            ws.second.as<int>(), ws.tie.as<int>(), n, K, S, R, N,
+                                            ws.val64.as<double>(), st));
+            HIP_OK(isac::launch_decode_best(ws.rec_c.as<PeakRec>(), ws.cand.as<int>(),
+                                            ws.best.as<int>(), ws.tie.as<int>(), n, K, a,
+                                            ws.shifts_f.as<float2>(), S, p0, bo->ref, bo->val,
+                                            bo->peak, bo->tie, st));
+        }
     }
     HIP_OK(hipStreamSynchronize(st));
     return 0;
@@ -671,6 +713,34 @@ int isac_mref_align_mixed_device(const float *d_sbj, const float *d_ref, const f
     ShiftView acc{d_acc_xy, 2};
     return run_mref_mixed(d_sbj, d_ref, acc, (int)n_sbj, (int)n_ref, (int)img_dim, (int)ring_num,
                           sh, bf16 != 0, d_peak_list, 4 * (int)n_sbj, d_d, (int)d_pitch, 0.9f, st);
+}
+
+int isac_mref_best_mixed_device(const float *d_sbj, const float *d_ref, const float *d_acc_xy,
+                                unsigned int n_sbj, unsigned int n_ref, unsigned int img_dim,
+                                unsigned int ring_num, unsigned int ring_len, float shift_step,
+                                float shift_rng_x, float shift_rng_y, int bf16, float tie_rel,
+                                float *d_peak_list, float *d_d, int *d_best_ref, float *d_best_val,
+                                float *d_best_peak, int *d_tie, void *stream) {
+    if (int e = ensure_device_ws(-1)) return e;
+    std::vector<float2> sh = make_shift_grid(shift_rng_x, shift_rng_y, shift_step);
+    if (int e = check_shape((int)img_dim, (int)ring_num, (int)ring_len, sh.size())) return e;
+    if (n_sbj == 0 || n_ref == 0) return 0;
+    if (!d_sbj || !d_ref || !d_peak_list || !d_d || !d_best_ref || !d_best_val || !d_best_peak ||
+        !d_tie) {
+        set_error(kErrArg, "isac_mref_best_mixed_device: NULL pointer");
+        return kErrArg;
+    }
+    hipStream_t st = stream ? (hipStream_t)stream : ws.stream;
+    ShiftView acc{d_acc_xy, 2};
+    BestOut bo;
+    bo.ref = d_best_ref;
+    bo.val = d_best_val;
+    bo.peak = d_best_peak;
+    bo.tie = d_tie;
+    bo.tie_rel = tie_rel;
+    return run_mref_mixed(d_sbj, d_ref, acc, (int)n_sbj, (int)n_ref, (int)img_dim, (int)ring_num,
+                          sh, bf16 != 0, d_peak_list, 4 * (int)n_sbj, d_d, (int)n_sbj, 0.9f, st,
+                          &bo);
 }
 
 void isac_profile_enable(int on) {

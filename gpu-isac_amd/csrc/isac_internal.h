@@ -68,6 +68,32 @@ hipError_t launch_ccf_single(const float2 *spec, const float2 *norm, const float
                              int n_shift, int n_ring, int n_ref, const float2 *twiddle,
                              PeakRec *rec, hipStream_t st);
 
+// Exact FP32 re-scoring of K candidate references per particle (ref cand[p*K+k]),
+// same arithmetic as launch_ccf_mref; rec[p*K + k].
+hipError_t launch_ccf_candidates(const float2 *spec, const float2 *norm, const float2 *rspec,
+                                 const float *rsum0, const int *cand, int K, int n_img,
+                                 int n_shift, int n_ring, int n_ref, const float2 *twiddle,
+                                 PeakRec *rec, hipStream_t st);
+
+// Top-K (K = 8) references per particle by approximate peak value.
+hipError_t launch_topk(const PeakRec *rec, int n_img, int n_ref, int K, int *cand, hipStream_t st);
+
+// Best / runner-up exact candidate per particle and near-tie flags.
+hipError_t launch_best_candidate(const PeakRec *rec, const int *cand, int n_img, int K,
+                                 float tie_rel, int *best, int *second, int *tie, hipStream_t st);
+
+// FP64 re-scoring of flagged near-ties; may swap best and runner-up.
+hipError_t launch_rescore_f64(const float2 *spec, const float2 *norm, const float2 *rspec,
+                              const PeakRec *rec, const int *cand, int *best, int *second,
+                              const int *tie, int n_img, int K, int n_shift, int n_ring,
+                              int n_ref, double *val64, hipStream_t st);
+
+// Per-particle best candidate -> outputs at particle p_off + p.
+hipError_t launch_decode_best(const PeakRec *rec, const int *cand, const int *best,
+                              const int *tie, int n_img, int K, ShiftView acc,
+                              const float2 *shifts_f, int n_shift, int p_off, int *best_ref,
+                              float *best_val, float *best_peak, int *tie_out, hipStream_t st);
+
 // Multi-reference decode into the caller's d / peak_list layout.
 hipError_t launch_decode_mref(const PeakRec *rec, int n_img, int n_ref, ShiftView acc,
                               const float2 *shifts_f, int n_shift, float *peak, int peak_row,

@@ -16,6 +16,7 @@
 
 #include <cfloat>
 #include <climits>
+#include <cmath>
 
 namespace isac {
 namespace {
@@ -325,14 +326,18 @@ __device__ __forceinline__ void write_rec(PeakRec *o, const float (&bv)[2], cons
     if (lane < 7) o->x[lane] = nb[m][lane];
 }
 
-// Single reference per particle (pre-alignment / reference-free): one wavefront
-// per particle walks the shifts four at a time (4 rows x 1 reference).  The
-// search starts from value 0 at flat index 0, like the reference's row argmax
-// (cuda/gpu_aln_noref.cu:1638-1664).
-__global__ __launch_bounds__(256) void k_ccf_single(
+// Items (particle, reference): one wavefront per item walks the shifts four at a
+// time (4 rows x 1 reference).  Item i uses particle i / K and reference
+// ref_of[i * ref_stride].  ZERO_INIT: the search starts from value 0 at flat index
+// 0 like the reference's row argmax for pre-alignment / reference-free
+// (cuda/gpu_aln_noref.cu:1638-1664); otherwise from -FLT_MAX like the
+// multi-reference wizard, giving exactly the FP32 multi-reference kernel's
+// arithmetic for that pair (used to re-score candidates).
+template <bool ZERO_INIT>
+__global__ __launch_bounds__(256) void k_ccf_items(
     const float2 *__restrict__ spec, const float2 *__restrict__ norm,
     const float2 *__restrict__ rspec, const float *__restrict__ rsum0,
-    const int *__restrict__ ref_of, int ref_stride, int n_img, int S, int R, int N,
+    const int *__restrict__ ref_of, int ref_stride, int K, int n_items, int S, int R, int N,
     const float2 *__restrict__ twg, PeakRec *__restrict__ rec) {
     __shared__ float2 tw[256];
     __shared__ float2 fbuf[4][512];
@@ -341,12 +346,15 @@ __global__ __launch_bounds__(256) void k_ccf_single(
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     tw[tid] = twg[tid];
     __syncthreads();
-    const int p = blockIdx.x * 4 + wave;
-    if (p >= n_img) return;
+    const int item = blockIdx.x * 4 + wave;
+    if (item >= n_items) return;
+    const int p = item / K;
     // the host validates ref ids; the clamp keeps every read in bounds
-    const int n = min(max(ref_of[(size_t)p * ref_stride], 0), N - 1);
-    float bv[2] = {0.0f, 0.0f};
-    int bf[2] = {0, 0};
+    const int n = min(max(ref_of[(size_t)item * ref_stride], 0), N - 1);
+    float bv[2];
+    int bf[2];
+    bv[0] = bv[1] = ZERO_INIT ? 0.0f : -3.40282e+38f;
+    bf[0] = bf[1] = ZERO_INIT ? 0 : INT_MAX;
     const float rs0 = rsum0[n];
     const float2 *rb0 = rspec + (size_t)n * R * kSlots;
     const size_t img_stride = (size_t)S * R * kSlots;
@@ -381,11 +389,166 @@ __global__ __launch_bounds__(256) void k_ccf_single(
         for (int r = 0; r < 4; ++r) {
             const int s = s0 + r;
             if (s >= S) break;
-            psi_update<true>(acc[r][0], acc[r][1], norm[(size_t)p * S + s], rs0, s, lane,
-                             fbuf[wave], fbuf[wave] + 256, tw, zbuf[wave], nbuf[wave], bv, bf);
+            psi_update<ZERO_INIT>(acc[r][0], acc[r][1], norm[(size_t)p * S + s], rs0, s, lane,
+                                  fbuf[wave], fbuf[wave] + 256, tw, zbuf[wave], nbuf[wave], bv, bf);
         }
     }
-    write_rec(rec + p, bv, bf, nbuf[wave], lane);
+    write_rec(rec + item, bv, bf, nbuf[wave], lane);
+}
+
+// Top-K references of every particle by approximate peak value (descending; ties
+// to the lower reference index).  One thread per particle.
+template <int K>
+__global__ void k_topk(const PeakRec *__restrict__ rec, int n_img, int N, int *__restrict__ cand) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_img) return;
+    float v[K];
+    int id[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        v[k] = -INFINITY;
+        id[k] = 0;
+    }
+    for (int n = 0; n < N; ++n) {
+        const float x = rec[(size_t)p * N + n].val;
+        if (!(x > v[K - 1])) continue;
+        float cv = x;
+        int ci = n;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (cv > v[k]) {
+                const float tv = v[k];
+                const int ti = id[k];
+                v[k] = cv;
+                id[k] = ci;
+                cv = tv;
+                ci = ti;
+            }
+        }
+    }
+    const int kk = min(K, N);
+#pragma unroll
+    for (int k = 0; k < K; ++k) cand[(size_t)p * K + k] = (k < kk) ? id[k] : id[0];
+}
+
+// Best exact candidate per particle: highest FP32 peak (ties to the lower
+// reference index); flags a near-tie when the best distinct candidate is within
+// tie_rel of it.  best[p] = candidate slot, second[p] = runner-up slot (-1 none).
+__global__ void k_best_candidate(const PeakRec *__restrict__ rec, const int *__restrict__ cand,
+                                 int n_img, int K, float tie_rel, int *__restrict__ best,
+                                 int *__restrict__ second, int *__restrict__ tie) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_img) return;
+    int b = -1, s2 = -1;
+    for (int k = 0; k < K; ++k) {
+        const int n = cand[(size_t)p * K + k];
+        bool dup = false;
+        for (int j = 0; j < k; ++j) dup |= cand[(size_t)p * K + j] == n;
+        if (dup) continue;
+        const float x = rec[(size_t)p * K + k].val;
+        if (b < 0) {
+            b = k;
+            continue;
+        }
+        const float xb = rec[(size_t)p * K + b].val;
+        const int nb = cand[(size_t)p * K + b];
+        if (x > xb || (x == xb && n < nb)) {
+            s2 = b;
+            b = k;
+        } else if (s2 < 0) {
+            s2 = k;
+        } else {
+            const float x2 = rec[(size_t)p * K + s2].val;
+            const int n2 = cand[(size_t)p * K + s2];
+            if (x > x2 || (x == x2 && n < n2)) s2 = k;
+        }
+    }
+    best[p] = b;
+    second[p] = s2;
+    int t = 0;
+    if (s2 >= 0) {
+        const float xb = rec[(size_t)p * K + b].val, x2 = rec[(size_t)p * K + s2].val;
+        t = (xb - x2) <= tie_rel * fabsf(xb);
+    }
+    tie[p] = t;
+}
+
+// Near-tie re-scoring in FP64: for a flagged particle, one wavefront per candidate
+// (best, runner-up) recomputes C0/C1 over rings in double from the FP32 spectra
+// and evaluates the psi CCF of its winning mirror / shift at t*-1, t*, t*+1 by
+// direct summation: c[t] = C[0] + C[128] (-1)^t + 2 sum_k Re(C[k] e^{2 pi i k t/256}).
+// val64[2p + j] = the largest of the three (j = 0 best, 1 runner-up).
+__global__ __launch_bounds__(256) void k_rescore_f64(
+    const float2 *__restrict__ spec, const float2 *__restrict__ norm,
+    const float2 *__restrict__ rspec, const PeakRec *__restrict__ rec,
+    const int *__restrict__ cand, const int *__restrict__ best, const int *__restrict__ second,
+    const int *__restrict__ tie, int n_img, int K, int S, int R, int N, double *__restrict__ val64) {
+    const int lane = threadIdx.x & 63;
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int p = w >> 1, j = w & 1;
+    if (p >= n_img || !tie[p]) return;
+    const int slot_k = j ? second[p] : best[p];
+    if (slot_k < 0) return;
+    const PeakRec r = rec[(size_t)p * K + slot_k];
+    const int n = min(max(cand[(size_t)p * K + slot_k], 0), N - 1);
+    const int s = min(max(r.s, 0), S - 1);
+    const float2 nm = norm[(size_t)p * S + s];
+    const double muL = nm.x, g = nm.y;
+    double P[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    const float2 *a = spec + ((size_t)p * S + s) * R * kSlots;
+    const float2 *b = rspec + (size_t)n * R * kSlots;
+    for (int i = 0; i < R; ++i) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int k = lane + 64 * h;
+            const float2 av = a[i * kSlots + k], bv = b[i * kSlots + k];
+            const double x = ((k == 0) ? ((double)av.x - muL) : (double)av.x) * g;
+            const double y = (double)av.y * g;
+            P[h][0] += (double)bv.x * x;
+            P[h][1] += (double)bv.y * y;
+            P[h][2] += (double)bv.x * y;
+            P[h][3] += (double)bv.y * x;
+        }
+    }
+    double best_v = -INFINITY;
+    for (int dt = -1; dt <= 1; ++dt) {
+        const int t = (r.t + dt + 256) & 255;
+        double sum = 0.0;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int k = lane + 64 * h;
+            const double c0r = P[h][0] + P[h][1], c0i = P[h][3] - P[h][2];
+            const double c1r = P[h][0] - P[h][1], c1i = -(P[h][2] + P[h][3]);
+            const double cr = r.m ? c1r : c0r, ci = r.m ? c1i : c0i;
+            if (k == 0) {  // DC and Nyquist (slot 0 packs both)
+                sum += P[h][0] + ((t & 1) ? -P[h][1] : P[h][1]);
+            } else {
+                double sn, cs;
+                sincospi((double)(2 * k * t % 512) / 256.0, &sn, &cs);
+                sum += 2.0 * (cr * cs - ci * sn);
+            }
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) sum += __shfl_xor(sum, off, 64);
+        best_v = fmax(best_v, sum);
+    }
+    if (lane == 0) val64[2 * p + j] = best_v;
+}
+
+// Swap best and runner-up of flagged particles when FP64 ranks them the other way
+// (equal values: the lower reference index wins).
+__global__ void k_resolve_ties(const int *__restrict__ cand, const double *__restrict__ val64,
+                               const int *__restrict__ tie, int n_img, int K, int *__restrict__ best,
+                               int *__restrict__ second) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_img || !tie[p] || second[p] < 0) return;
+    const double vb = val64[2 * p], vs = val64[2 * p + 1];
+    const int nb = cand[(size_t)p * K + best[p]], ns = cand[(size_t)p * K + second[p]];
+    if (vs > vb || (vs == vb && ns < nb)) {
+        const int t = best[p];
+        best[p] = second[p];
+        second[p] = t;
+    }
 }
 
 // Multi-reference: one workgroup = one particle x NW groups of 4 references (one
@@ -524,15 +687,10 @@ __global__ __launch_bounds__(NW * 64) void k_ccf_mref(
 }
 
 // ------------------------------------------------------------------ decode
-// Multi-reference decode (reference wizard, cuda/gpu_aln_noref.cu:1758-1793, with
-// its float/double mix) and output scatter (:517-540).
-__global__ void k_decode_mref(const PeakRec *__restrict__ rec, int n_img, int N, ShiftView acc,
-                              const float2 *__restrict__ shifts_f, int n_shift, float *__restrict__ peak,
-                              int peak_row, int p_off, float *__restrict__ d, int d_pitch) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n_img * N) return;
-    const int p = idx / N, n = idx % N;
-    const PeakRec r = rec[idx];
+// Angle of a multi-reference peak, as the reference wizard computes it
+// (cuda/gpu_aln_noref.cu:1777-1793): float coefficients from double
+// expressions, the offset in double, the angle stored as float.
+__device__ __forceinline__ float wizard_angle(const PeakRec &r) {
     const float *x = r.x;
     const float c2 = (float)(49. * x[0] + 6. * x[1] - 21. * x[2] - 32. * x[3] - 27. * x[4] -
                              6. * x[5] + 31. * x[6]);
@@ -543,6 +701,46 @@ __global__ void k_decode_mref(const PeakRec *__restrict__ rec, int n_img, int N,
     angle = (float)(360.0 - (double)angle);
     if (r.m == 1) angle = (float)((double)angle + 180.0);
     if (angle >= 360.0f) angle = (float)((double)angle - 360.0);
+    return angle;
+}
+
+// Best candidate per particle -> reference index, peak value, {angle, sx, sy, mirror}.
+__global__ void k_decode_best(const PeakRec *__restrict__ rec, const int *__restrict__ cand,
+                              const int *__restrict__ best, const int *__restrict__ tie, int n_img,
+                              int K, ShiftView acc, const float2 *__restrict__ shifts_f,
+                              int n_shift, int p_off, int *__restrict__ best_ref,
+                              float *__restrict__ best_val, float *__restrict__ best_peak,
+                              int *__restrict__ tie_out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_img) return;
+    const int k = max(best[p], 0);
+    const PeakRec r = rec[(size_t)p * K + k];
+    float ax = 0.0f, ay = 0.0f;
+    if (acc.base) {
+        ax = acc.base[(size_t)p * acc.stride];
+        ay = acc.base[(size_t)p * acc.stride + 1];
+    }
+    const float2 sh = shifts_f[min(max(r.s, 0), n_shift - 1)];
+    const size_t pp = (size_t)p_off + p;
+    best_ref[pp] = cand[(size_t)p * K + k];
+    best_val[pp] = r.val;
+    best_peak[4 * pp + 0] = wizard_angle(r);
+    best_peak[4 * pp + 1] = ax + sh.x;
+    best_peak[4 * pp + 2] = ay + sh.y;
+    best_peak[4 * pp + 3] = r.m ? 1.0f : 0.0f;
+    tie_out[pp] = tie[p];
+}
+
+// Multi-reference decode (reference wizard, cuda/gpu_aln_noref.cu:1758-1793, with
+// its float/double mix) and output scatter (:517-540).
+__global__ void k_decode_mref(const PeakRec *__restrict__ rec, int n_img, int N, ShiftView acc,
+                              const float2 *__restrict__ shifts_f, int n_shift, float *__restrict__ peak,
+                              int peak_row, int p_off, float *__restrict__ d, int d_pitch) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n_img * N) return;
+    const int p = idx / N, n = idx % N;
+    const PeakRec r = rec[idx];
+    const float angle = wizard_angle(r);
     float ax = 0.0f, ay = 0.0f;
     if (acc.base) {
         ax = acc.base[(size_t)p * acc.stride];
@@ -760,8 +958,60 @@ hipError_t launch_ccf_single(const float2 *spec, const float2 *norm, const float
                              PeakRec *rec, hipStream_t st) {
     if (n_img <= 0) return hipSuccess;
     const int blocks = (n_img + 3) / 4;
-    hipLaunchKernelGGL(k_ccf_single, dim3(blocks), dim3(256), 0, st, spec, norm, rspec, rsum0,
-                       ref_of, ref_stride, n_img, n_shift, n_ring, n_ref, twiddle, rec);
+    hipLaunchKernelGGL(k_ccf_items<true>, dim3(blocks), dim3(256), 0, st, spec, norm, rspec, rsum0,
+                       ref_of, ref_stride, 1, n_img, n_shift, n_ring, n_ref, twiddle, rec);
+    return hipGetLastError();
+}
+
+hipError_t launch_ccf_candidates(const float2 *spec, const float2 *norm, const float2 *rspec,
+                                 const float *rsum0, const int *cand, int K, int n_img,
+                                 int n_shift, int n_ring, int n_ref, const float2 *twiddle,
+                                 PeakRec *rec, hipStream_t st) {
+    const int items = n_img * K;
+    if (items <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ccf_items<false>, dim3((items + 3) / 4), dim3(256), 0, st, spec, norm,
+                       rspec, rsum0, cand, 1, K, items, n_shift, n_ring, n_ref, twiddle, rec);
+    return hipGetLastError();
+}
+
+hipError_t launch_topk(const PeakRec *rec, int n_img, int n_ref, int K, int *cand, hipStream_t st) {
+    if (n_img <= 0) return hipSuccess;
+    if (K != 8) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_topk<8>, dim3((n_img + 127) / 128), dim3(128), 0, st, rec, n_img, n_ref, cand);
+    return hipGetLastError();
+}
+
+hipError_t launch_best_candidate(const PeakRec *rec, const int *cand, int n_img, int K,
+                                 float tie_rel, int *best, int *second, int *tie, hipStream_t st) {
+    if (n_img <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_best_candidate, dim3((n_img + 127) / 128), dim3(128), 0, st, rec, cand,
+                       n_img, K, tie_rel, best, second, tie);
+    return hipGetLastError();
+}
+
+hipError_t launch_rescore_f64(const float2 *spec, const float2 *norm, const float2 *rspec,
+                              const PeakRec *rec, const int *cand, int *best, int *second,
+                              const int *tie, int n_img, int K, int n_shift, int n_ring,
+                              int n_ref, double *val64, hipStream_t st) {
+    if (n_img <= 0) return hipSuccess;
+    const int waves = 2 * n_img;
+    hipLaunchKernelGGL(k_rescore_f64, dim3((waves + 3) / 4), dim3(256), 0, st, spec, norm, rspec,
+                       rec, cand, best, second, tie, n_img, K, n_shift, n_ring, n_ref, val64);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_resolve_ties, dim3((n_img + 127) / 128), dim3(128), 0, st, cand, val64,
+                       tie, n_img, K, best, second);
+    return hipGetLastError();
+}
+
+hipError_t launch_decode_best(const PeakRec *rec, const int *cand, const int *best,
+                              const int *tie, int n_img, int K, ShiftView acc,
+                              const float2 *shifts_f, int n_shift, int p_off, int *best_ref,
+                              float *best_val, float *best_peak, int *tie_out, hipStream_t st) {
+    if (n_img <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_decode_best, dim3((n_img + 127) / 128), dim3(128), 0, st, rec, cand, best,
+                       tie, n_img, K, acc, shifts_f, n_shift, p_off, best_ref, best_val, best_peak,
+                       tie_out);
     return hipGetLastError();
 }
 

@@ -31,7 +31,8 @@ REFERENCE_SYMBOLS = (
 )
 EXTRA_SYMBOLS = (
     "gpu_last_error", "gpu_last_error_string", "gpu_clear_error", "isac_mref_align_device",
-    "isac_mref_align_mixed_device", "isac_profile_enable", "isac_profile_read",
+    "isac_mref_align_mixed_device", "isac_mref_best_mixed_device", "isac_profile_enable",
+    "isac_profile_read",
 )
 
 
@@ -115,6 +116,10 @@ def load(path: Optional[str] = None):
     lib.isac_mref_align_mixed_device.argtypes = [vp, vp, vp, c_uint, c_uint, c_uint, c_uint, c_uint,
                                                  c_float, c_float, c_float, c_int, vp, vp, c_uint, vp]
     lib.isac_mref_align_mixed_device.restype = c_int
+    lib.isac_mref_best_mixed_device.argtypes = [vp, vp, vp, c_uint, c_uint, c_uint, c_uint, c_uint,
+                                                c_float, c_float, c_float, c_int, c_float,
+                                                vp, vp, vp, vp, vp, vp, vp]
+    lib.isac_mref_best_mixed_device.restype = c_int
     lib.isac_profile_enable.argtypes = [c_int]
     lib.isac_profile_enable.restype = None
     lib.isac_profile_read.argtypes = [ctypes.POINTER(c_int), ctypes.POINTER(ctypes.c_double),
@@ -358,3 +363,34 @@ def mref_align_mixed_device(sbj, ref, ring_num: int, shift_step: float, shift_rn
         check(lib)
         raise GpuAlignError(f"isac_mref_align_mixed_device returned {rc}")
     return peak, d
+
+
+def mref_best_mixed_device(sbj, ref, ring_num: int, shift_step: float, shift_rng_x: float,
+                           shift_rng_y: float, precision: str = "fp16", acc_xy=None,
+                           tie_rel: float = 1e-5, stream=None):
+    """Mixed-precision search + exact best reference per particle (torch tensors).
+
+    Returns a dict of device tensors: peak_list, d (approximate, every pair),
+    best_ref, best_val, best_peak [P, 4], tie (per particle)."""
+    import torch
+    lib = load()
+    P, D = sbj.shape[0], sbj.shape[-1]
+    N = ref.shape[0]
+    dev = sbj.device
+    out = {"peak_list": torch.empty((N, 4 * P), dtype=torch.float32, device=dev),
+           "d": torch.empty((N, P), dtype=torch.float32, device=dev),
+           "best_ref": torch.empty(P, dtype=torch.int32, device=dev),
+           "best_val": torch.empty(P, dtype=torch.float32, device=dev),
+           "best_peak": torch.empty((P, 4), dtype=torch.float32, device=dev),
+           "tie": torch.empty(P, dtype=torch.int32, device=dev)}
+    acc_ptr = acc_xy.data_ptr() if acc_xy is not None else None
+    st = (stream or torch.cuda.current_stream()).cuda_stream
+    rc = lib.isac_mref_best_mixed_device(
+        sbj.data_ptr(), ref.data_ptr(), acc_ptr, P, N, D, ring_num, RING_LEN, shift_step,
+        shift_rng_x, shift_rng_y, 1 if precision == "bf16" else 0, tie_rel,
+        out["peak_list"].data_ptr(), out["d"].data_ptr(), out["best_ref"].data_ptr(),
+        out["best_val"].data_ptr(), out["best_peak"].data_ptr(), out["tie"].data_ptr(), st)
+    if rc:
+        check(lib)
+        raise GpuAlignError(f"isac_mref_best_mixed_device returned {rc}")
+    return out

@@ -170,6 +170,20 @@ int isac_mref_align_mixed_device(const float *d_sbj, const float *d_ref, const f
                                  float *d_peak_list, float *d_d, unsigned int d_pitch,
                                  void *stream);
 
+/* No counterpart in the reference.  Mixed-precision search plus exact choice of
+ * each particle's best reference: the eight references with the highest
+ * approximate peaks are re-scored in FP32 (same arithmetic as
+ * isac_mref_align_device) and, when the best two are within tie_rel
+ * (relative), re-scored in FP64.  Outputs (device): approximate d [n_ref][n_sbj]
+ * and peak_list [n_ref][4 n_sbj] for every pair; best reference, its peak value,
+ * {angle, shift_x, shift_y, mirror} and the near-tie flag per particle. */
+int isac_mref_best_mixed_device(const float *d_sbj, const float *d_ref, const float *d_acc_xy,
+                                unsigned int n_sbj, unsigned int n_ref, unsigned int img_dim,
+                                unsigned int ring_num, unsigned int ring_len, float shift_step,
+                                float shift_rng_x, float shift_rng_y, int bf16, float tie_rel,
+                                float *d_peak_list, float *d_d, int *d_best_ref, float *d_best_val,
+                                float *d_best_peak, int *d_tie, void *stream);
+
 /* No counterpart in the reference.  When enabled, the library brackets every
  * launch of its contraction+psi kernel with HIP events on the stream it
  * launches on, and accumulates launch count and total milliseconds. */
