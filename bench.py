@@ -9,7 +9,7 @@ images already resident in HBM (isac_mref_align_device); the metric is
 particle x reference alignments per second.
 
 Also reported:
-  roofline      the fused contraction + psi-IFFT kernel (k_ccf_psi): algorithmic
+  roofline      the fused contraction + psi-IFFT kernel (k_ccf_mref): algorithmic
                 flops per launch (pairs x (8 S R K_f + 2 S 2.5 L log2 L)) over its
                 average launch time, timed with HIP events on its stream
   cpu_baseline  the same algorithm's C restatement (FP32 build, OpenMP) on the
@@ -81,6 +81,62 @@ def cpu_baseline(sbj_host, ref_host, acc, R, rng, gpu_peak, gpu_d, budget_s, cor
     return out
 
 
+def precision_report(n_particles=1000):
+    """Config 3: config-1 inputs (1000 x 64^2, 8 references, rings 1..29, +-2 shifts)
+    with the ring contraction in FP16 and in BF16 on MFMA, the top-8 candidates
+    re-scored in FP32 and near-ties in FP64, against the FP64 oracle."""
+    import numpy as np
+    import torch
+    from isac_amd import binding, synth
+    from oracle import coracle
+    ds = synth.config1(n_particles=n_particles)
+    P, N = ds.sbj.shape[0], ds.ref.shape[0]
+    s = torch.from_numpy(ds.sbj).cuda()
+    r = torch.from_numpy(ds.ref).cuda()
+    t0 = time.perf_counter()
+    ref = coracle.mref(ds.sbj, ds.ref, np.zeros((P, 2), np.float32), 29, 2.0, 2.0, 1.0,
+                       nthreads=min(16, len(os.sched_getaffinity(0))))
+    t_oracle = time.perf_counter() - t0
+    cols = np.arange(P)
+    order = np.argsort(-ref["d"], axis=0, kind="stable")
+    ob = order[0]
+    vb, v2 = ref["d"][ob, cols], ref["d"][order[1], cols]
+    otie = (vb - v2) <= 1e-5 * np.abs(vb)
+    out = {"report": "precision (config 3)", "particles": P, "refs": N,
+           "oracle": f"FP64 C restatement, {t_oracle:.1f} s", "oracle_best_ref_ties": int(otie.sum())}
+    for prec in ("fp16", "bf16"):
+        o = binding.mref_best_mixed_device(s, r, 29, 1.0, 2.0, 2.0, prec)
+        torch.cuda.synchronize()
+        o = {k: v.cpu().numpy() for k, v in o.items()}
+        d = o["d"].astype(np.float64)
+        err = np.abs(d - ref["d"])
+        rel = err / np.abs(ref["d"])
+        pk = o["peak_list"]
+        tie = np.asarray(ref["tie"], bool)
+        pair_same = (pk[:, 3::4] == ref["mirror"]) & (np.abs(pk[:, 1::4] - ref["sx"]) < 1e-4) & \
+                    (np.abs(pk[:, 2::4] - ref["sy"]) < 1e-4)
+        dpsi_pair = np.abs((pk[:, 0::4] - ref["angle"] + 180) % 360 - 180)
+        br = o["best_ref"]
+        bp = o["best_peak"]
+        same_ref = br == ob
+        best_same = same_ref & (bp[:, 3] == ref["mirror"][ob, cols]) & \
+            (np.abs(bp[:, 1] - ref["sx"][ob, cols]) < 1e-4) & (np.abs(bp[:, 2] - ref["sy"][ob, cols]) < 1e-4)
+        dpsi = np.abs((bp[:, 0] - ref["angle"][ob, cols] + 180) % 360 - 180)
+        ok = ~otie
+        out[prec] = {
+            "approx_all_pairs": {
+                "ccf_max_abs_err": float(err.max()), "ccf_max_rel_err": float(rel.max()),
+                "ccf_median_rel_err": float(np.median(rel)),
+                "ref_shift_mirror_exact_frac": float(pair_same[~tie].mean()),
+                "psi_err_deg_p50_p99_max": [float(np.percentile(dpsi_pair[pair_same], q)) for q in (50, 99, 100)]},
+            "best_per_particle_after_rescoring": {
+                "ref_shift_mirror_exact_frac": float(best_same[ok].mean()),
+                "psi_within_0.5deg_frac": float((dpsi[ok & best_same] <= 0.5).mean()),
+                "psi_err_deg_p50_p99_max": [float(np.percentile(dpsi[ok & best_same], q)) for q in (50, 99, 100)],
+                "near_ties_flagged": int(o["tie"].sum())}}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,16 +146,23 @@ def main():
     ap.add_argument("--particles", type=int, default=None, help="override particle count")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-forms", action="store_true", help="skip the mixed-precision forms")
+    ap.add_argument("--report", choices=["precision"], default=None,
+                    help="precision: config-3 accuracy report instead of the throughput line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
-              "the path shards over particles with no cross-GPU exchange until the end, "
-              "see DESIGN.md", file=sys.stderr)
+              "the path shards over particles (references replicated) with one gather of d and "
+              "peak_list at the end", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
     import torch
     from isac_amd import binding, synth
+
+    if args.report == "precision":
+        print(json.dumps(precision_report()))
+        return
 
     P, N, D, radius, snr, max_shift, int_shifts, R, rng = WORKLOADS[args.workload]
     if args.particles:
@@ -110,6 +173,8 @@ def main():
     acc = torch.zeros((P, 2), dtype=torch.float32, device=dev)
     peak = torch.empty((N, 4 * P), dtype=torch.float32, device=dev)
     dval = torch.empty((N, P), dtype=torch.float32, device=dev)
+    peak2 = torch.empty_like(peak)
+    dval2 = torch.empty_like(dval)
     lib = binding.load()
     torch.cuda.synchronize()
 
@@ -149,7 +214,7 @@ def main():
         "config": {"workload": f"{args.workload}: {P} particles x {N} refs, {D}x{D} px, "
                                f"rings 1..{R}, shifts +-{int(rng)} step 1 ({S}), psi 256 x 2 mirrors",
                    "particles": P, "refs": N, "img_dim": D, "rings": R, "shifts": S},
-        "roofline": {"bound": "mfma", "kernel": "k_ccf_psi (ring contraction + psi IFFT + search)",
+        "roofline": {"bound": "mfma", "kernel": "k_ccf_mref (ring contraction + psi IFFT + search)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                      "avg_launch_ms": avg_launch_ms, "launches": launches,
@@ -157,6 +222,26 @@ def main():
                      "polar_kernel_ms_per_step": ms_pol.value / args.steps},
         "cpu_baseline": None,
     }
+    if not args.no_forms:
+        # the mixed-precision (MFMA) forms on the same workload, same timing protocol
+        forms = {}
+        for name, fn in (
+                ("fp16", lambda: binding.mref_align_mixed_device(sbj, ref, R, 1.0, rng, rng, "fp16",
+                                                                 acc_xy=acc, out=(peak2, dval2))),
+                ("bf16", lambda: binding.mref_align_mixed_device(sbj, ref, R, 1.0, rng, rng, "bf16",
+                                                                 acc_xy=acc, out=(peak2, dval2))),
+                ("fp16_best_exact", lambda: binding.mref_best_mixed_device(sbj, ref, R, 1.0, rng, rng,
+                                                                           "fp16", acc_xy=acc))):
+            fn()
+            torch.cuda.synchronize()
+            k = max(2, args.steps // 2)
+            t0 = time.perf_counter()
+            for _ in range(k):
+                fn()
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / k
+            forms[name] = {"value": P * N / dt, "ms_per_step": dt * 1e3, "speedup_vs_fp32": ms_step / (dt * 1e3)}
+        res["mfma_forms"] = forms
     if not args.no_cpu_baseline:
         cores = min(16, len(os.sched_getaffinity(0)))
         m = min(P, 4096)
