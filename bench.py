@@ -137,6 +137,51 @@ def precision_report(n_particles=1000):
     return out
 
 
+def batched_report(steps=3, warmup=1, n_subsets=64):
+    """Config 4: 64 uneven subsets (500..5000 particles, box 64/76/96/128, 20..200
+    references, +-2..+-4 shifts, rings to box/2-4) in one captured graph around
+    the persistent work-queue kernel, against one launch sequence per subset."""
+    import numpy as np
+    import torch
+    from isac_amd import binding, synth
+    shapes = synth.config4_shapes(n_subsets)
+    subs = synth.make_subsets_torch(shapes)
+    torch.cuda.synchronize()
+    pairs = sum(sh["P"] * sh["N"] for sh in shapes)
+    bm = binding.BatchedMref(subs)
+    out = {"report": "batched uneven subsets (config 4)", "subsets": n_subsets,
+           "particles": sum(sh["P"] for sh in shapes), "pairs": pairs,
+           "boxes": {str(b): sum(1 for sh in shapes if sh["D"] == b) for b in synth.BATCHED_BOXES}}
+    t0 = time.perf_counter()
+    bm.run(binding.BATCH_GRAPH)   # plan, tune, capture, first launch
+    torch.cuda.synchronize()
+    out["graph_first_call_s"] = time.perf_counter() - t0
+    out["graph_plan"] = bm.report()
+    ref_out = [(p.clone(), d.clone()) for p, d in bm.outputs]
+
+    def timed(mode):
+        for _ in range(warmup):
+            bm.run(mode)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            bm.run(mode)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps
+
+    t_graph = timed(binding.BATCH_GRAPH)
+    assert bm.report()["replayed"]
+    t_each = timed(binding.BATCH_PER_SUBSET)
+    same = all(torch.equal(a, p) and torch.equal(b, d)
+               for (a, b), (p, d) in zip(ref_out, bm.outputs))
+    out.update({
+        "graph_s_per_batch": t_graph, "graph_pairs_per_s": pairs / t_graph,
+        "per_subset_s_per_batch": t_each, "per_subset_pairs_per_s": pairs / t_each,
+        "speedup_graph_vs_per_subset": t_each / t_graph,
+        "outputs_identical": bool(same), "steps": steps, "warmup": warmup})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,8 +192,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the mixed-precision forms")
-    ap.add_argument("--report", choices=["precision"], default=None,
-                    help="precision: config-3 accuracy report instead of the throughput line")
+    ap.add_argument("--report", choices=["precision", "batched"], default=None,
+                    help="precision: config-3 accuracy report; batched: config-4 uneven-subset "
+                         "throughput, graph vs one launch per subset")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -162,6 +208,10 @@ def main():
 
     if args.report == "precision":
         print(json.dumps(precision_report()))
+        return
+    if args.report == "batched":
+        print(json.dumps(batched_report(steps=args.steps if args.steps != 10 else 3,
+                                        warmup=min(args.warmup, 1))))
         return
 
     P, N, D, radius, snr, max_shift, int_shifts, R, rng = WORKLOADS[args.workload]

@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -89,6 +90,34 @@ struct DevBuf {
     T *as() const { return static_cast<T *>(p); }
 };
 
+// Batched multi-reference run: concatenated per-subset tables, the group
+// workspace, work-queue tables and the captured graph (kept for replay).
+struct BatchWs {
+    DevBuf shifts_i, shifts_f, tmpl, rspec, rnorm, rsum0;   // per-subset tables
+    DevBuf spec, norm, rec;                                // group workspace
+    DevBuf jobs, items, qbeg, counters;                    // work queues of every group
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<unsigned char> key;          // descriptor bytes the graph was captured for
+    std::map<std::string, int> tuned;        // group shape signature -> waves per workgroup
+    int groups = 0, nodes = 0, replayed = 0, nw = 0, blocks = 0;
+    float cand_ms[3] = {0.0f, 0.0f, 0.0f};
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        key.clear();
+    }
+    void release() {
+        drop_graph();
+        DevBuf *all[] = {&shifts_i, &shifts_f, &tmpl, &rspec, &rnorm, &rsum0, &spec, &norm,
+                         &rec, &jobs, &items, &qbeg, &counters};
+        for (DevBuf *b : all) b->release();
+        tuned.clear();
+    }
+};
+
 // Work buffers shared by every entry point (grow-only until gpu_clear).
 struct Workspace {
     int dev = -1;
@@ -100,7 +129,9 @@ struct Workspace {
     DevBuf mix_p, mix_pn, mix_r, mix_rn;   // 16-bit MFMA operands and row norms
     DevBuf cand, rec_c, best, second, tie, val64;   // candidate re-scoring
     std::vector<float2> shifts_uploaded;
+    BatchWs batch;
     void release() {
+        batch.release();
         DevBuf *all[] = {&tmpl, &twiddle, &zero_shift, &shifts_i, &shifts_f, &spec, &norm,
                          &rspec, &rnorm, &rsum0, &rec, &peak, &dvals, &acc,
                          &mix_p, &mix_pn, &mix_r, &mix_rn, &cand, &rec_c, &best, &second,
@@ -452,6 +483,393 @@ int run_single(const float *d_sbj, DevParam *prm, const int *ref_of, int ref_str
     return 0;
 }
 
+// ------------------------------------------------------------ batched subsets
+// Many independent multi-reference subsets (uneven particle count, box, reference
+// count, rings, shift range) in one captured graph.  Particles are cut into
+// memory groups whose spectra fit the workspace; each group is: the polar
+// launches of its slices, one persistent work-queue kernel over every
+// (slice, particle, reference block) item, and the decode launches.  Every
+// kernel is the one the single-subset path runs, so results are bit-identical.
+struct BatchSub {
+    const IsacMrefSubset *u;
+    int P, N, D, R, S;
+    size_t sh_off, tm_off, rs_off, rn_off;   // element offsets into the batch tables
+    size_t per;                              // workspace bytes per particle
+};
+struct BatchSlice {
+    int sub, p0, n;
+    size_t spec_off, norm_off, rec_off;      // element offsets into the group workspace
+};
+struct BatchGroup {
+    std::vector<BatchSlice> sl;
+    size_t spec = 0, norm = 0, rec = 0;      // workspace elements used
+    int nw = 4, blocks = 0;
+    size_t job_off = 0, item_off = 0, q_off = 0;
+    int n_items = 0;
+};
+
+// Items (slice, particle, reference block, 0) of a group for `nw` waves per
+// workgroup; particle i of the group goes to XCD queue i % 8 with all its
+// blocks.  `limit[k]` caps the particles taken from slice k (sampling).
+void build_queue(const std::vector<BatchSub> &subs, const BatchGroup &gr, int nw,
+                 const std::vector<int> *limit, std::vector<int4> &items, int qbeg[9]) {
+    std::vector<int4> q[8];
+    int running = 0;
+    for (size_t k = 0; k < gr.sl.size(); ++k) {
+        const BatchSlice &sl = gr.sl[k];
+        const int G = (subs[sl.sub].N + 3) / 4, GB = (G + nw - 1) / nw;
+        const int n = limit ? std::min(sl.n, (*limit)[k]) : sl.n;
+        for (int i = 0; i < n; ++i, ++running)
+            for (int b = 0; b < GB; ++b) q[running & 7].push_back(make_int4((int)k, i, b, 0));
+    }
+    items.clear();
+    for (int x = 0; x < 8; ++x) {
+        qbeg[x] = (int)items.size();
+        items.insert(items.end(), q[x].begin(), q[x].end());
+    }
+    qbeg[8] = (int)items.size();
+}
+
+std::vector<isac::MrefJob> group_jobs(const std::vector<BatchSub> &subs, const BatchGroup &gr,
+                                      int nw) {
+    BatchWs &b = ws.batch;
+    std::vector<isac::MrefJob> jobs;
+    for (const BatchSlice &sl : gr.sl) {
+        const BatchSub &u = subs[sl.sub];
+        jobs.push_back(isac::mref_job(b.spec.as<float2>() + sl.spec_off,
+                                      b.norm.as<float2>() + sl.norm_off,
+                                      b.rspec.as<float2>() + u.rs_off, b.rsum0.as<float>() + u.rn_off,
+                                      b.rec.as<PeakRec>() + sl.rec_off, sl.n, u.S, u.R, u.N, nw));
+    }
+    return jobs;
+}
+
+ShiftView subset_acc(const BatchSub &u, int p0) {
+    ShiftView a{u.u->d_acc_xy, 2};
+    if (a.base) a.base += (size_t)p0 * 2;
+    return a;
+}
+
+int batch_polar(const std::vector<BatchSub> &subs, const BatchSlice &sl, int n, hipStream_t st) {
+    BatchWs &b = ws.batch;
+    const BatchSub &u = subs[sl.sub];
+    HIP_OK(isac::launch_polar_spectra(u.u->d_sbj + (size_t)sl.p0 * u.D * u.D, n, u.D,
+                                      subset_acc(u, sl.p0), b.shifts_i.as<int2>() + u.sh_off, u.S,
+                                      b.tmpl.as<float2>() + u.tm_off, u.R, ws.twiddle.as<float2>(),
+                                      b.spec.as<float2>() + sl.spec_off,
+                                      b.norm.as<float2>() + sl.norm_off, st));
+    return 0;
+}
+
+// Chooses waves per workgroup (1, 2 or 4) for a group by timing the persistent
+// kernel on a sample of the group's particles (about four rounds of resident
+// wavefronts); the winner is cached by the group's shape signature.
+int tune_group(const std::vector<BatchSub> &subs, BatchGroup &gr, int n_cu, bool report,
+               hipStream_t st) {
+    BatchWs &b = ws.batch;
+    std::string sig;
+    for (const BatchSlice &sl : gr.sl) {
+        const BatchSub &u = subs[sl.sub];
+        sig += std::to_string(u.R) + "," + std::to_string(u.S) + "," + std::to_string(u.N) + ";";
+    }
+    const int cand[3] = {1, 2, 4};
+    int bpc[3];
+    for (int c = 0; c < 3; ++c) bpc[c] = isac::mref_queue_blocks_per_cu(cand[c]);
+    auto grid_for = [&](int c) { return std::max(8, (bpc[c] * n_cu + 7) / 8 * 8); };
+    auto it = b.tuned.find(sig);
+    if (it != b.tuned.end()) {
+        gr.nw = it->second;
+        gr.blocks = grid_for(gr.nw == 1 ? 0 : gr.nw == 2 ? 1 : 2);
+        return 0;
+    }
+    // sample: particles from every slice in proportion, ~4 rounds of resident waves
+    long long wave_items = 0;
+    for (const BatchSlice &sl : gr.sl) wave_items += (long long)sl.n * ((subs[sl.sub].N + 3) / 4);
+    const long long want = 4LL * 8 * n_cu;
+    std::vector<int> limit;
+    for (const BatchSlice &sl : gr.sl) {
+        const long long m = wave_items > want ? ((long long)sl.n * want + wave_items - 1) / wave_items
+                                              : sl.n;
+        limit.push_back((int)std::max<long long>(1, std::min<long long>(m, sl.n)));
+    }
+    for (size_t k = 0; k < gr.sl.size(); ++k)
+        if (int e = batch_polar(subs, gr.sl[k], limit[k], st)) return e;
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    float best_ms = 0.0f;
+    int best = 4;
+    float ms_c[3] = {0.0f, 0.0f, 0.0f};
+    int rc = 0;
+    for (int c = 0; c < 3 && !rc; ++c) {
+        if (bpc[c] <= 0) continue;
+        std::vector<isac::MrefJob> jobs = group_jobs(subs, gr, cand[c]);
+        std::vector<int4> items;
+        int qb[9];
+        build_queue(subs, gr, cand[c], &limit, items, qb);
+        if (b.jobs.reserve(jobs.size() * sizeof(isac::MrefJob)) != hipSuccess ||
+            b.items.reserve(std::max<size_t>(items.size(), 1) * sizeof(int4)) != hipSuccess ||
+            b.qbeg.reserve(9 * sizeof(int)) != hipSuccess ||
+            b.counters.reserve(8 * sizeof(int)) != hipSuccess) {
+            rc = kErrHip;
+            break;
+        }
+        (void)hipMemcpy(b.jobs.p, jobs.data(), jobs.size() * sizeof(isac::MrefJob), hipMemcpyHostToDevice);
+        (void)hipMemcpy(b.items.p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice);
+        (void)hipMemcpy(b.qbeg.p, qb, 9 * sizeof(int), hipMemcpyHostToDevice);
+        for (int rep = 0; rep < 2 && !rc; ++rep) {
+            (void)hipEventRecord(e0, st);
+            if (isac::launch_ccf_mref_queue(b.jobs.as<isac::MrefJob>(), b.items.as<int4>(),
+                                            b.qbeg.as<int>(), b.counters.as<int>(), cand[c],
+                                            grid_for(c), ws.twiddle.as<float2>(), st) != hipSuccess)
+                rc = kErrHip;
+            (void)hipEventRecord(e1, st);
+            if (hipEventSynchronize(e1) != hipSuccess) rc = kErrHip;
+        }
+        (void)hipEventElapsedTime(&ms_c[c], e0, e1);
+        if (!rc && (best_ms == 0.0f || ms_c[c] < best_ms)) {
+            best_ms = ms_c[c];
+            best = cand[c];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) {
+        set_error(kErrHip, "batched tuning launch failed");
+        return rc;
+    }
+    gr.nw = best;
+    gr.blocks = grid_for(best == 1 ? 0 : best == 2 ? 1 : 2);
+    b.tuned[sig] = best;
+    if (report) std::copy(ms_c, ms_c + 3, b.cand_ms);
+    return 0;
+}
+
+// Group workspace cap: 16 GiB (beyond it a larger group buys nothing), or
+// ISAC_BATCH_WORKSPACE_MB when set.
+size_t batch_cap() {
+    size_t cap = (size_t)16 << 30;
+    if (const char *env = getenv("ISAC_BATCH_WORKSPACE_MB")) {
+        const long long mb = atoll(env);
+        if (mb > 0) cap = std::min(cap, (size_t)mb << 20);
+    }
+    return cap;
+}
+
+// Validates the descriptors, builds the per-subset tables and the groups.
+int batch_plan(const IsacMrefSubset *d, unsigned n, std::vector<BatchSub> &subs,
+               std::vector<BatchGroup> &groups, hipStream_t st) {
+    BatchWs &b = ws.batch;
+    std::vector<int2> shi;
+    std::vector<float2> shf, tm;
+    size_t rs = 0, rn = 0;
+    for (unsigned i = 0; i < n; ++i) {
+        const IsacMrefSubset &x = d[i];
+        std::vector<float2> sh = make_shift_grid(x.shift_rng_x, x.shift_rng_y, x.shift_step);
+        if (int e = check_shape((int)x.img_dim, (int)x.ring_num, kRingLen, sh.size())) return e;
+        if (x.n_sbj == 0 || x.n_ref == 0) continue;
+        if (!x.d_sbj || !x.d_ref || !x.d_peak_list || !x.d_d || x.d_pitch < x.n_sbj ||
+            x.n_sbj > (1u << 30) || x.n_ref > (1u << 20)) {
+            set_error(kErrArg, "subset %u: bad pointers or sizes (d_pitch < n_sbj?)", i);
+            return kErrArg;
+        }
+        BatchSub u;
+        u.u = &x;
+        u.P = (int)x.n_sbj;
+        u.N = (int)x.n_ref;
+        u.D = (int)x.img_dim;
+        u.R = (int)x.ring_num;
+        u.S = (int)sh.size();
+        u.sh_off = shf.size();
+        u.tm_off = tm.size();
+        u.rs_off = rs;
+        u.rn_off = rn;
+        u.per = (size_t)u.S * u.R * kSlots * sizeof(float2) + (size_t)u.S * sizeof(float2) +
+                (size_t)u.N * sizeof(PeakRec);
+        for (const float2 &v : sh) {   // the kernel argument is int (quirk Q1)
+            shi.push_back(make_int2((int)v.x, (int)v.y));
+            shf.push_back(v);
+        }
+        std::vector<float2> t = make_template(u.R);
+        tm.insert(tm.end(), t.begin(), t.end());
+        rs += (size_t)u.N * u.R * kSlots;
+        rn += (size_t)u.N;
+        subs.push_back(u);
+    }
+    if (subs.empty()) return 0;
+    HIP_OK(b.shifts_i.reserve(shi.size() * sizeof(int2)));
+    HIP_OK(b.shifts_f.reserve(shf.size() * sizeof(float2)));
+    HIP_OK(b.tmpl.reserve(tm.size() * sizeof(float2)));
+    HIP_OK(b.rspec.reserve(rs * sizeof(float2)));
+    HIP_OK(b.rnorm.reserve(rn * sizeof(float2)));
+    HIP_OK(b.rsum0.reserve(rn * sizeof(float)));
+    HIP_OK(hipMemcpy(b.shifts_i.p, shi.data(), shi.size() * sizeof(int2), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(b.shifts_f.p, shf.data(), shf.size() * sizeof(float2), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(b.tmpl.p, tm.data(), tm.size() * sizeof(float2), hipMemcpyHostToDevice));
+
+    // groups: subsets by work per particle, largest first, cut to the budget
+    std::vector<int> order(subs.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+        return (long long)subs[a].S * subs[a].R * subs[a].N > (long long)subs[c].S * subs[c].R * subs[c].N;
+    });
+    size_t max_per = 0;
+    for (const BatchSub &u : subs) max_per = std::max(max_per, u.per);
+    const size_t have = (size_t)(0.9 * (double)(free_device_bytes() + b.spec.cap + b.norm.cap + b.rec.cap));
+    const size_t budget = std::max(std::min(have, batch_cap()), max_per);
+    BatchGroup cur;
+    size_t used = 0;
+    for (int si : order) {
+        const BatchSub &u = subs[si];
+        int p0 = 0;
+        while (p0 < u.P) {
+            int fit = (int)std::min<size_t>((budget - used) / u.per, (size_t)(u.P - p0));
+            if (fit <= 0) {
+                groups.push_back(cur);
+                cur = BatchGroup();
+                used = 0;
+                continue;
+            }
+            BatchSlice sl{si, p0, fit, cur.spec, cur.norm, cur.rec};
+            cur.sl.push_back(sl);
+            cur.spec += (size_t)fit * u.S * u.R * kSlots;
+            cur.norm += (size_t)fit * u.S;
+            cur.rec += (size_t)fit * u.N;
+            used += (size_t)fit * u.per;
+            p0 += fit;
+        }
+    }
+    if (!cur.sl.empty()) groups.push_back(cur);
+    size_t spec = 0, norm = 0, rec = 0;
+    for (const BatchGroup &gr : groups) {
+        spec = std::max(spec, gr.spec);
+        norm = std::max(norm, gr.norm);
+        rec = std::max(rec, gr.rec);
+    }
+    HIP_OK(b.spec.reserve(spec * sizeof(float2)));
+    HIP_OK(b.norm.reserve(norm * sizeof(float2)));
+    HIP_OK(b.rec.reserve(rec * sizeof(PeakRec)));
+    (void)st;
+    return 0;
+}
+
+// Records the whole batch on `st` (being captured).
+int batch_record(const std::vector<BatchSub> &subs, const std::vector<BatchGroup> &groups,
+                 hipStream_t st) {
+    BatchWs &b = ws.batch;
+    for (const BatchSub &u : subs) {
+        HIP_OK(isac::launch_polar_spectra(u.u->d_ref, u.N, u.D, ShiftView{nullptr, 0},
+                                          ws.zero_shift.as<int2>(), 1, b.tmpl.as<float2>() + u.tm_off,
+                                          u.R, ws.twiddle.as<float2>(), b.rspec.as<float2>() + u.rs_off,
+                                          b.rnorm.as<float2>() + u.rn_off, st));
+        HIP_OK(isac::launch_ref_finalize(b.rspec.as<float2>() + u.rs_off,
+                                         b.rnorm.as<float2>() + u.rn_off, u.N, u.R,
+                                         b.rsum0.as<float>() + u.rn_off, st));
+    }
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+        const BatchGroup &gr = groups[gi];
+        for (const BatchSlice &sl : gr.sl)
+            if (int e = batch_polar(subs, sl, sl.n, st)) return e;
+        HIP_OK(isac::launch_ccf_mref_queue(b.jobs.as<isac::MrefJob>() + gr.job_off,
+                                           b.items.as<int4>() + gr.item_off,
+                                           b.qbeg.as<int>() + gr.q_off, b.counters.as<int>() + 8 * gi,
+                                           gr.nw, gr.blocks, ws.twiddle.as<float2>(), st));
+        for (const BatchSlice &sl : gr.sl) {
+            const BatchSub &u = subs[sl.sub];
+            HIP_OK(isac::launch_decode_mref(b.rec.as<PeakRec>() + sl.rec_off, sl.n, u.N,
+                                            subset_acc(u, sl.p0), b.shifts_f.as<float2>() + u.sh_off,
+                                            u.S, u.u->d_peak_list, 4 * u.P, sl.p0, u.u->d_d,
+                                            (int)u.u->d_pitch, st));
+        }
+    }
+    return 0;
+}
+
+int run_batched_graph(const IsacMrefSubset *d, unsigned n, hipStream_t st) {
+    BatchWs &b = ws.batch;
+    const unsigned char *kb = reinterpret_cast<const unsigned char *>(d);
+    std::vector<unsigned char> key(kb, kb + (size_t)n * sizeof(IsacMrefSubset));
+    const size_t cap = batch_cap();
+    const unsigned char *cb = reinterpret_cast<const unsigned char *>(&cap);
+    key.insert(key.end(), cb, cb + sizeof(cap));
+    if (b.exec && key == b.key) {
+        HIP_OK(hipGraphLaunch(b.exec, st));
+        HIP_OK(hipStreamSynchronize(st));
+        b.replayed = 1;
+        return 0;
+    }
+    b.drop_graph();
+    b.replayed = 0;
+    std::vector<BatchSub> subs;
+    std::vector<BatchGroup> groups;
+    if (int e = batch_plan(d, n, subs, groups, ws.stream)) return e;
+    b.groups = (int)groups.size();
+    b.nodes = 0;
+    if (subs.empty()) return 0;
+    HIP_OK(isac::prepare_launches());
+    int dev = 0, n_cu = 0;
+    HIP_OK(hipGetDevice(&dev));
+    HIP_OK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    // reference spectra first (the tuning sample needs them)
+    for (const BatchSub &u : subs) {
+        HIP_OK(isac::launch_polar_spectra(u.u->d_ref, u.N, u.D, ShiftView{nullptr, 0},
+                                          ws.zero_shift.as<int2>(), 1, b.tmpl.as<float2>() + u.tm_off,
+                                          u.R, ws.twiddle.as<float2>(), b.rspec.as<float2>() + u.rs_off,
+                                          b.rnorm.as<float2>() + u.rn_off, ws.stream));
+        HIP_OK(isac::launch_ref_finalize(b.rspec.as<float2>() + u.rs_off,
+                                         b.rnorm.as<float2>() + u.rn_off, u.N, u.R,
+                                         b.rsum0.as<float>() + u.rn_off, ws.stream));
+    }
+    for (size_t gi = 0; gi < groups.size(); ++gi)
+        if (int e = tune_group(subs, groups[gi], n_cu, gi == 0, ws.stream)) return e;
+    b.nw = groups[0].nw;
+    b.blocks = groups[0].blocks;
+    // work-queue tables of every group
+    std::vector<isac::MrefJob> jobs;
+    std::vector<int4> items;
+    std::vector<int> qbeg;
+    for (BatchGroup &gr : groups) {
+        std::vector<isac::MrefJob> j = group_jobs(subs, gr, gr.nw);
+        std::vector<int4> it;
+        int qb[9];
+        build_queue(subs, gr, gr.nw, nullptr, it, qb);
+        gr.job_off = jobs.size();
+        gr.item_off = items.size();
+        gr.q_off = qbeg.size();
+        gr.n_items = (int)it.size();
+        jobs.insert(jobs.end(), j.begin(), j.end());
+        items.insert(items.end(), it.begin(), it.end());
+        qbeg.insert(qbeg.end(), qb, qb + 9);
+    }
+    HIP_OK(b.jobs.reserve(jobs.size() * sizeof(isac::MrefJob)));
+    HIP_OK(b.items.reserve(std::max<size_t>(items.size(), 1) * sizeof(int4)));
+    HIP_OK(b.qbeg.reserve(qbeg.size() * sizeof(int)));
+    HIP_OK(b.counters.reserve(groups.size() * 8 * sizeof(int)));
+    HIP_OK(hipMemcpy(b.jobs.p, jobs.data(), jobs.size() * sizeof(isac::MrefJob), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(b.items.p, items.data(), items.size() * sizeof(int4), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(b.qbeg.p, qbeg.data(), qbeg.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_OK(hipStreamSynchronize(ws.stream));
+    // capture
+    HIP_OK(hipStreamBeginCapture(ws.stream, hipStreamCaptureModeRelaxed));
+    const int rec_rc = batch_record(subs, groups, ws.stream);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(ws.stream, &graph);
+    if (rec_rc || ec != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (!rec_rc) set_error(kErrHip, "graph capture failed: %s", hipGetErrorString(ec));
+        return rec_rc ? rec_rc : kErrHip;
+    }
+    b.graph = graph;
+    size_t nodes = 0;
+    (void)hipGraphGetNodes(graph, nullptr, &nodes);
+    b.nodes = (int)nodes;
+    HIP_OK(hipGraphInstantiate(&b.exec, graph, nullptr, nullptr, 0));
+    b.key = key;
+    HIP_OK(hipGraphLaunch(b.exec, st));
+    HIP_OK(hipStreamSynchronize(st));
+    return 0;
+}
+
 // cuda/gpu_aln_noref.cu:86-115
 int alignment_init(const AlignConfig *cfg, unsigned dev) {
     if (!cfg) {
@@ -741,6 +1159,45 @@ int isac_mref_best_mixed_device(const float *d_sbj, const float *d_ref, const fl
     return run_mref_mixed(d_sbj, d_ref, acc, (int)n_sbj, (int)n_ref, (int)img_dim, (int)ring_num,
                           sh, bf16 != 0, d_peak_list, 4 * (int)n_sbj, d_d, (int)n_sbj, 0.9f, st,
                           &bo);
+}
+
+int isac_mref_align_batched_device(const IsacMrefSubset *subsets, unsigned int n_subsets,
+                                   int mode, void *stream) {
+    if (int e = ensure_device_ws(-1)) return e;
+    if (n_subsets == 0) return 0;
+    if (!subsets) {
+        set_error(kErrArg, "isac_mref_align_batched_device: NULL descriptor array");
+        return kErrArg;
+    }
+    hipStream_t st = stream ? (hipStream_t)stream : ws.stream;
+    if (mode == ISAC_BATCH_GRAPH) {
+        HIP_OK(hipStreamSynchronize(st));   // inputs written on the caller's stream
+        return run_batched_graph(subsets, n_subsets, st);
+    }
+    if (mode != ISAC_BATCH_PER_SUBSET) {
+        set_error(kErrArg, "isac_mref_align_batched_device: unknown mode %d", mode);
+        return kErrArg;
+    }
+    for (unsigned i = 0; i < n_subsets; ++i) {
+        const IsacMrefSubset &x = subsets[i];
+        const int e = isac_mref_align_device(x.d_sbj, x.d_ref, x.d_acc_xy, x.n_sbj, x.n_ref,
+                                             x.img_dim, x.ring_num, kRingLen, x.shift_step,
+                                             x.shift_rng_x, x.shift_rng_y, x.d_peak_list, x.d_d,
+                                             x.d_pitch, stream);
+        if (e) return e;
+    }
+    return 0;
+}
+
+void isac_batched_report(int *n_groups, int *n_nodes, int *replayed, int *nw, int *blocks,
+                         float *cand_ms3) {
+    const BatchWs &b = ws.batch;
+    if (n_groups) *n_groups = b.groups;
+    if (n_nodes) *n_nodes = b.nodes;
+    if (replayed) *replayed = b.replayed;
+    if (nw) *nw = b.nw;
+    if (blocks) *blocks = b.blocks;
+    if (cand_ms3) std::copy(b.cand_ms, b.cand_ms + 3, cand_ms3);
 }
 
 void isac_profile_enable(int on) {

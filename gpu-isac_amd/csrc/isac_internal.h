@@ -44,6 +44,31 @@ struct ShiftView {
     int stride;
 };
 
+// One multi-reference job (a particle batch of one subset) for the FP32 kernel.
+struct MrefJob {
+    const float2 *spec;     // [n_img][S][R][kSlots] raw particle spectra
+    const float2 *norm;     // [n_img][S]
+    const float2 *rspec;    // [N][R][kSlots] normalised, ring-weighted reference spectra
+    const float *rsum0;     // [N]
+    PeakRec *rec;           // [n_img][N]
+    int n_img, S, R, N, G, GB;   // G reference groups of 4, GB workgroups per particle
+};
+
+MrefJob mref_job(const float2 *spec, const float2 *norm, const float2 *rspec, const float *rsum0,
+                 PeakRec *rec, int n_img, int n_shift, int n_ring, int n_ref, int nw);
+
+// Persistent multi-reference kernel over many jobs: items (job, particle, block)
+// in 8 per-XCD queues [qbeg[q], qbeg[q+1]); counters (8 ints) are reset by a
+// kernel first (no memset node, safe inside captured graphs).
+// nw (waves per workgroup) is 1, 2 or 4; blocks should be a multiple of 8.
+hipError_t launch_ccf_mref_queue(const MrefJob *jobs, const int4 *items, const int *qbeg,
+                                 int *counters, int nw, int blocks, const float2 *twiddle,
+                                 hipStream_t st);
+int mref_queue_blocks_per_cu(int nw);   // occupancy of the queue kernel (0 on error)
+
+// One-time launch attributes (dynamic LDS limit); call before any stream capture.
+hipError_t prepare_launches();
+
 // Stage A: normalised-later ring spectra of every image at every grid shift.
 // spec[img][s][ring][kSlots] float2 (raw, unnormalised), norm[img][s] = (mu*L, 1/(sd+1e-7)).
 hipError_t launch_polar_spectra(const float *imgs, int n_img, int img_dim, ShiftView acc,

@@ -561,25 +561,23 @@ __global__ void k_resolve_ties(const int *__restrict__ cand, const double *__res
 constexpr int kRingChunk = 4;
 
 template <int NW>
-__global__ __launch_bounds__(NW * 64) void k_ccf_mref(
-    const float2 *__restrict__ spec, const float2 *__restrict__ norm,
-    const float2 *__restrict__ rspec, const float *__restrict__ rsum0, int n_img, int S, int R,
-    int N, int G, int GB, const float2 *__restrict__ twg, PeakRec *__restrict__ rec) {
-    constexpr int NT = NW * 64;
+struct MrefLds {
+    float2 tw[256];
+    float4 abuf4[2][4 * kRingChunk * kSlots / 2];   // [buf][ring][row][slot] float2 pairs
+    float2 fbuf[NW][512];
+    float zbuf[NW][256];
+    float nbuf[NW][4][2][8];
+};
+
+// Work of one workgroup: particle p of job j, reference groups gb*NW .. gb*NW+NW-1
+// (the twiddles must already be in sh.tw).
+template <int NW>
+__device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb, MrefLds<NW> &sh) {
     constexpr int CH4 = 4 * kRingChunk * kSlots / 2;   // float4 per staged chunk
-    __shared__ float2 tw[256];
-    __shared__ float4 abuf4[2][CH4];                    // [buf][ring][row][slot] float2 pairs
-    __shared__ float2 fbuf[NW][512];
-    __shared__ float zbuf[NW][256];
-    __shared__ float nbuf[NW][4][2][8];
+    const int S = j.S, R = j.R, N = j.N;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int i = tid; i < 256; i += NT) tw[i] = twg[i];
-    __syncthreads();
-    const int lb = xcd_remap(blockIdx.x, gridDim.x);
-    const int p = lb / GB;
-    if (p >= n_img) return;  // uniform over the workgroup
-    const int g = (lb % GB) * NW + wave;
-    const bool active = g < G;
+    const int g = gb * NW + wave;
+    const bool active = g < j.G;
     int nref[4];
     bool nval[4];
 #pragma unroll
@@ -593,19 +591,19 @@ __global__ __launch_bounds__(NW * 64) void k_ccf_mref(
     float rs0[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        rs0[c] = rsum0[nref[c]];
+        rs0[c] = j.rsum0[nref[c]];
         bv[c][0] = bv[c][1] = -3.40282e+38f;
         bf[c][0] = bf[c][1] = INT_MAX;
     }
     const size_t img_stride = (size_t)S * R * kSlots;
     const int nchunk = (R + kRingChunk - 1) / kRingChunk;
-    const float2 *abuf = reinterpret_cast<const float2 *>(&abuf4[0][0]);
+    const float2 *abuf = reinterpret_cast<const float2 *>(&sh.abuf4[0][0]);
 
     for (int s0 = 0; s0 < S; s0 += 4) {
         const float4 *rows4[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-            rows4[r] = reinterpret_cast<const float4 *>(spec + (size_t)p * img_stride +
+            rows4[r] = reinterpret_cast<const float4 *>(j.spec + (size_t)p * img_stride +
                                                         (size_t)min(s0 + r, S - 1) * R * kSlots);
         // Stage one chunk (4 rows x kRingChunk rings, 1 KB per row-ring) with direct
         // global->LDS loads: one wavefront instruction moves one row-ring, each lane
@@ -617,7 +615,7 @@ __global__ __launch_bounds__(NW * 64) void k_ccf_mref(
                 if (ring >= R) continue;
                 __builtin_amdgcn_global_load_lds(
                     (const void *)(rows4[r] + ring * (kSlots / 2) + lane),
-                    (__attribute__((address_space(3))) void *)(&abuf4[buf][(ii * 4 + r) * (kSlots / 2)]),
+                    (__attribute__((address_space(3))) void *)(&sh.abuf4[buf][(ii * 4 + r) * (kSlots / 2)]),
                     16, 0, 0);
             }
         };
@@ -648,7 +646,7 @@ __global__ __launch_bounds__(NW * 64) void k_ccf_mref(
                 }
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    const float2 *rb = rspec + ((size_t)nref[c] * R + i) * kSlots;
+                    const float2 *rb = j.rspec + ((size_t)nref[c] * R + i) * kSlots;
                     b[c][0] = rb[lane];
                     b[c][1] = rb[lane + 64];
                 }
@@ -672,18 +670,71 @@ __global__ __launch_bounds__(NW * 64) void k_ccf_mref(
         for (int r = 0; r < 4; ++r) {
             const int s = s0 + r;
             if (s >= S) break;
-            const float2 nm = norm[(size_t)p * S + s];
+            const float2 nm = j.norm[(size_t)p * S + s];
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 if (!nval[c]) continue;
-                psi_update<false>(acc[r][c][0], acc[r][c][1], nm, rs0[c], s, lane, fbuf[wave],
-                                  fbuf[wave] + 256, tw, zbuf[wave], nbuf[wave][c], bv[c], bf[c]);
+                psi_update<false>(acc[r][c][0], acc[r][c][1], nm, rs0[c], s, lane, sh.fbuf[wave],
+                                  sh.fbuf[wave] + 256, sh.tw, sh.zbuf[wave], sh.nbuf[wave][c],
+                                  bv[c], bf[c]);
             }
         }
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c)
-        if (nval[c]) write_rec(rec + (size_t)p * N + nref[c], bv[c], bf[c], nbuf[wave][c], lane);
+        if (nval[c])
+            write_rec(j.rec + (size_t)p * N + nref[c], bv[c], bf[c], sh.nbuf[wave][c], lane);
+}
+
+// Multi-reference: one workgroup = one particle x NW groups of 4 references (one
+// group per wavefront).  The workgroup walks the shifts four at a time; the four
+// shift rows' ring spectra are staged cooperatively in LDS in double-buffered
+// chunks of kRingChunk rings, so only the references' spectra come through the
+// vector L1.  Lane l accumulates, over rings, P1..P4 of slots l and l+64 for
+// 4 rows x 4 references (128 accumulators), then each (row, reference) pair goes
+// through psi_update.
+template <int NW>
+__global__ __launch_bounds__(NW * 64) void k_ccf_mref(MrefJob j, const float2 *__restrict__ twg) {
+    __shared__ MrefLds<NW> sh;
+    for (int i = threadIdx.x; i < 256; i += NW * 64) sh.tw[i] = twg[i];
+    __syncthreads();
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int p = lb / j.GB;
+    if (p >= j.n_img) return;  // uniform over the workgroup
+    mref_block<NW>(j, p, lb % j.GB, sh);
+}
+
+// Persistent form over many jobs (batched subsets): work items (job, particle,
+// reference block) sit in 8 queues, one per XCD (items of one particle share a
+// queue, so its spectra stay in one L2); each workgroup takes items from the
+// queue of the XCD it runs on (blockIdx % 8) until the queue is exhausted.  The
+// claim is a vector atomic by one lane, broadcast through LDS.  Every workgroup
+// leaves the loop once its queue is empty.
+template <int NW>
+__global__ __launch_bounds__(NW * 64) void k_ccf_mref_queue(const MrefJob *__restrict__ jobs,
+                                                            const int4 *__restrict__ items,
+                                                            const int *__restrict__ qbeg,
+                                                            int *__restrict__ counters,
+                                                            const float2 *__restrict__ twg) {
+    __shared__ MrefLds<NW> sh;
+    __shared__ int claim;
+    for (int i = threadIdx.x; i < 256; i += NW * 64) sh.tw[i] = twg[i];
+    const int q = blockIdx.x & 7;
+    const int q0 = qbeg[q], q1 = qbeg[q + 1];
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) claim = atomicAdd(&counters[q], 1);
+        __syncthreads();
+        const int idx = q0 + claim;
+        if (idx >= q1) break;
+        const int4 w = items[idx];
+        mref_block<NW>(jobs[w.x], w.y, w.z, sh);
+    }
+}
+
+__global__ void k_reset_counters(int *__restrict__ c, int n) {
+    const int i = threadIdx.x;
+    if (i < n) c[i] = 0;
 }
 
 // ------------------------------------------------------------------ decode
@@ -902,12 +953,7 @@ __global__ void k_row_c2r(float *__restrict__ refs, int D, const float2 *__restr
 
 // ---------------------------------------------------------------- launchers
 
-hipError_t launch_polar_spectra(const float *imgs, int n_img, int img_dim, ShiftView acc,
-                                const int2 *shift_i, int n_shift, const float2 *tmpl, int n_ring,
-                                const float2 *twiddle, float2 *spec, float2 *norm,
-                                hipStream_t st) {
-    if (n_img <= 0 || n_shift <= 0) return hipSuccess;
-    const size_t lds = (256 + 4 * 512) * sizeof(float2) + (size_t)img_dim * img_dim * sizeof(float);
+hipError_t prepare_launches() {
     static bool lds_raised = false;
     if (!lds_raised) {
         const hipError_t e = hipFuncSetAttribute((const void *)k_polar_spectra,
@@ -917,6 +963,16 @@ hipError_t launch_polar_spectra(const float *imgs, int n_img, int img_dim, Shift
         if (e != hipSuccess) return e;
         lds_raised = true;
     }
+    return hipSuccess;
+}
+
+hipError_t launch_polar_spectra(const float *imgs, int n_img, int img_dim, ShiftView acc,
+                                const int2 *shift_i, int n_shift, const float2 *tmpl, int n_ring,
+                                const float2 *twiddle, float2 *spec, float2 *norm,
+                                hipStream_t st) {
+    if (n_img <= 0 || n_shift <= 0) return hipSuccess;
+    const size_t lds = (256 + 4 * 512) * sizeof(float2) + (size_t)img_dim * img_dim * sizeof(float);
+    if (hipError_t e = prepare_launches()) return e;
     dim3 grid(n_img);
     hipLaunchKernelGGL(k_polar_spectra, grid, dim3(256), lds, st, imgs, img_dim, acc, shift_i,
                        n_shift, tmpl, n_ring, twiddle, spec, norm);
@@ -931,13 +987,9 @@ hipError_t launch_ref_finalize(float2 *rspec, const float2 *rnorm, int n_ref, in
 }
 
 template <int NW>
-static hipError_t launch_mref_nw(const float2 *spec, const float2 *norm, const float2 *rspec,
-                                 const float *rsum0, int n_img, int S, int R, int N, int G,
-                                 const float2 *tw, PeakRec *rec, hipStream_t st) {
-    const int GB = (G + NW - 1) / NW;
-    const long long blocks = (long long)n_img * GB;
-    hipLaunchKernelGGL((k_ccf_mref<NW>), dim3((unsigned)blocks), dim3(NW * 64), 0, st, spec, norm,
-                       rspec, rsum0, n_img, S, R, N, G, GB, tw, rec);
+static hipError_t launch_mref_nw(const MrefJob &j, const float2 *tw, hipStream_t st) {
+    const long long blocks = (long long)j.n_img * j.GB;
+    hipLaunchKernelGGL((k_ccf_mref<NW>), dim3((unsigned)blocks), dim3(NW * 64), 0, st, j, tw);
     return hipGetLastError();
 }
 
@@ -947,9 +999,53 @@ hipError_t launch_ccf_mref(const float2 *spec, const float2 *norm, const float2 
     if (n_img <= 0 || n_ref <= 0) return hipSuccess;
     // Four waves (16 references) per workgroup: at ~250 VGPRs two waves fit per
     // SIMD, so two workgroups share a CU; measured faster than 2, 3, 5 or 8 waves.
-    const int G = (n_ref + 3) / 4;
-    return launch_mref_nw<4>(spec, norm, rspec, rsum0, n_img, n_shift, n_ring, n_ref, G, twiddle,
-                             rec, st);
+    MrefJob j = mref_job(spec, norm, rspec, rsum0, rec, n_img, n_shift, n_ring, n_ref, 4);
+    return launch_mref_nw<4>(j, twiddle, st);
+}
+
+MrefJob mref_job(const float2 *spec, const float2 *norm, const float2 *rspec, const float *rsum0,
+                 PeakRec *rec, int n_img, int n_shift, int n_ring, int n_ref, int nw) {
+    MrefJob j;
+    j.spec = spec;
+    j.norm = norm;
+    j.rspec = rspec;
+    j.rsum0 = rsum0;
+    j.rec = rec;
+    j.n_img = n_img;
+    j.S = n_shift;
+    j.R = n_ring;
+    j.N = n_ref;
+    j.G = (n_ref + 3) / 4;
+    j.GB = (j.G + nw - 1) / nw;
+    return j;
+}
+
+int mref_queue_blocks_per_cu(int nw) {
+    int n = 0;
+    hipError_t e = hipErrorInvalidValue;
+    if (nw == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_ccf_mref_queue<1>, 64, 0);
+    if (nw == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_ccf_mref_queue<2>, 128, 0);
+    if (nw == 4) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_ccf_mref_queue<4>, 256, 0);
+    return e == hipSuccess ? n : 0;
+}
+
+hipError_t launch_ccf_mref_queue(const MrefJob *jobs, const int4 *items, const int *qbeg,
+                                 int *counters, int nw, int blocks, const float2 *twiddle,
+                                 hipStream_t st) {
+    if (nw != 1 && nw != 2 && nw != 4) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, counters, 8);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (nw == 1)
+        hipLaunchKernelGGL((k_ccf_mref_queue<1>), dim3(blocks), dim3(64), 0, st, jobs, items, qbeg,
+                           counters, twiddle);
+    else if (nw == 2)
+        hipLaunchKernelGGL((k_ccf_mref_queue<2>), dim3(blocks), dim3(128), 0, st, jobs, items, qbeg,
+                           counters, twiddle);
+    else
+        hipLaunchKernelGGL((k_ccf_mref_queue<4>), dim3(blocks), dim3(256), 0, st, jobs, items, qbeg,
+                           counters, twiddle);
+    return hipGetLastError();
 }
 
 hipError_t launch_ccf_single(const float2 *spec, const float2 *norm, const float2 *rspec,

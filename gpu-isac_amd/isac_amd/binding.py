@@ -32,8 +32,11 @@ REFERENCE_SYMBOLS = (
 EXTRA_SYMBOLS = (
     "gpu_last_error", "gpu_last_error_string", "gpu_clear_error", "isac_mref_align_device",
     "isac_mref_align_mixed_device", "isac_mref_best_mixed_device", "isac_profile_enable",
-    "isac_profile_read",
+    "isac_profile_read", "isac_mref_align_batched_device", "isac_batched_report",
 )
+
+BATCH_PER_SUBSET = 0   # ISAC_BATCH_PER_SUBSET
+BATCH_GRAPH = 1        # ISAC_BATCH_GRAPH
 
 
 class AlignConfig(ctypes.Structure):
@@ -50,6 +53,16 @@ class AlignParam(ctypes.Structure):
     _fields_ = [("sbj_id", ctypes.c_int), ("ref_id", ctypes.c_int),
                 ("shift_x", ctypes.c_float), ("shift_y", ctypes.c_float),
                 ("angle", ctypes.c_float), ("mirror", ctypes.c_bool)]
+
+
+class IsacMrefSubset(ctypes.Structure):
+    """Layout of IsacMrefSubset in include/gpu_aln_pack.h (72 bytes)."""
+    _fields_ = [("d_sbj", ctypes.c_void_p), ("d_ref", ctypes.c_void_p),
+                ("d_acc_xy", ctypes.c_void_p), ("d_peak_list", ctypes.c_void_p),
+                ("d_d", ctypes.c_void_p), ("n_sbj", ctypes.c_uint), ("n_ref", ctypes.c_uint),
+                ("img_dim", ctypes.c_uint), ("ring_num", ctypes.c_uint),
+                ("shift_step", ctypes.c_float), ("shift_rng_x", ctypes.c_float),
+                ("shift_rng_y", ctypes.c_float), ("d_pitch", ctypes.c_uint)]
 
 
 float_ptr = ctypes.POINTER(ctypes.c_float)
@@ -120,6 +133,11 @@ def load(path: Optional[str] = None):
                                                 c_float, c_float, c_float, c_int, c_float,
                                                 vp, vp, vp, vp, vp, vp, vp]
     lib.isac_mref_best_mixed_device.restype = c_int
+    lib.isac_mref_align_batched_device.argtypes = [ctypes.POINTER(IsacMrefSubset), c_uint, c_int, vp]
+    lib.isac_mref_align_batched_device.restype = c_int
+    ip = ctypes.POINTER(c_int)
+    lib.isac_batched_report.argtypes = [ip, ip, ip, ip, ip, ctypes.POINTER(c_float)]
+    lib.isac_batched_report.restype = None
     lib.isac_profile_enable.argtypes = [c_int]
     lib.isac_profile_enable.restype = None
     lib.isac_profile_read.argtypes = [ctypes.POINTER(c_int), ctypes.POINTER(ctypes.c_double),
@@ -394,3 +412,55 @@ def mref_best_mixed_device(sbj, ref, ring_num: int, shift_step: float, shift_rng
         check(lib)
         raise GpuAlignError(f"isac_mref_best_mixed_device returned {rc}")
     return out
+
+
+class BatchedMref:
+    """Many multi-reference subsets through one call (torch tensors on cuda:0).
+
+    ``subsets`` is a list of dicts with keys sbj (P, D, D), ref (N, D, D),
+    ring_num, shift_rng (or shift_rng_x / shift_rng_y), shift_step (default 1)
+    and optionally acc_xy (P, 2).  Output tensors (peak_list [N, 4P], d [N, P])
+    are allocated once; the descriptor array is kept, so repeated ``run`` calls
+    in graph mode replay the captured graph."""
+
+    def __init__(self, subsets):
+        import torch
+        self.lib = load()
+        self.subsets = subsets
+        self.outputs = []
+        arr = (IsacMrefSubset * len(subsets))()
+        for i, s in enumerate(subsets):
+            sbj, ref = s["sbj"], s["ref"]
+            assert sbj.is_cuda and ref.is_cuda and sbj.is_contiguous() and ref.is_contiguous()
+            assert sbj.dtype == torch.float32 and ref.dtype == torch.float32
+            P, D, N = sbj.shape[0], sbj.shape[-1], ref.shape[0]
+            peak = torch.empty((N, 4 * P), dtype=torch.float32, device=sbj.device)
+            d = torch.empty((N, P), dtype=torch.float32, device=sbj.device)
+            self.outputs.append((peak, d))
+            acc = s.get("acc_xy")
+            rx = s.get("shift_rng_x", s.get("shift_rng"))
+            ry = s.get("shift_rng_y", s.get("shift_rng"))
+            arr[i] = IsacMrefSubset(sbj.data_ptr(), ref.data_ptr(),
+                                    acc.data_ptr() if acc is not None else None,
+                                    peak.data_ptr(), d.data_ptr(), P, N, D, s["ring_num"],
+                                    s.get("shift_step", 1.0), rx, ry, P)
+        self.desc = arr
+
+    def run(self, mode: int = BATCH_GRAPH, stream=None):
+        import torch
+        st = (stream or torch.cuda.current_stream()).cuda_stream
+        rc = self.lib.isac_mref_align_batched_device(self.desc, len(self.subsets), mode, st)
+        if rc:
+            check(self.lib)
+            raise GpuAlignError(f"isac_mref_align_batched_device returned {rc}")
+        return self.outputs
+
+    def report(self) -> dict:
+        c = ctypes.c_int
+        g, n, r, nw, b = c(), c(), c(), c(), c()
+        ms = (ctypes.c_float * 3)()
+        self.lib.isac_batched_report(ctypes.byref(g), ctypes.byref(n), ctypes.byref(r),
+                                     ctypes.byref(nw), ctypes.byref(b), ms)
+        return {"groups": g.value, "graph_nodes": n.value, "replayed": bool(r.value),
+                "waves_per_workgroup": nw.value, "persistent_blocks": b.value,
+                "tuning_ms": {"1": ms[0], "2": ms[1], "4": ms[2]}}

@@ -215,3 +215,36 @@ def make_dataset_torch(n_particles: int, n_refs: int, D: int, radius: float, snr
             img = img + sig * torch.randn(img.shape, generator=gen, device=dev)
         sbj[p0:p1] = img
     return sbj, torch.from_numpy(ref).to(dev), Dataset(None, ref, ref_idx, phi, shift, mirror)
+
+
+BATCHED_BOXES = (64, 76, 96, 128)
+
+
+def config4_shapes(n_subsets: int = 64, seed: int = 4):
+    """Shapes of the batched uneven workload: per subset a particle count
+    uniform in 500..5000, a box from {64, 76, 96, 128}, a reference count
+    uniform in 20..200, a shift range of +-2..+-4 (step 1), rings 1..box/2-4 and
+    SNR uniform in 0.05..0.2."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_subsets):
+        D = int(rng.choice(BATCHED_BOXES))
+        out.append({"P": int(rng.integers(500, 5001)), "D": D, "N": int(rng.integers(20, 201)),
+                    "shift_rng": float(rng.integers(2, 5)), "ring_num": D // 2 - 4,
+                    "snr": float(rng.uniform(0.05, 0.2)), "seed": 1000 + 3 * i})
+    return out
+
+
+def make_subsets_torch(shapes, device="cuda"):
+    """Render every subset of ``shapes`` on the device: blob references inside
+    radius box/2-4, particles with random psi, mirror and sub-pixel shifts within
+    the subset's search range.  Returns dicts ready for binding.BatchedMref."""
+    subs = []
+    for sh in shapes:
+        D = sh["D"]
+        sbj, ref, _ = make_dataset_torch(sh["P"], sh["N"], D, float(D // 2 - 4), sh["snr"],
+                                         sh["shift_rng"], False, device, seed_refs=sh["seed"],
+                                         seed_particles=sh["seed"] + 1, seed_noise=sh["seed"] + 2)
+        subs.append({"sbj": sbj, "ref": ref, "ring_num": sh["ring_num"],
+                     "shift_rng": sh["shift_rng"], "acc_xy": None})
+    return subs

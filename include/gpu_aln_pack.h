@@ -184,6 +184,44 @@ int isac_mref_best_mixed_device(const float *d_sbj, const float *d_ref, const fl
                                 float *d_peak_list, float *d_d, int *d_best_ref, float *d_best_val,
                                 float *d_best_peak, int *d_tie, void *stream);
 
+/* No counterpart in the reference (the reference aligns one subset per
+ * multi_ref_align call, cuda/gpu_aln_noref.cu:326-554, driven once per ISAC
+ * subset by bin/sp_isac2_gpu.py:979-1003).  One subset of a batched
+ * multi-reference run; every pointer is a device pointer and the outputs have
+ * the layout of isac_mref_align_device (ring_len is 256). */
+typedef struct IsacMrefSubset {
+    const float *d_sbj;        /* [n_sbj][img_dim][img_dim]                  */
+    const float *d_ref;        /* [n_ref][img_dim][img_dim]                  */
+    const float *d_acc_xy;     /* [n_sbj][2] accumulated shifts, NULL = zero  */
+    float *d_peak_list;        /* [n_ref][4*n_sbj]                           */
+    float *d_d;                /* [n_ref][d_pitch]                           */
+    unsigned int n_sbj, n_ref, img_dim, ring_num;
+    float shift_step, shift_rng_x, shift_rng_y;
+    unsigned int d_pitch;
+} IsacMrefSubset;
+
+#define ISAC_BATCH_PER_SUBSET 0  /* one launch sequence (and synchronisation) per subset */
+#define ISAC_BATCH_GRAPH 1       /* every subset in one captured HIP graph around one
+                                    persistent work-queue kernel per memory group */
+
+/* No counterpart in the reference.  Aligns n_subsets independent subsets of
+ * uneven particle count, box size, reference count, ring count and shift range;
+ * results are bit-identical to one isac_mref_align_device call per subset.  In
+ * ISAC_BATCH_GRAPH mode the tile shape (waves per workgroup) of each memory
+ * group is chosen by timing the candidates on a sample of the group, and the
+ * captured graph is kept and replayed while the descriptor array is unchanged
+ * (same pointers and shapes; the data behind them may change).  Returns after
+ * the work has completed. */
+int isac_mref_align_batched_device(const IsacMrefSubset *subsets, unsigned int n_subsets,
+                                   int mode, void *stream);
+
+/* No counterpart in the reference.  What the last ISAC_BATCH_GRAPH call did:
+ * memory groups, graph nodes, whether the graph was replayed from the cache,
+ * and for the first group the chosen waves per workgroup, the persistent grid
+ * size and the timed sample (ms) for 1, 2 and 4 waves. */
+void isac_batched_report(int *n_groups, int *n_nodes, int *replayed, int *nw, int *blocks,
+                         float *cand_ms3);
+
 /* No counterpart in the reference.  When enabled, the library brackets every
  * launch of its contraction+psi kernel with HIP events on the stream it
  * launches on, and accumulates launch count and total milliseconds. */

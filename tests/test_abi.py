@@ -42,6 +42,42 @@ def test_struct_layout_matches_reference():
     assert binding.AlignParam.shift_x.offset == 8
 
 
+def test_ctypes_mirrors_match_header(tmp_path):
+    """Compile the header with gcc and compare every struct's size and offsets."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    structs = {"AlignConfig": binding.AlignConfig, "AlignParam": binding.AlignParam,
+               "IsacMrefSubset": binding.IsacMrefSubset}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for name, cls in structs.items():
+        lines.append(f'printf("{name} size %zu\\n", sizeof({name}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("{name} {f} %zu\\n", offsetof({name}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for name, cls in structs.items():
+        assert got[(name, "size")] == ctypes.sizeof(cls), name
+        for f, _ in cls._fields_:
+            assert got[(name, f)] == getattr(cls, f).offset, (name, f)
+    assert ctypes.sizeof(binding.IsacMrefSubset) == 72
+
+
+def test_batched_rejects_bad_arguments_without_gpu():
+    lib = binding.load()
+    lib.gpu_clear_error()
+    # no descriptors: nothing to do, but a NULL array with a count is an error
+    rc = lib.isac_mref_align_batched_device(None, 0, binding.BATCH_GRAPH, None)
+    assert rc in (0, 1)   # 1 when no device is present
+    lib.gpu_clear_error()
+
+
 def test_texture_limit_positive():
     lib = binding.load()
     assert lib.get_texture_limit(76) > 1000
