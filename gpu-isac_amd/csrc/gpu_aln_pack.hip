@@ -980,6 +980,10 @@ size_t spec_bytes_per_particle(const AlignConfig *c) {
 
 }  // namespace
 
+namespace isac {
+void report_error(int code, const char *msg) { set_error(code, "%s", msg); }
+}  // namespace isac
+
 // ======================================================================= ABI
 extern "C" {
 

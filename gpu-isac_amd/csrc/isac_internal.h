@@ -66,6 +66,9 @@ hipError_t launch_ccf_mref_queue(const MrefJob *jobs, const int4 *items, const i
                                  hipStream_t st);
 int mref_queue_blocks_per_cu(int nw);   // occupancy of the queue kernel (0 on error)
 
+// Records an error for gpu_last_error() (codes: 1 HIP, 2 argument, 3 state, 4 unsupported).
+void report_error(int code, const char *msg);
+
 // One-time launch attributes (dynamic LDS limit); call before any stream capture.
 hipError_t prepare_launches();
 

@@ -1,0 +1,730 @@
+// Kernels of the pose-proposal network (CDNA4 / gfx950): bf16 weights and
+// activations in NHWC order, fp32 accumulation.
+//
+// The convolutions are implicit GEMMs on v_mfma_f32_16x16x16_bf16 with the
+// weights as the A operand and the activations as B, so that every operand is
+// one 8-byte load per lane and every result one 8-byte store:
+//   A[i = co][k = ci]   lane l: co = l%16, ci = 4(l/16)+v  -> w[co][tap][ci..ci+3]
+//   B[k = ci][j = px]   lane l: px = l%16, ci = 4(l/16)+v  -> x[px + tap offset][ci..ci+3]
+//   D[i = co][j = px]   lane l: co = 4(l/16)+v, px = l%16  -> y[px][co..co+3]
+// A wavefront owns 64 output channels x 64 pixels (16 accumulator blocks); a
+// workgroup is 4 wavefronts along the pixels.  The same kernel computes the
+// input gradient of a layer from the flipped weights.  The weight gradient is
+// a reduction over pixels and runs on the vector ALUs from LDS tiles, split
+// over pixel ranges into partial sums that a second kernel adds in a fixed
+// order.  No reference counterpart: the reference has no learned component.
+#include "isac_internal.h"
+#include "../../include/isac_posenet.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <climits>
+#include <cstdio>
+
+namespace isac {
+namespace {
+
+typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+// fp32 -> bf16, round to nearest even (NaN stays NaN).
+__device__ __forceinline__ unsigned short f2bf(float x) {
+    unsigned u = __float_as_uint(x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+__device__ __forceinline__ uint2 pack4(float a, float b, float c, float d) {
+    return make_uint2((unsigned)f2bf(a) | ((unsigned)f2bf(b) << 16),
+                      (unsigned)f2bf(c) | ((unsigned)f2bf(d) << 16));
+}
+
+__device__ __forceinline__ float4_t mfma_bf16(uint2 a, uint2 b, float4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(short4_t, a),
+                                                      __builtin_bit_cast(short4_t, b), c, 0, 0, 0);
+}
+
+// ------------------------------------------------------------ standardize
+__global__ __launch_bounds__(256) void k_standardize(const float *__restrict__ img, int D,
+                                                     unsigned short *__restrict__ x0) {
+    __shared__ double red[2][4];
+    const int i = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const float *src = img + (size_t)i * D * D;
+    double s = 0.0, q = 0.0;
+    for (int k = tid; k < D * D; k += 256) {
+        const double v = (double)src[k];
+        s += v;
+        q += v * v;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        s += __shfl_xor(s, off, kWave);
+        q += __shfl_xor(q, off, kWave);
+    }
+    if (lane == 0) {
+        red[0][wave] = s;
+        red[1][wave] = q;
+    }
+    __syncthreads();
+    s = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    q = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    const double nn = (double)D * D;
+    const double mu = s / nn;
+    const double sd = sqrt(fmax(q / nn - mu * mu, 0.0));
+    const float fm = (float)mu, fi = (float)(1.0 / (sd + 1e-6));
+    unsigned short *dst = x0 + (size_t)i * D * D;
+    for (int k = tid; k < D * D; k += 256) dst[k] = f2bf((src[k] - fm) * fi);
+}
+
+// ------------------------------------------------------------ conv1 (Cin = 1)
+// One thread per pixel computes 32 channels; weights (bf16-rounded) and bias in LDS.
+__global__ __launch_bounds__(256) void k_conv1(const unsigned short *__restrict__ x0, int n, int H,
+                                               int W, const float *__restrict__ master,
+                                               unsigned short *__restrict__ y) {
+    constexpr int CO = 32;
+    __shared__ float w[9][CO];
+    __shared__ float b[CO];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < CO * 9; e += 256) {
+        const int co = e / 9, t = e % 9;
+        w[t][co] = bf2f(f2bf(master[co * 10 + t]));
+    }
+    if (tid < CO) b[tid] = master[tid * 10 + 9];
+    __syncthreads();
+    const long long P = (long long)blockIdx.x * 256 + tid;
+    if (P >= (long long)n * H * W) return;
+    const int img = (int)(P / ((long long)H * W));
+    const int rem = (int)(P - (long long)img * H * W);
+    const int yy = rem / W, xx = rem % W;
+    float xin[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const int sy = yy + t / 3 - 1, sx = xx + t % 3 - 1;
+        xin[t] = (sy >= 0 && sy < H && sx >= 0 && sx < W)
+                     ? bf2f(x0[((size_t)img * H + sy) * W + sx])
+                     : 0.0f;
+    }
+    uint4 *dst = reinterpret_cast<uint4 *>(y + (size_t)P * CO);
+#pragma unroll
+    for (int c8 = 0; c8 < CO / 8; ++c8) {
+        float o[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int t = 0; t < 9; ++t) acc += w[t][c8 * 8 + e] * xin[t];
+            o[e] = fmaxf(acc + b[c8 * 8 + e], 0.0f);
+        }
+        const uint2 lo = pack4(o[0], o[1], o[2], o[3]), hi = pack4(o[4], o[5], o[6], o[7]);
+        dst[c8] = make_uint4(lo.x, lo.y, hi.x, hi.y);
+    }
+}
+
+// ------------------------------------------------------------ MFMA convolution
+template <int KS, bool OUT_F32>
+__global__ __launch_bounds__(256) void k_conv_mfma(const unsigned short *__restrict__ x, int n,
+                                                   int H, int W, int Cin,
+                                                   const unsigned short *__restrict__ w,
+                                                   const float *__restrict__ bias, int bias_stride,
+                                                   int Cout, int relu, void *__restrict__ yv) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = (lane >> 4) * 4;
+    const long long NP = (long long)n * H * W;
+    const long long pbase = (long long)blockIdx.x * 256 + wave * 64;
+    const int cb0 = blockIdx.y * 64;
+    if (pbase >= NP) return;   // uniform over the wavefront; no barriers below
+    int pimg[4], py[4], px[4];
+    bool pval[4];
+#pragma unroll
+    for (int pb = 0; pb < 4; ++pb) {
+        const long long P = pbase + pb * 16 + li;
+        pval[pb] = P < NP;
+        const long long Pc = pval[pb] ? P : 0;
+        pimg[pb] = (int)(Pc / ((long long)H * W));
+        const int rem = (int)(Pc - (long long)pimg[pb] * H * W);
+        py[pb] = rem / W;
+        px[pb] = rem % W;
+    }
+    bool cval[4];
+    const unsigned short *wrow[4];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+        const int co = cb0 + cb * 16 + li;
+        cval[cb] = cb0 + cb * 16 < Cout;   // uniform: Cout is a multiple of 16
+        wrow[cb] = w + (size_t)min(co, Cout - 1) * KS * KS * Cin + lk;
+    }
+    float4_t acc[4][4];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+        for (int pb = 0; pb < 4; ++pb) acc[cb][pb] = float4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int t = 0; t < KS * KS; ++t) {
+        const int dy = t / KS - KS / 2, dx = t % KS - KS / 2;
+        const unsigned short *src[4];
+#pragma unroll
+        for (int pb = 0; pb < 4; ++pb) {
+            const int sy = py[pb] + dy, sx = px[pb] + dx;
+            const bool ok = pval[pb] && sy >= 0 && sy < H && sx >= 0 && sx < W;
+            src[pb] = ok ? x + (((size_t)pimg[pb] * H + sy) * W + sx) * Cin + lk : nullptr;
+        }
+        for (int c0 = 0; c0 < Cin; c0 += 16) {
+            uint2 a[4], b[4];
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb)
+                a[cb] = cval[cb] ? *reinterpret_cast<const uint2 *>(wrow[cb] + (size_t)t * Cin + c0)
+                                 : make_uint2(0u, 0u);
+#pragma unroll
+            for (int pb = 0; pb < 4; ++pb)
+                b[pb] = src[pb] ? *reinterpret_cast<const uint2 *>(src[pb] + c0) : make_uint2(0u, 0u);
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) {
+                if (!cval[cb]) continue;
+#pragma unroll
+                for (int pb = 0; pb < 4; ++pb) acc[cb][pb] = mfma_bf16(a[cb], b[pb], acc[cb][pb]);
+            }
+        }
+    }
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+        if (!cval[cb]) continue;
+        const int co = cb0 + cb * 16 + lk;
+        float bb[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (bias) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) bb[v] = bias[(size_t)(co + v) * bias_stride];
+        }
+#pragma unroll
+        for (int pb = 0; pb < 4; ++pb) {
+            if (!pval[pb]) continue;
+            const long long P = pbase + pb * 16 + li;
+            float o[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                o[v] = acc[cb][pb][v] + bb[v];
+                if (relu) o[v] = fmaxf(o[v], 0.0f);
+            }
+            if (OUT_F32) {
+                float *y = static_cast<float *>(yv) + (size_t)P * Cout + co;
+                *reinterpret_cast<float4 *>(y) = make_float4(o[0], o[1], o[2], o[3]);
+            } else {
+                unsigned short *y = static_cast<unsigned short *>(yv) + (size_t)P * Cout + co;
+                *reinterpret_cast<uint2 *>(y) = pack4(o[0], o[1], o[2], o[3]);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ pooling
+// One thread per (output pixel, 8 channels).
+__global__ void k_maxpool2(const unsigned short *__restrict__ y, int n, int H, int W, int C,
+                           unsigned short *__restrict__ a) {
+    const int Ho = H / 2, Wo = W / 2, C8 = C / 8;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n * Ho * Wo * C8) return;
+    const int c8 = (int)(idx % C8);
+    long long r = idx / C8;
+    const int xo = (int)(r % Wo);
+    r /= Wo;
+    const int yo = (int)(r % Ho);
+    const int img = (int)(r / Ho);
+    float m[8];
+    unsigned short h[8];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int sy = 2 * yo + q / 2, sx = 2 * xo + q % 2;
+        const uint4 v = *reinterpret_cast<const uint4 *>(y + (((size_t)img * H + sy) * W + sx) * C + c8 * 8);
+        const unsigned vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const unsigned short u = (unsigned short)(vv[e / 2] >> (16 * (e & 1)));
+            const float f = bf2f(u);
+            if (q == 0 || f > m[e]) {
+                m[e] = f;
+                h[e] = u;
+            }
+        }
+    }
+    uint4 o;
+    o.x = h[0] | ((unsigned)h[1] << 16);
+    o.y = h[2] | ((unsigned)h[3] << 16);
+    o.z = h[4] | ((unsigned)h[5] << 16);
+    o.w = h[6] | ((unsigned)h[7] << 16);
+    *reinterpret_cast<uint4 *>(a + (((size_t)img * Ho + yo) * Wo + xo) * C + c8 * 8) = o;
+}
+
+// One thread per (input pixel, 8 channels): the gradient lands on the first
+// maximum of its window when that maximum is positive (ReLU before the pool).
+__global__ void k_maxpool2_bwd(const unsigned short *__restrict__ y,
+                               const unsigned short *__restrict__ da, int n, int H, int W, int C,
+                               unsigned short *__restrict__ dz) {
+    const int Ho = H / 2, Wo = W / 2, C8 = C / 8;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)n * H * W * C8) return;
+    const int c8 = (int)(idx % C8);
+    long long r = idx / C8;
+    const int xi = (int)(r % W);
+    r /= W;
+    const int yi = (int)(r % H);
+    const int img = (int)(r / H);
+    const int yo = yi / 2, xo = xi / 2;
+    unsigned out[4] = {0u, 0u, 0u, 0u};
+    if (yo < Ho && xo < Wo) {
+        const int me = (yi & 1) * 2 + (xi & 1);
+        float m[8];
+        int arg[8];
+        float mine[8];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int sy = 2 * yo + q / 2, sx = 2 * xo + q % 2;
+            const uint4 v = *reinterpret_cast<const uint4 *>(y + (((size_t)img * H + sy) * W + sx) * C + c8 * 8);
+            const unsigned vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const float f = bf2f((unsigned short)(vv[e / 2] >> (16 * (e & 1))));
+                if (q == 0 || f > m[e]) {
+                    m[e] = f;
+                    arg[e] = q;
+                }
+                if (q == me) mine[e] = f;
+            }
+        }
+        const uint4 g = *reinterpret_cast<const uint4 *>(da + (((size_t)img * Ho + yo) * Wo + xo) * C + c8 * 8);
+        const unsigned gg[4] = {g.x, g.y, g.z, g.w};
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const unsigned short u = (unsigned short)(gg[e / 2] >> (16 * (e & 1)));
+            const unsigned short o = (arg[e] == me && mine[e] > 0.0f) ? u : (unsigned short)0;
+            out[e / 2] |= (unsigned)o << (16 * (e & 1));
+        }
+    }
+    *reinterpret_cast<uint4 *>(dz + (((size_t)img * H + yi) * W + xi) * C + c8 * 8) =
+        make_uint4(out[0], out[1], out[2], out[3]);
+}
+
+__global__ void k_relu_bwd(const unsigned short *__restrict__ y, const unsigned short *__restrict__ dy,
+                           size_t count, unsigned short *__restrict__ dz) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    dz[i] = bf2f(y[i]) > 0.0f ? dy[i] : (unsigned short)0;
+}
+
+// ------------------------------------------------------------ weight gradient
+// Tile: 64 output channels x 64 columns k of [K+1] (k = tap*Cin + ci; k == K is
+// the bias column, input 1).  Blocks along z take pixel ranges [z*chunk,
+// (z+1)*chunk); partial sums go to part[z][Cout][K+1].
+__global__ __launch_bounds__(256) void k_wgrad(const unsigned short *__restrict__ x,
+                                               const unsigned short *__restrict__ dz, int n, int H,
+                                               int W, int Cin, int Cout, int KS, long long chunk,
+                                               float *__restrict__ part) {
+    __shared__ float sd[32][64];
+    __shared__ float sx[32][64];
+    const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+    const int K = KS * KS * Cin, K1 = K + 1;
+    const int k0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+    const long long NP = (long long)n * H * W;
+    const long long p0 = (long long)blockIdx.z * chunk;
+    const long long p1 = min(NP, p0 + chunk);
+    float acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
+    for (long long pc = p0; pc < p1; pc += 32) {
+        __syncthreads();
+        for (int e = tid; e < 32 * 64; e += 256) {
+            const int pr = e >> 6, col = e & 63;
+            const long long P = pc + pr;
+            float dv = 0.0f, xv = 0.0f;
+            if (P < p1) {
+                const int co = c0 + col;
+                if (co < Cout) dv = bf2f(dz[(size_t)P * Cout + co]);
+                const int k = k0 + col;
+                if (k == K) {
+                    xv = 1.0f;
+                } else if (k < K) {
+                    const int t = k / Cin, ci = k - t * Cin;
+                    const int img = (int)(P / ((long long)H * W));
+                    const int rem = (int)(P - (long long)img * H * W);
+                    const int sy = rem / W + t / KS - KS / 2, sx2 = rem % W + t % KS - KS / 2;
+                    if (sy >= 0 && sy < H && sx2 >= 0 && sx2 < W)
+                        xv = bf2f(x[(((size_t)img * H + sy) * W + sx2) * Cin + ci]);
+                }
+            }
+            sd[pr][col] = dv;
+            sx[pr][col] = xv;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int pr = 0; pr < 32; ++pr) {
+            const float4 a = *reinterpret_cast<const float4 *>(&sd[pr][ty * 4]);
+            const float4 b = *reinterpret_cast<const float4 *>(&sx[pr][tx * 4]);
+            const float av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] += av[i] * bv[j];
+        }
+    }
+    float *out = part + (size_t)blockIdx.z * Cout * K1;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int co = c0 + ty * 4 + i;
+        if (co >= Cout) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = k0 + tx * 4 + j;
+            if (k < K1) out[(size_t)co * K1 + k] = acc[i][j];
+        }
+    }
+}
+
+__global__ void k_sum_parts(const float *__restrict__ part, int nparts, size_t count,
+                            float *__restrict__ g) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    float s = 0.0f;
+    for (int z = 0; z < nparts; ++z) s += part[(size_t)z * count + i];
+    g[i] = s;
+}
+
+// ------------------------------------------------------------ loss
+// One wavefront per particle.
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_softmax_xent(const float *__restrict__ logits, int n, int ld,
+                                                      int h0, int h1, int h2,
+                                                      const int *__restrict__ labels,
+                                                      float *__restrict__ loss,
+                                                      unsigned short *__restrict__ dlogits) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const float *z = logits + (size_t)i * ld;
+    unsigned short *dz = dlogits + (size_t)i * ld;
+    const int off[3] = {0, h0, h0 + h1}, sz[3] = {h0, h1, h2};
+    const float inv_n = 1.0f / (float)n;
+    float total = 0.0f;
+    for (int h = 0; h < 3; ++h) {
+        const int lab = min(max(labels[(size_t)i * 3 + h], 0), sz[h] - 1);
+        float m = -FLT_MAX;
+        for (int c = lane; c < sz[h]; c += 64) m = fmaxf(m, z[off[h] + c]);
+        m = wave_max(m);
+        float s = 0.0f;
+        for (int c = lane; c < sz[h]; c += 64) s += expf(z[off[h] + c] - m);
+        s = wave_sum(s);
+        const float lse = m + logf(s);
+        total += lse - z[off[h] + lab];
+        for (int c = lane; c < sz[h]; c += 64) {
+            const float pr = expf(z[off[h] + c] - lse);
+            dz[off[h] + c] = f2bf((pr - (c == lab ? 1.0f : 0.0f)) * inv_n);
+        }
+    }
+    for (int c = h0 + h1 + h2 + lane; c < ld; c += 64) dz[c] = 0;
+    if (lane == 0) loss[i] = total;
+}
+
+// ------------------------------------------------------------ optimiser
+__global__ void k_adamw(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                        float *__restrict__ v, const unsigned char *__restrict__ mask, size_t count,
+                        float lr, float b1, float b2, float eps, float wd, float c1, float c2) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    if (mask && !mask[i]) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float mh = mi / c1, vh = vi / c2;
+    p[i] -= lr * (mh / (sqrtf(vh) + eps) + wd * p[i]);
+}
+
+__global__ void k_pack_weights(const float *__restrict__ master, int Cout, int Cin, int KS,
+                               unsigned short *__restrict__ w, unsigned short *__restrict__ wflip) {
+    const int K = KS * KS * Cin;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)Cout * K) return;
+    const int co = (int)(i / K), k = (int)(i % K);
+    const unsigned short h = f2bf(master[(size_t)co * (K + 1) + k]);
+    w[i] = h;
+    if (wflip) {
+        const int t = k / Cin, ci = k % Cin;
+        const int tf = KS * KS - 1 - t;
+        wflip[((size_t)ci * KS * KS + tf) * Cout + co] = h;
+    }
+}
+
+// ------------------------------------------------------------ proposals
+// Wave top-8 of one head: values v[j] at index lane + 64 j (j < J), -inf padded.
+// Returns (in every lane) the k-th best value and index for k < 8, in lane k's
+// registers: top_v / top_i valid in lanes 0..7.
+template <int J>
+__device__ __forceinline__ void head_top8(float (&v)[J], int lane, float &top_v, int &top_i) {
+    top_v = -FLT_MAX;
+    top_i = -1;
+    for (int r = 0; r < 8; ++r) {
+        float bv = -FLT_MAX;
+        int bi = INT_MAX;
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const int idx = lane + 64 * j;
+            if (v[j] > bv || (v[j] == bv && idx < bi)) {
+                bv = v[j];
+                bi = idx;
+            }
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const float ov = __shfl_xor(bv, off, kWave);
+            const int oi = __shfl_xor(bi, off, kWave);
+            if (ov > bv || (ov == bv && oi < bi)) {
+                bv = ov;
+                bi = oi;
+            }
+        }
+        if (lane == r && bi != INT_MAX && bv > -FLT_MAX) {
+            top_v = bv;
+            top_i = bi;
+        }
+#pragma unroll
+        for (int j = 0; j < J; ++j)
+            if (lane + 64 * j == bi) v[j] = -FLT_MAX;
+    }
+}
+
+// One wavefront per particle; heads up to 512 (references), 256 (psi classes)
+// and 64 (shifts) entries.
+__global__ __launch_bounds__(256) void k_proposals(const float *__restrict__ logits, int n, int ld,
+                                                   int n_ref, int n_psi, int n_shift, int k,
+                                                   int4 *__restrict__ prop) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const float *z = logits + (size_t)i * ld;
+    float rv[8], pv[4], sv[1];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) rv[j] = lane + 64 * j < n_ref ? z[lane + 64 * j] : -FLT_MAX;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) pv[j] = lane + 64 * j < 2 * n_psi ? z[n_ref + lane + 64 * j] : -FLT_MAX;
+    sv[0] = lane < n_shift ? z[n_ref + 2 * n_psi + lane] : -FLT_MAX;
+    float tr, tp, ts;
+    int ir, ip, is;
+    head_top8<8>(rv, lane, tr, ir);
+    head_top8<4>(pv, lane, tp, ip);
+    head_top8<1>(sv, lane, ts, is);
+    // lane l scores triples (ref rank l/8, psi rank l%8, shift rank c), c = 0..7
+    const float r_l = __shfl(tr, lane >> 3, kWave), p_l = __shfl(tp, lane & 7, kWave);
+    const bool ab_ok = __shfl(ir, lane >> 3, kWave) >= 0 && __shfl(ip, lane & 7, kWave) >= 0;
+    float sc[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const bool ok = ab_ok && __shfl(is, c, kWave) >= 0;
+        sc[c] = ok ? r_l + p_l + __shfl(ts, c, kWave) : -INFINITY;
+    }
+    for (int r = 0; r < k; ++r) {
+        float bv = -FLT_MAX;
+        int bi = INT_MAX;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int idx = lane * 8 + c;   // (ref rank, psi rank, shift rank) in lexicographic order
+            if (sc[c] > bv || (sc[c] == bv && idx < bi)) {
+                bv = sc[c];
+                bi = idx;
+            }
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const float ov = __shfl_xor(bv, off, kWave);
+            const int oi = __shfl_xor(bi, off, kWave);
+            if (ov > bv || (ov == bv && oi < bi)) {
+                bv = ov;
+                bi = oi;
+            }
+        }
+        if (bi == INT_MAX) {   // fewer valid triples than k
+            if (lane == 0) prop[(size_t)i * k + r] = make_int4(-1, 0, 0, 0);
+            continue;
+        }
+        const int a = bi >> 6, b = (bi >> 3) & 7, c = bi & 7;
+        const int ref = __shfl(ir, a, kWave), pc = __shfl(ip, b, kWave), sh = __shfl(is, c, kWave);
+        if (lane == 0) prop[(size_t)i * k + r] = make_int4(ref, pc / n_psi, pc % n_psi, sh);
+        if (lane == (bi >> 3)) sc[c] = -INFINITY;
+    }
+}
+
+int fail(int code, const char *msg) {
+    report_error(code, msg);
+    return code;
+}
+
+int launched(const char *what) {
+    const hipError_t e = hipGetLastError();
+    if (e == hipSuccess) return 0;
+    char buf[256];
+    snprintf(buf, sizeof(buf), "%s launch failed: %s", what, hipGetErrorString(e));
+    report_error(1, buf);
+    return 1;
+}
+
+unsigned blocks_for(size_t count, int threads) { return (unsigned)((count + threads - 1) / threads); }
+
+}  // namespace
+}  // namespace isac
+
+using namespace isac;
+
+extern "C" {
+
+int isac_nn_standardize(const float *img, int n, int D, uint16_t *x0, void *stream) {
+    if (n < 0 || D < 1 || (n > 0 && (!img || !x0))) return fail(2, "isac_nn_standardize: bad arguments");
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_standardize, dim3(n), dim3(256), 0, (hipStream_t)stream, img, D, x0);
+    return launched("isac_nn_standardize");
+}
+
+int isac_nn_conv1_fwd(const uint16_t *x0, int n, int H, int W, const float *master, int Cout,
+                      uint16_t *y, void *stream) {
+    if (Cout != 32 || n < 0 || H < 1 || W < 1 || (n > 0 && (!x0 || !master || !y)))
+        return fail(2, "isac_nn_conv1_fwd: bad arguments (Cout must be 32)");
+    const size_t np = (size_t)n * H * W;
+    if (np == 0) return 0;
+    hipLaunchKernelGGL(k_conv1, dim3(blocks_for(np, 256)), dim3(256), 0, (hipStream_t)stream, x0, n, H,
+                       W, master, y);
+    return launched("isac_nn_conv1_fwd");
+}
+
+int isac_nn_conv_fwd(const uint16_t *x, int n, int H, int W, int Cin, const uint16_t *w,
+                     const float *bias, int bias_stride, int Cout, int ks, int relu, int out_f32,
+                     void *y, void *stream) {
+    if ((ks != 1 && ks != 3) || Cin < 16 || Cin % 16 || Cout < 16 || Cout % 16 || n < 0 || H < 1 ||
+        W < 1 || (n > 0 && (!x || !w || !y)))
+        return fail(2, "isac_nn_conv_fwd: bad arguments (Cin, Cout multiples of 16; ks 1 or 3)");
+    const size_t np = (size_t)n * H * W;
+    if (np == 0) return 0;
+    dim3 grid(blocks_for(np, 256), (unsigned)((Cout + 63) / 64));
+    hipStream_t st = (hipStream_t)stream;
+    if (ks == 3 && !out_f32)
+        hipLaunchKernelGGL((k_conv_mfma<3, false>), grid, dim3(256), 0, st, x, n, H, W, Cin, w, bias,
+                           bias_stride, Cout, relu, y);
+    else if (ks == 3)
+        hipLaunchKernelGGL((k_conv_mfma<3, true>), grid, dim3(256), 0, st, x, n, H, W, Cin, w, bias,
+                           bias_stride, Cout, relu, y);
+    else if (!out_f32)
+        hipLaunchKernelGGL((k_conv_mfma<1, false>), grid, dim3(256), 0, st, x, n, H, W, Cin, w, bias,
+                           bias_stride, Cout, relu, y);
+    else
+        hipLaunchKernelGGL((k_conv_mfma<1, true>), grid, dim3(256), 0, st, x, n, H, W, Cin, w, bias,
+                           bias_stride, Cout, relu, y);
+    return launched("isac_nn_conv_fwd");
+}
+
+int isac_nn_maxpool2(const uint16_t *y, int n, int H, int W, int C, uint16_t *a, void *stream) {
+    if (C % 8 || C < 8 || n < 0 || H < 2 || W < 2 || (n > 0 && (!y || !a)))
+        return fail(2, "isac_nn_maxpool2: bad arguments");
+    const size_t cnt = (size_t)n * (H / 2) * (W / 2) * (C / 8);
+    if (cnt == 0) return 0;
+    hipLaunchKernelGGL(k_maxpool2, dim3(blocks_for(cnt, 256)), dim3(256), 0, (hipStream_t)stream, y, n,
+                       H, W, C, a);
+    return launched("isac_nn_maxpool2");
+}
+
+int isac_nn_maxpool2_bwd(const uint16_t *y, const uint16_t *da, int n, int H, int W, int C,
+                         uint16_t *dz, void *stream) {
+    if (C % 8 || C < 8 || n < 0 || H < 2 || W < 2 || (n > 0 && (!y || !da || !dz)))
+        return fail(2, "isac_nn_maxpool2_bwd: bad arguments");
+    const size_t cnt = (size_t)n * H * W * (C / 8);
+    if (cnt == 0) return 0;
+    hipLaunchKernelGGL(k_maxpool2_bwd, dim3(blocks_for(cnt, 256)), dim3(256), 0, (hipStream_t)stream, y,
+                       da, n, H, W, C, dz);
+    return launched("isac_nn_maxpool2_bwd");
+}
+
+int isac_nn_relu_bwd(const uint16_t *y, const uint16_t *dy, size_t count, uint16_t *dz, void *stream) {
+    if (count && (!y || !dy || !dz)) return fail(2, "isac_nn_relu_bwd: NULL pointer");
+    if (!count) return 0;
+    hipLaunchKernelGGL(k_relu_bwd, dim3(blocks_for(count, 256)), dim3(256), 0, (hipStream_t)stream, y, dy,
+                       count, dz);
+    return launched("isac_nn_relu_bwd");
+}
+
+int isac_nn_conv_wgrad(const uint16_t *x, const uint16_t *dz, int n, int H, int W, int Cin,
+                       int Cout, int ks, float *g, float *part, size_t part_bytes, void *stream) {
+    if ((ks != 1 && ks != 3) || Cin < 1 || Cout < 1 || n < 1 || H < 1 || W < 1 || !x || !dz || !g || !part)
+        return fail(2, "isac_nn_conv_wgrad: bad arguments");
+    const int K1 = ks * ks * Cin + 1;
+    const size_t per = (size_t)Cout * K1 * sizeof(float);
+    const long long NP = (long long)n * H * W;
+    long long nparts = (long long)(part_bytes / per);
+    nparts = std::min<long long>(nparts, std::max<long long>(1, NP / 512));
+    nparts = std::min<long long>(nparts, 512);
+    if (nparts < 1) return fail(2, "isac_nn_conv_wgrad: scratch smaller than one partial sum");
+    const long long chunk = ((NP + nparts - 1) / nparts + 31) / 32 * 32;
+    nparts = (NP + chunk - 1) / chunk;
+    hipStream_t st = (hipStream_t)stream;
+    dim3 grid((unsigned)((K1 + 63) / 64), (unsigned)((Cout + 63) / 64), (unsigned)nparts);
+    hipLaunchKernelGGL(k_wgrad, grid, dim3(256), 0, st, x, dz, n, H, W, Cin, Cout, ks, chunk, part);
+    if (int e = launched("isac_nn_conv_wgrad")) return e;
+    const size_t cnt = (size_t)Cout * K1;
+    hipLaunchKernelGGL(k_sum_parts, dim3(blocks_for(cnt, 256)), dim3(256), 0, st, part, (int)nparts, cnt, g);
+    return launched("isac_nn_conv_wgrad (sum)");
+}
+
+int isac_nn_softmax_xent(const float *logits, int n, int ld, int h0, int h1, int h2,
+                         const int *labels, float *loss, uint16_t *dlogits, void *stream) {
+    if (n < 0 || h0 < 1 || h1 < 1 || h2 < 1 || ld < h0 + h1 + h2 ||
+        (n > 0 && (!logits || !labels || !loss || !dlogits)))
+        return fail(2, "isac_nn_softmax_xent: bad arguments");
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_softmax_xent, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, (hipStream_t)stream,
+                       logits, n, ld, h0, h1, h2, labels, loss, dlogits);
+    return launched("isac_nn_softmax_xent");
+}
+
+int isac_nn_adamw(float *p, const float *g, float *m, float *v, const uint8_t *mask, size_t count,
+                  float lr, float beta1, float beta2, float eps, float wd, int t, void *stream) {
+    if (t < 1 || (count && (!p || !g || !m || !v))) return fail(2, "isac_nn_adamw: bad arguments");
+    if (!count) return 0;
+    const float c1 = 1.0f - powf(beta1, (float)t), c2 = 1.0f - powf(beta2, (float)t);
+    hipLaunchKernelGGL(k_adamw, dim3(blocks_for(count, 256)), dim3(256), 0, (hipStream_t)stream, p, g, m,
+                       v, mask, count, lr, beta1, beta2, eps, wd, c1, c2);
+    return launched("isac_nn_adamw");
+}
+
+int isac_nn_pack_weights(const float *master, int Cout, int Cin, int ks, uint16_t *w,
+                         uint16_t *wflip, void *stream) {
+    if (Cout < 1 || Cin < 1 || (ks != 1 && ks != 3) || !master || !w)
+        return fail(2, "isac_nn_pack_weights: bad arguments");
+    const size_t cnt = (size_t)Cout * ks * ks * Cin;
+    hipLaunchKernelGGL(k_pack_weights, dim3(blocks_for(cnt, 256)), dim3(256), 0, (hipStream_t)stream,
+                       master, Cout, Cin, ks, w, wflip);
+    return launched("isac_nn_pack_weights");
+}
+
+int isac_nn_proposals(const float *logits, int n, int ld, int n_ref, int n_psi, int n_shift, int k,
+                      int *prop, void *stream) {
+    if (n < 0 || k < 1 || k > 8 || n_ref < 1 || n_ref > 512 || n_psi < 1 || 2 * n_psi > 256 ||
+        n_shift < 1 || n_shift > 64 || ld < n_ref + 2 * n_psi + n_shift ||
+        (n > 0 && (!logits || !prop)))
+        return fail(2, "isac_nn_proposals: bad arguments (refs <= 512, psi classes <= 256, shifts <= 64, k <= 8)");
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_proposals, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, (hipStream_t)stream,
+                       logits, n, ld, n_ref, n_psi, n_shift, k, reinterpret_cast<int4 *>(prop));
+    return launched("isac_nn_proposals");
+}
+
+}  // extern "C"
