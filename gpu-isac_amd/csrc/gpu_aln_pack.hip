@@ -128,6 +128,7 @@ struct Workspace {
     DevBuf peak, dvals, acc;
     DevBuf mix_p, mix_pn, mix_r, mix_rn;   // 16-bit MFMA operands and row norms
     DevBuf cand, rec_c, best, second, tie, val64;   // candidate re-scoring
+    DevBuf items, smask, fb_idx, fb_count, fb_img, fb_acc;   // narrowed search + fallback
     std::vector<float2> shifts_uploaded;
     BatchWs batch;
     void release() {
@@ -135,7 +136,7 @@ struct Workspace {
         DevBuf *all[] = {&tmpl, &twiddle, &zero_shift, &shifts_i, &shifts_f, &spec, &norm,
                          &rspec, &rnorm, &rsum0, &rec, &peak, &dvals, &acc,
                          &mix_p, &mix_pn, &mix_r, &mix_rn, &cand, &rec_c, &best, &second,
-                         &tie, &val64};
+                         &tie, &val64, &items, &smask, &fb_idx, &fb_count, &fb_img, &fb_acc};
         for (DevBuf *b : all) b->release();
         tmpl_rings = 0;
         shifts_uploaded.clear();
@@ -440,6 +441,99 @@ int run_mref_mixed(const float *d_sbj, const float *d_ref, ShiftView acc, int P,
         }
     }
     HIP_OK(hipStreamSynchronize(st));
+    return 0;
+}
+
+// Narrowed search around proposals with the exhaustive search as fallback.
+// Particle p's proposals d_prop[p][K] = {ref, mirror, psi bin, shift index}; each
+// is searched over the +-hw shift window and the psi samples within psi_half
+// degrees of the bin centre, with the exhaustive kernels' arithmetic.  Particles
+// whose best value is below `threshold` are searched exhaustively.
+int run_proposal(const float *d_sbj, const float *d_ref, ShiftView acc, int P, int N, int D, int R,
+                 const std::vector<float2> &shifts, int nx, int ny, const int *d_prop, int K,
+                 int n_bins, float psi_half, int hw, float threshold, int *best_ref,
+                 float *best_val, float *best_peak, int *fallback, unsigned *n_fb_out,
+                 hipStream_t st) {
+    const int S = (int)shifts.size();
+    if (int e = check_shape(D, R, kRingLen, shifts.size())) return e;
+    if (S > 64 || nx * ny != S) {
+        set_error(kErrUnsupported, "narrowed search needs a shift grid of at most 64 shifts (got %d)", S);
+        return kErrUnsupported;
+    }
+    if (n_fb_out) *n_fb_out = 0;
+    if (P <= 0 || N <= 0) return 0;
+    if (int e = ensure_template(R)) return e;
+    if (int e = ensure_shifts(shifts)) return e;
+    if (int e = ref_spectra(d_ref, N, D, R, st)) return e;
+    const size_t spec_row = (size_t)S * R * kSlots * sizeof(float2);
+    const size_t per = spec_row + (size_t)S * sizeof(float2) +
+                       (size_t)K * (sizeof(isac::PropItem) + sizeof(PeakRec)) + sizeof(uint64_t);
+    const int pb = batch_for(per, (size_t)P * 2 * sizeof(int), 0.9f, P);
+    HIP_OK(ws.spec.reserve((size_t)pb * spec_row));
+    HIP_OK(ws.norm.reserve((size_t)pb * S * sizeof(float2)));
+    HIP_OK(ws.items.reserve((size_t)pb * K * sizeof(isac::PropItem)));
+    HIP_OK(ws.rec.reserve((size_t)pb * K * sizeof(PeakRec)));
+    HIP_OK(ws.smask.reserve((size_t)pb * sizeof(uint64_t)));
+    HIP_OK(ws.fb_idx.reserve((size_t)P * sizeof(int)));
+    HIP_OK(ws.fb_count.reserve(sizeof(int)));
+    for (int p0 = 0; p0 < P; p0 += pb) {
+        const int n = std::min(pb, P - p0);
+        ShiftView a = acc;
+        if (a.base) a.base += (size_t)p0 * a.stride;
+        HIP_OK(isac::launch_prop_items(d_prop + (size_t)p0 * K * 4, n, K, N, nx, ny, hw, n_bins,
+                                       psi_half, ws.items.as<isac::PropItem>(),
+                                       ws.smask.as<unsigned long long>(), st));
+        HIP_OK(isac::launch_polar_spectra_masked(d_sbj + (size_t)p0 * D * D, n, D, a,
+                                                 ws.shifts_i.as<int2>(), S, ws.tmpl.as<float2>(), R,
+                                                 ws.twiddle.as<float2>(), ws.spec.as<float2>(),
+                                                 ws.norm.as<float2>(),
+                                                 ws.smask.as<unsigned long long>(), st));
+        {
+            ProfScope ps(g.prof, st);
+            HIP_OK(isac::launch_ccf_prop(ws.spec.as<float2>(), ws.norm.as<float2>(),
+                                         ws.rspec.as<float2>(), ws.rsum0.as<float>(),
+                                         ws.items.as<isac::PropItem>(), K, n, S, R, N, nx, ny, hw,
+                                         ws.twiddle.as<float2>(), ws.rec.as<PeakRec>(), st));
+            if (g.prof) { g.prof_ccf_ms += ps.finish(); g.prof_ccf_n++; }
+        }
+        HIP_OK(isac::launch_best_prop(ws.rec.as<PeakRec>(), ws.items.as<isac::PropItem>(), n, K, a,
+                                      ws.shifts_f.as<float2>(), S, threshold, p0, best_ref, best_val,
+                                      best_peak, fallback, st));
+    }
+    // exhaustive fallback for the flagged particles
+    HIP_OK(isac::launch_compact(fallback, P, ws.fb_idx.as<int>(), ws.fb_count.as<int>(), st));
+    int n_fb = 0;
+    HIP_OK(hipMemcpyAsync(&n_fb, ws.fb_count.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (n_fb > 0) {
+        const size_t per_fb = (size_t)D * D * sizeof(float) + sizeof(float2) + spec_row +
+                              (size_t)S * sizeof(float2) + (size_t)N * sizeof(PeakRec);
+        const int fb = batch_for(per_fb, 0, 0.9f, n_fb);
+        HIP_OK(ws.fb_img.reserve((size_t)fb * D * D * sizeof(float)));
+        HIP_OK(ws.fb_acc.reserve((size_t)fb * sizeof(float2)));
+        HIP_OK(ws.spec.reserve((size_t)fb * spec_row));
+        HIP_OK(ws.norm.reserve((size_t)fb * S * sizeof(float2)));
+        HIP_OK(ws.rec.reserve((size_t)fb * N * sizeof(PeakRec)));
+        for (int j0 = 0; j0 < n_fb; j0 += fb) {
+            const int m = std::min(fb, n_fb - j0);
+            const int *idx = ws.fb_idx.as<int>() + j0;
+            HIP_OK(isac::launch_gather(d_sbj, D, acc, idx, m, ws.fb_img.as<float>(),
+                                       ws.fb_acc.as<float2>(), st));
+            HIP_OK(isac::launch_polar_spectra(ws.fb_img.as<float>(), m, D,
+                                              ShiftView{ws.fb_acc.as<float>(), 2},
+                                              ws.shifts_i.as<int2>(), S, ws.tmpl.as<float2>(), R,
+                                              ws.twiddle.as<float2>(), ws.spec.as<float2>(),
+                                              ws.norm.as<float2>(), st));
+            HIP_OK(isac::launch_ccf_mref(ws.spec.as<float2>(), ws.norm.as<float2>(),
+                                         ws.rspec.as<float2>(), ws.rsum0.as<float>(), m, S, R, N,
+                                         ws.twiddle.as<float2>(), ws.rec.as<PeakRec>(), st));
+            HIP_OK(isac::launch_best_refs_scatter(ws.rec.as<PeakRec>(), m, N, ws.fb_acc.as<float2>(),
+                                                  ws.shifts_f.as<float2>(), S, idx, best_ref,
+                                                  best_val, best_peak, st));
+        }
+        HIP_OK(hipStreamSynchronize(st));
+    }
+    if (n_fb_out) *n_fb_out = (unsigned)n_fb;
     return 0;
 }
 
@@ -1163,6 +1257,37 @@ int isac_mref_best_mixed_device(const float *d_sbj, const float *d_ref, const fl
     return run_mref_mixed(d_sbj, d_ref, acc, (int)n_sbj, (int)n_ref, (int)img_dim, (int)ring_num,
                           sh, bf16 != 0, d_peak_list, 4 * (int)n_sbj, d_d, (int)n_sbj, 0.9f, st,
                           &bo);
+}
+
+int isac_mref_proposal_device(const float *d_sbj, const float *d_ref, const float *d_acc_xy,
+                              unsigned int n_sbj, unsigned int n_ref, unsigned int img_dim,
+                              unsigned int ring_num, unsigned int ring_len, float shift_step,
+                              float shift_rng_x, float shift_rng_y, const int *d_prop,
+                              unsigned int n_prop, unsigned int n_psi_bins, float psi_half_deg,
+                              unsigned int shift_half, float threshold, int *d_best_ref,
+                              float *d_best_val, float *d_best_peak, int *d_fallback,
+                              unsigned int *n_fallback, void *stream) {
+    if (int e = ensure_device_ws(-1)) return e;
+    std::vector<float2> sh = make_shift_grid(shift_rng_x, shift_rng_y, shift_step);
+    if (int e = check_shape((int)img_dim, (int)ring_num, (int)ring_len, sh.size())) return e;
+    if (n_fallback) *n_fallback = 0;
+    if (n_sbj == 0 || n_ref == 0) return 0;
+    if (!d_sbj || !d_ref || !d_prop || !d_best_ref || !d_best_val || !d_best_peak || !d_fallback ||
+        n_prop < 1 || n_prop > (unsigned)isac::kMaxProposals || n_psi_bins < 1 || n_psi_bins > 256 ||
+        !(psi_half_deg > 0.0f && psi_half_deg <= 180.0f) || shift_half > 2 ||
+        n_sbj > (1u << 30) || n_ref > (1u << 20)) {
+        set_error(kErrArg, "isac_mref_proposal_device: bad arguments (1..8 proposals, psi bins "
+                           "1..256, 0 < psi_half <= 180, shift_half <= 2)");
+        return kErrArg;
+    }
+    int ny = 0;
+    while (ny < (int)sh.size() && sh[ny].x == sh[0].x) ++ny;
+    const int nx = ny ? (int)sh.size() / ny : 0;
+    hipStream_t st = stream ? (hipStream_t)stream : ws.stream;
+    return run_proposal(d_sbj, d_ref, ShiftView{d_acc_xy, 2}, (int)n_sbj, (int)n_ref, (int)img_dim,
+                        (int)ring_num, sh, nx, ny, d_prop, (int)n_prop, (int)n_psi_bins, psi_half_deg,
+                        (int)shift_half, threshold, d_best_ref, d_best_val, d_best_peak, d_fallback,
+                        n_fallback, st);
 }
 
 int isac_mref_align_batched_device(const IsacMrefSubset *subsets, unsigned int n_subsets,

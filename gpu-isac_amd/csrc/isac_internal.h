@@ -37,6 +37,16 @@ struct PeakRec {
 };
 static_assert(sizeof(PeakRec) == 48, "PeakRec layout");
 
+// Narrowed-search work item: a reference, the centre of a shift window and the
+// (mirror, psi sample) mask, bit m * 256 + t.  ref < 0: empty.
+constexpr int kMaxProposals = 8;
+struct PropItem {
+    int ref;
+    int s;
+    unsigned mask[16];
+};
+static_assert(sizeof(PropItem) == 72, "PropItem layout");
+
 // Strided view of per-image accumulated shifts: x at base[i*stride], y at +1.
 // float2 arrays use stride 2; DevParam arrays use base = &p[0].shift_x, stride 6.
 struct ShiftView {
@@ -65,6 +75,36 @@ hipError_t launch_ccf_mref_queue(const MrefJob *jobs, const int4 *items, const i
                                  int *counters, int nw, int blocks, const float2 *twiddle,
                                  hipStream_t st);
 int mref_queue_blocks_per_cu(int nw);   // occupancy of the queue kernel (0 on error)
+
+// Polar spectra of the shifts marked in smask[img] only (n_shift <= 64).
+hipError_t launch_polar_spectra_masked(const float *imgs, int n_img, int img_dim, ShiftView acc,
+                                       const int2 *shift_i, int n_shift, const float2 *tmpl,
+                                       int n_ring, const float2 *twiddle, float2 *spec,
+                                       float2 *norm, const unsigned long long *smask,
+                                       hipStream_t st);
+
+// Narrowed search: prop [n_img][K] int4 (ref, mirror, psi bin, shift) -> items
+// [n_img][K] and shift masks; shift grid nx x ny (x-major), window +-hw shifts.
+hipError_t launch_prop_items(const int *prop, int n_img, int K, int N, int nx, int ny, int hw,
+                             int n_bins, float psi_half, PropItem *items,
+                             unsigned long long *smask, hipStream_t st);
+hipError_t launch_ccf_prop(const float2 *spec, const float2 *norm, const float2 *rspec,
+                           const float *rsum0, const PropItem *items, int K, int n_img, int n_shift,
+                           int n_ring, int n_ref, int nx, int ny, int hw, const float2 *twiddle,
+                           PeakRec *rec, hipStream_t st);
+hipError_t launch_best_prop(const PeakRec *rec, const PropItem *items, int n_img, int K,
+                            ShiftView acc, const float2 *shifts_f, int n_shift, float threshold,
+                            int p_off, int *best_ref, float *best_val, float *best_peak,
+                            int *fallback, hipStream_t st);
+// Exhaustive fallback: compact flagged particles, gather them, scatter the best
+// reference of each back to its index.
+hipError_t launch_compact(const int *flag, int n, int *idx, int *count, hipStream_t st);
+hipError_t launch_gather(const float *imgs, int D, ShiftView acc, const int *idx, int n, float *out,
+                         float2 *out_acc, hipStream_t st);
+hipError_t launch_best_refs_scatter(const PeakRec *rec, int n, int N, const float2 *acc,
+                                    const float2 *shifts_f, int n_shift, const int *idx,
+                                    int *best_ref, float *best_val, float *best_peak,
+                                    hipStream_t st);
 
 // Records an error for gpu_last_error() (codes: 1 HIP, 2 argument, 3 state, 4 unsupported).
 void report_error(int code, const char *msg);

@@ -133,11 +133,17 @@ __device__ __forceinline__ void wave_argmax(float &v, int &t) {
 // A wavefront samples two rings at a time (ring a real part, ring b imaginary
 // part), runs one 256-point FFT and unpacks both real spectra.  The normalisation
 // sums (radius-weighted sum and sum of squares) accumulate in double.
+// MASKED: only the shifts whose bit is set in smask[p] (S <= 64) are computed;
+// the other rows of spec / norm are left as they were.
+template <bool MASKED>
 __global__ __launch_bounds__(256) void k_polar_spectra(
     const float *__restrict__ imgs, int D, ShiftView acc, const int2 *__restrict__ shift_i,
     int S, const float2 *__restrict__ tmpl, int R, const float2 *__restrict__ twg,
-    float2 *__restrict__ spec, float2 *__restrict__ norm) {
+    float2 *__restrict__ spec, float2 *__restrict__ norm,
+    const unsigned long long *__restrict__ smask) {
     extern __shared__ float4 smem4[];
+    __shared__ int slist[64];
+    __shared__ int scnt;
     float2 *tw = reinterpret_cast<float2 *>(smem4);
     float2 *fb = tw + 256;
     float *img = reinterpret_cast<float *>(fb + 4 * 512);
@@ -146,13 +152,22 @@ __global__ __launch_bounds__(256) void k_polar_spectra(
     const float *src = imgs + (size_t)p * D * D;
     for (int i = tid; i < D * D; i += 256) img[i] = src[i];
     tw[tid] = twg[tid];
+    if (MASKED && tid == 0) {
+        const unsigned long long mk = smask[p];
+        int c = 0;
+        for (int b = 0; b < S && b < 64; ++b)
+            if ((mk >> b) & 1ull) slist[c++] = b;
+        scnt = c;
+    }
     __syncthreads();
+    const int n_rows = MASKED ? scnt : S;
     float ax = 0.0f, ay = 0.0f;
     if (acc.base) {
         ax = acc.base[(size_t)p * acc.stride];
         ay = acc.base[(size_t)p * acc.stride + 1];
     }
-    for (int s = wave; s < S; s += 4) {
+    for (int j = wave; j < n_rows; j += 4) {
+    const int s = MASKED ? slist[j] : j;
     const int2 si = shift_i[s];
     const float cx = (float)(D / 2 + si.x) + ax;
     const float cy = (float)(D / 2 + si.y) + ay;
@@ -249,11 +264,15 @@ __global__ __launch_bounds__(128) void k_ref_finalize(float2 *__restrict__ rspec
 // samples around it (nb, LDS) are updated; the 6-step wave argmax only runs
 // when some lane beats the running maximum (rows arrive in increasing shift
 // order, so an equal value can never displace the earlier maximum).
-template <bool SINGLE>
+//
+// MASKED: only samples whose bit (m * 4 + q) is set in `allow` (lane's samples
+// lane + 64 q of mirror m) take part in the maximum; the seven neighbours stay
+// the unmasked CCF values.
+template <bool SINGLE, bool MASKED = false>
 __device__ __forceinline__ void psi_update(const float (&Q)[4], const float (&P)[4], float2 nm,
                                            float rs0, int s, int lane, float2 *b0, float2 *b1,
                                            const float2 *tw, float *zb, float (*nb)[8],
-                                           float (&bv)[2], int (&bf)[2]) {
+                                           float (&bv)[2], int (&bf)[2], unsigned allow = 0xffu) {
     const int pl = (64 - lane) & 63;
     const float gs = nm.y;
     float A = P[0] + P[1], B = P[3] - P[2], Cc = P[0] - P[1], Dd = -(P[2] + P[3]);
@@ -282,10 +301,12 @@ __device__ __forceinline__ void psi_update(const float (&Q)[4], const float (&P)
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
         float bvl = m ? v[0].y : v[0].x;
+        if (MASKED && !((allow >> (m * 4)) & 1u)) bvl = -INFINITY;
         int bt = lane;
 #pragma unroll
         for (int q = 1; q < 4; ++q) {
-            const float x = m ? v[q].y : v[q].x;
+            float x = m ? v[q].y : v[q].x;
+            if (MASKED && !((allow >> (m * 4 + q)) & 1u)) x = -INFINITY;
             if (x > bvl) {
                 bvl = x;
                 bt = lane + 64 * q;
@@ -755,6 +776,242 @@ __device__ __forceinline__ float wizard_angle(const PeakRec &r) {
     return angle;
 }
 
+// ------------------------------------------------------------------ narrowed search
+// Proposals (ref, mirror, psi bin, shift) -> work items.  Proposals with the
+// same (ref, shift) share one item whose mask is the union of their (mirror,
+// psi) windows: bit m*256 + t for the psi samples t within psi_half degrees of
+// the bin centre (b + 0.5) * 360 / n_bins.  smask[p] marks every shift inside
+// the (2 hw + 1)^2 windows (for the masked polar pass).  Invalid proposals
+// (ref or shift out of range) are dropped.  One thread per particle.
+__global__ void k_prop_items(const int4 *__restrict__ prop, int n_img, int K, int N, int nx,
+                             int ny, int hw, int n_bins, float psi_half, PropItem *__restrict__ items,
+                             unsigned long long *__restrict__ smask) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_img) return;
+    PropItem it[kMaxProposals];
+    int cnt = 0;
+    unsigned long long mk = 0ull;
+    const double step = 360.0 / (double)kRingLen, bw = 360.0 / (double)n_bins;
+    for (int k = 0; k < K; ++k) {
+        const int4 q = prop[(size_t)p * K + k];
+        const int ref = q.x, m = q.y, b = q.z, sh = q.w;
+        if (ref < 0 || ref >= N || m < 0 || m > 1 || b < 0 || b >= n_bins || sh < 0 || sh >= nx * ny)
+            continue;
+        int j = 0;
+        while (j < cnt && !(it[j].ref == ref && it[j].s == sh)) ++j;
+        if (j == cnt) {
+            it[j].ref = ref;
+            it[j].s = sh;
+            for (int w = 0; w < 16; ++w) it[j].mask[w] = 0u;
+            ++cnt;
+        }
+        const double c = ((double)b + 0.5) * bw;
+        const int t0 = (int)ceil((c - (double)psi_half) / step);
+        const int t1 = (int)floor((c + (double)psi_half) / step);
+        for (int t = t0; t <= t1 && t < t0 + kRingLen; ++t) {
+            const int tt = ((t % kRingLen) + kRingLen) % kRingLen;
+            const int bit = m * kRingLen + tt;
+            it[j].mask[bit >> 5] |= 1u << (bit & 31);
+        }
+        const int ix0 = sh / ny, iy0 = sh % ny;
+        for (int ix = max(0, ix0 - hw); ix <= min(nx - 1, ix0 + hw); ++ix)
+            for (int iy = max(0, iy0 - hw); iy <= min(ny - 1, iy0 + hw); ++iy)
+                mk |= 1ull << (ix * ny + iy);
+    }
+    for (int j = 0; j < K; ++j) {
+        PropItem o;
+        if (j < cnt) {
+            o = it[j];
+        } else {
+            o.ref = -1;
+            o.s = 0;
+            for (int w = 0; w < 16; ++w) o.mask[w] = 0u;
+        }
+        items[(size_t)p * K + j] = o;
+    }
+    smask[p] = mk;
+}
+
+// One wavefront per item: the shifts of the item's (2 hw + 1)^2 window (in
+// increasing grid order) three at a time against its reference, with exactly
+// the arithmetic of the exhaustive kernels (ring contraction, psi FFT), the
+// maximum taken over the item's (mirror, psi) mask only.
+__global__ __launch_bounds__(256) void k_ccf_prop(
+    const float2 *__restrict__ spec, const float2 *__restrict__ norm,
+    const float2 *__restrict__ rspec, const float *__restrict__ rsum0,
+    const PropItem *__restrict__ items, int K, int n_items, int S, int R, int N, int nx, int ny,
+    int hw, const float2 *__restrict__ twg, PeakRec *__restrict__ rec) {
+    __shared__ float2 tw[256];
+    __shared__ float2 fbuf[4][512];
+    __shared__ float zbuf[4][256];
+    __shared__ float nbuf[4][2][8];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    tw[tid] = twg[tid];
+    __syncthreads();
+    const int item = blockIdx.x * 4 + wave;
+    if (item >= n_items) return;
+    const int p = item / K;
+    const PropItem &it = items[item];
+    float bv[2];
+    int bf[2];
+    bv[0] = bv[1] = -3.40282e+38f;
+    bf[0] = bf[1] = INT_MAX;
+    const int n = it.ref;
+    if (n >= 0 && n < N) {
+        unsigned allow = 0u;
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const unsigned wd = it.mask[m * 8 + 2 * q + (lane >> 5)];
+                allow |= ((wd >> (lane & 31)) & 1u) << (m * 4 + q);
+            }
+        // window rows j = 0..nr-1 in increasing grid order
+        const int ix0 = it.s / ny, iy0 = it.s % ny;
+        const int ixa = max(0, ix0 - hw), ixb = min(nx - 1, ix0 + hw);
+        const int iya = max(0, iy0 - hw), iyb = min(ny - 1, iy0 + hw);
+        const int wy = iyb - iya + 1, nr = (ixb - ixa + 1) * wy;
+        auto row = [&](int j) { return (ixa + j / wy) * ny + iya + j % wy; };
+        const float rs0 = rsum0[n];
+        const float2 *rb0 = rspec + (size_t)n * R * kSlots;
+        const size_t img_stride = (size_t)S * R * kSlots;
+        for (int r0 = 0; r0 < nr; r0 += 3) {
+            float acc[3][2][4];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) acc[r][h][e] = 0.0f;
+            const float2 *arow[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                arow[r] = spec + (size_t)p * img_stride + (size_t)row(min(r0 + r, nr - 1)) * R * kSlots;
+            for (int i = 0; i < R; ++i) {
+                const float2 b0v = rb0[i * kSlots + lane], b1v = rb0[i * kSlots + lane + 64];
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    const float2 a0 = arow[r][i * kSlots + lane], a1 = arow[r][i * kSlots + lane + 64];
+                    acc[r][0][0] += b0v.x * a0.x;
+                    acc[r][0][1] += b0v.y * a0.y;
+                    acc[r][0][2] += b0v.x * a0.y;
+                    acc[r][0][3] += b0v.y * a0.x;
+                    acc[r][1][0] += b1v.x * a1.x;
+                    acc[r][1][1] += b1v.y * a1.y;
+                    acc[r][1][2] += b1v.x * a1.y;
+                    acc[r][1][3] += b1v.y * a1.x;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                if (r0 + r >= nr) break;
+                const int s = row(r0 + r);
+                psi_update<false, true>(acc[r][0], acc[r][1], norm[(size_t)p * S + s], rs0, s, lane,
+                                        fbuf[wave], fbuf[wave] + 256, tw, zbuf[wave], nbuf[wave], bv,
+                                        bf, allow);
+            }
+        }
+    }
+    write_rec(rec + item, bv, bf, nbuf[wave], lane);
+}
+
+// Best item per particle (value, then lower reference, mirror, shift, psi
+// sample), decoded like the multi-reference wizard; fallback[p] = 1 when the
+// best value is below the threshold (or no item had a valid sample).
+__global__ void k_best_prop(const PeakRec *__restrict__ rec, const PropItem *__restrict__ items,
+                            int n_img, int K, ShiftView acc, const float2 *__restrict__ shifts_f,
+                            int n_shift, float threshold, int p_off, int *__restrict__ best_ref,
+                            float *__restrict__ best_val, float *__restrict__ best_peak,
+                            int *__restrict__ fallback) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_img) return;
+    int bk = -1;
+    for (int k = 0; k < K; ++k) {
+        const PropItem &it = items[(size_t)p * K + k];
+        const PeakRec &r = rec[(size_t)p * K + k];
+        if (it.ref < 0 || !(r.val > -3.40282e+38f)) continue;
+        if (bk < 0) {
+            bk = k;
+            continue;
+        }
+        const PeakRec &b = rec[(size_t)p * K + bk];
+        const int br = items[(size_t)p * K + bk].ref;
+        const bool better = r.val > b.val ||
+                            (r.val == b.val &&
+                             (it.ref < br || (it.ref == br && (r.m < b.m || (r.m == b.m &&
+                              (r.s < b.s || (r.s == b.s && r.t < b.t)))))));
+        if (better) bk = k;
+    }
+    const size_t pp = (size_t)p_off + p;
+    if (bk < 0) {
+        best_ref[pp] = -1;
+        best_val[pp] = -3.40282e+38f;
+        best_peak[4 * pp] = best_peak[4 * pp + 1] = best_peak[4 * pp + 2] = best_peak[4 * pp + 3] = 0.0f;
+        fallback[pp] = 1;
+        return;
+    }
+    const PeakRec r = rec[(size_t)p * K + bk];
+    float ax = 0.0f, ay = 0.0f;
+    if (acc.base) {
+        ax = acc.base[(size_t)p * acc.stride];
+        ay = acc.base[(size_t)p * acc.stride + 1];
+    }
+    const float2 sh = shifts_f[min(max(r.s, 0), n_shift - 1)];
+    best_ref[pp] = items[(size_t)p * K + bk].ref;
+    best_val[pp] = r.val;
+    best_peak[4 * pp + 0] = wizard_angle(r);
+    best_peak[4 * pp + 1] = ax + sh.x;
+    best_peak[4 * pp + 2] = ay + sh.y;
+    best_peak[4 * pp + 3] = r.m ? 1.0f : 0.0f;
+    fallback[pp] = (r.val >= threshold) ? 0 : 1;
+}
+
+// Indices of the particles flagged for the exhaustive fallback (any order).
+__global__ void k_compact(const int *__restrict__ flag, int n, int *__restrict__ idx,
+                          int *__restrict__ count) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || !flag[p]) return;
+    idx[atomicAdd(count, 1)] = p;
+}
+
+// Copies the images (and accumulated shifts) of the listed particles.
+__global__ void k_gather(const float *__restrict__ imgs, int D, ShiftView acc,
+                         const int *__restrict__ idx, int n, float *__restrict__ out,
+                         float2 *__restrict__ out_acc) {
+    const int j = blockIdx.x;
+    if (j >= n) return;
+    const int p = idx[j];
+    const float *src = imgs + (size_t)p * D * D;
+    float *dst = out + (size_t)j * D * D;
+    for (int i = threadIdx.x; i < D * D; i += blockDim.x) dst[i] = src[i];
+    if (threadIdx.x == 0)
+        out_acc[j] = acc.base ? f2(acc.base[(size_t)p * acc.stride], acc.base[(size_t)p * acc.stride + 1])
+                              : f2(0.0f, 0.0f);
+}
+
+// Exhaustive records rec[j][N] of gathered particle j -> best reference (first
+// maximum over references), decoded into the outputs of particle idx[j0 + j].
+__global__ void k_best_refs_scatter(const PeakRec *__restrict__ rec, int n, int N,
+                                    const float2 *__restrict__ acc, const float2 *__restrict__ shifts_f,
+                                    int n_shift, const int *__restrict__ idx, int *__restrict__ best_ref,
+                                    float *__restrict__ best_val, float *__restrict__ best_peak) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int bn = 0;
+    for (int k = 1; k < N; ++k)
+        if (rec[(size_t)j * N + k].val > rec[(size_t)j * N + bn].val) bn = k;
+    const PeakRec r = rec[(size_t)j * N + bn];
+    const float2 a = acc[j];
+    const float2 sh = shifts_f[min(max(r.s, 0), n_shift - 1)];
+    const size_t pp = (size_t)idx[j];
+    best_ref[pp] = bn;
+    best_val[pp] = r.val;
+    best_peak[4 * pp + 0] = wizard_angle(r);
+    best_peak[4 * pp + 1] = a.x + sh.x;
+    best_peak[4 * pp + 2] = a.y + sh.y;
+    best_peak[4 * pp + 3] = r.m ? 1.0f : 0.0f;
+}
+
 // Best candidate per particle -> reference index, peak value, {angle, sx, sy, mirror}.
 __global__ void k_decode_best(const PeakRec *__restrict__ rec, const int *__restrict__ cand,
                               const int *__restrict__ best, const int *__restrict__ tie, int n_img,
@@ -956,10 +1213,12 @@ __global__ void k_row_c2r(float *__restrict__ refs, int D, const float2 *__restr
 hipError_t prepare_launches() {
     static bool lds_raised = false;
     if (!lds_raised) {
-        const hipError_t e = hipFuncSetAttribute((const void *)k_polar_spectra,
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                 (int)((256 + 4 * 512) * sizeof(float2) +
-                                                       kMaxImgDim * kMaxImgDim * sizeof(float)));
+        const int lds = (int)((256 + 4 * 512) * sizeof(float2) + kMaxImgDim * kMaxImgDim * sizeof(float));
+        hipError_t e = hipFuncSetAttribute((const void *)k_polar_spectra<false>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        if (e != hipSuccess) return e;
+        e = hipFuncSetAttribute((const void *)k_polar_spectra<true>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds);
         if (e != hipSuccess) return e;
         lds_raised = true;
     }
@@ -974,8 +1233,22 @@ hipError_t launch_polar_spectra(const float *imgs, int n_img, int img_dim, Shift
     const size_t lds = (256 + 4 * 512) * sizeof(float2) + (size_t)img_dim * img_dim * sizeof(float);
     if (hipError_t e = prepare_launches()) return e;
     dim3 grid(n_img);
-    hipLaunchKernelGGL(k_polar_spectra, grid, dim3(256), lds, st, imgs, img_dim, acc, shift_i,
-                       n_shift, tmpl, n_ring, twiddle, spec, norm);
+    hipLaunchKernelGGL(k_polar_spectra<false>, grid, dim3(256), lds, st, imgs, img_dim, acc,
+                       shift_i, n_shift, tmpl, n_ring, twiddle, spec, norm, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_polar_spectra_masked(const float *imgs, int n_img, int img_dim, ShiftView acc,
+                                       const int2 *shift_i, int n_shift, const float2 *tmpl,
+                                       int n_ring, const float2 *twiddle, float2 *spec,
+                                       float2 *norm, const unsigned long long *smask,
+                                       hipStream_t st) {
+    if (n_img <= 0 || n_shift <= 0) return hipSuccess;
+    if (n_shift > 64) return hipErrorInvalidValue;
+    const size_t lds = (256 + 4 * 512) * sizeof(float2) + (size_t)img_dim * img_dim * sizeof(float);
+    if (hipError_t e = prepare_launches()) return e;
+    hipLaunchKernelGGL(k_polar_spectra<true>, dim3(n_img), dim3(256), lds, st, imgs, img_dim, acc,
+                       shift_i, n_shift, tmpl, n_ring, twiddle, spec, norm, smask);
     return hipGetLastError();
 }
 
@@ -1156,6 +1429,62 @@ hipError_t launch_tanl_filter(float *refs, int n, int img_dim, float cutoff, flo
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_row_c2r, dim3(D, n), dim3(128), 0, st, refs, D, work, tw_d);
+    return hipGetLastError();
+}
+
+hipError_t launch_prop_items(const int *prop, int n_img, int K, int N, int nx, int ny, int hw,
+                             int n_bins, float psi_half, PropItem *items,
+                             unsigned long long *smask, hipStream_t st) {
+    if (n_img <= 0) return hipSuccess;
+    if (K < 1 || K > kMaxProposals || nx * ny > 64 || hw < 0 || hw > 2) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_prop_items, dim3((n_img + 127) / 128), dim3(128), 0, st,
+                       reinterpret_cast<const int4 *>(prop), n_img, K, N, nx, ny, hw, n_bins, psi_half,
+                       items, smask);
+    return hipGetLastError();
+}
+
+hipError_t launch_ccf_prop(const float2 *spec, const float2 *norm, const float2 *rspec,
+                           const float *rsum0, const PropItem *items, int K, int n_img, int n_shift,
+                           int n_ring, int n_ref, int nx, int ny, int hw, const float2 *twiddle,
+                           PeakRec *rec, hipStream_t st) {
+    const int n_items = n_img * K;
+    if (n_items <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ccf_prop, dim3((n_items + 3) / 4), dim3(256), 0, st, spec, norm, rspec, rsum0,
+                       items, K, n_items, n_shift, n_ring, n_ref, nx, ny, hw, twiddle, rec);
+    return hipGetLastError();
+}
+
+hipError_t launch_best_prop(const PeakRec *rec, const PropItem *items, int n_img, int K,
+                            ShiftView acc, const float2 *shifts_f, int n_shift, float threshold,
+                            int p_off, int *best_ref, float *best_val, float *best_peak,
+                            int *fallback, hipStream_t st) {
+    if (n_img <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_best_prop, dim3((n_img + 127) / 128), dim3(128), 0, st, rec, items, n_img, K,
+                       acc, shifts_f, n_shift, threshold, p_off, best_ref, best_val, best_peak, fallback);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact(const int *flag, int n, int *idx, int *count, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, count, 1);
+    hipLaunchKernelGGL(k_compact, dim3((n + 255) / 256), dim3(256), 0, st, flag, n, idx, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const float *imgs, int D, ShiftView acc, const int *idx, int n, float *out,
+                         float2 *out_acc, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather, dim3(n), dim3(256), 0, st, imgs, D, acc, idx, n, out, out_acc);
+    return hipGetLastError();
+}
+
+hipError_t launch_best_refs_scatter(const PeakRec *rec, int n, int N, const float2 *acc,
+                                    const float2 *shifts_f, int n_shift, const int *idx,
+                                    int *best_ref, float *best_val, float *best_peak,
+                                    hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_best_refs_scatter, dim3((n + 127) / 128), dim3(128), 0, st, rec, n, N, acc,
+                       shifts_f, n_shift, idx, best_ref, best_val, best_peak);
     return hipGetLastError();
 }
 

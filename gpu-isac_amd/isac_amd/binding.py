@@ -33,6 +33,7 @@ EXTRA_SYMBOLS = (
     "gpu_last_error", "gpu_last_error_string", "gpu_clear_error", "isac_mref_align_device",
     "isac_mref_align_mixed_device", "isac_mref_best_mixed_device", "isac_profile_enable",
     "isac_profile_read", "isac_mref_align_batched_device", "isac_batched_report",
+    "isac_mref_proposal_device",
 )
 
 BATCH_PER_SUBSET = 0   # ISAC_BATCH_PER_SUBSET
@@ -133,6 +134,11 @@ def load(path: Optional[str] = None):
                                                 c_float, c_float, c_float, c_int, c_float,
                                                 vp, vp, vp, vp, vp, vp, vp]
     lib.isac_mref_best_mixed_device.restype = c_int
+    lib.isac_mref_proposal_device.argtypes = [vp, vp, vp, c_uint, c_uint, c_uint, c_uint, c_uint,
+                                              c_float, c_float, c_float, vp, c_uint, c_uint,
+                                              c_float, c_uint, c_float, vp, vp, vp, vp,
+                                              ctypes.POINTER(c_uint), vp]
+    lib.isac_mref_proposal_device.restype = c_int
     lib.isac_mref_align_batched_device.argtypes = [ctypes.POINTER(IsacMrefSubset), c_uint, c_int, vp]
     lib.isac_mref_align_batched_device.restype = c_int
     ip = ctypes.POINTER(c_int)
@@ -412,6 +418,48 @@ def mref_best_mixed_device(sbj, ref, ring_num: int, shift_step: float, shift_rng
         check(lib)
         raise GpuAlignError(f"isac_mref_best_mixed_device returned {rc}")
     return out
+
+
+def mref_proposal_device(sbj, ref, ring_num: int, shift_step: float, shift_rng_x: float,
+                         shift_rng_y: float, prop, threshold: float, n_psi_bins: int = 72,
+                         psi_half: float = 10.0, shift_half: int = 1, acc_xy=None, stream=None):
+    """Narrowed search around proposals prop [P, K, 4] int32 (ref, mirror, psi
+    bin, shift index) with the exhaustive fallback below ``threshold``.
+    Returns a dict of device tensors best_ref, best_val, best_peak [P, 4],
+    fallback, plus n_fallback (int)."""
+    import torch
+    lib = load()
+    P, D = sbj.shape[0], sbj.shape[-1]
+    N = ref.shape[0]
+    assert prop.dtype == torch.int32 and prop.is_contiguous() and prop.shape[0] == P
+    dev = sbj.device
+    out = {"best_ref": torch.empty(P, dtype=torch.int32, device=dev),
+           "best_val": torch.empty(P, dtype=torch.float32, device=dev),
+           "best_peak": torch.empty((P, 4), dtype=torch.float32, device=dev),
+           "fallback": torch.empty(P, dtype=torch.int32, device=dev)}
+    nfb = ctypes.c_uint(0)
+    st = (stream or torch.cuda.current_stream()).cuda_stream
+    rc = lib.isac_mref_proposal_device(
+        sbj.data_ptr(), ref.data_ptr(), acc_xy.data_ptr() if acc_xy is not None else None, P, N, D,
+        ring_num, RING_LEN, shift_step, shift_rng_x, shift_rng_y, prop.data_ptr(), prop.shape[1],
+        n_psi_bins, psi_half, shift_half, threshold, out["best_ref"].data_ptr(),
+        out["best_val"].data_ptr(), out["best_peak"].data_ptr(), out["fallback"].data_ptr(),
+        ctypes.byref(nfb), st)
+    if rc:
+        check(lib)
+        raise GpuAlignError(f"isac_mref_proposal_device returned {rc}")
+    out["n_fallback"] = int(nfb.value)
+    return out
+
+
+def best_of_exhaustive(d, peak_list):
+    """Best reference per particle of an exhaustive result (first maximum over
+    references, like the fallback): (best_ref [P], best_val [P], best_peak [P, 4])."""
+    import torch
+    N, P = d.shape
+    best = torch.argmax(d, dim=0)
+    cols = torch.arange(P, device=d.device)
+    return best.to(torch.int32), d[best, cols], peak_list.view(N, P, 4)[best, cols]
 
 
 class BatchedMref:

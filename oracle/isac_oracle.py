@@ -470,3 +470,72 @@ def ref_free_iteration(sbj, refs, cid, acc_xy, ring_num, shifts, img_dim,
     cid = np.asarray(cid)
     cid_idx = list(np.flatnonzero(np.r_[True, cid[1:] != cid[:-1]])) + [len(cid)]
     return prm, class_averages(tr, cid_idx)
+
+
+# ------------------------------------------------------------ narrowed search
+# No counterpart in the reference (its search is exhaustive).  Restates the
+# narrowed search of isac_mref_proposal_device: a proposal (ref, mirror, psi
+# bin b, shift index s0) admits the shifts within hw grid steps of s0 and the psi
+# samples t with |t * 360/L - (b + 0.5) * 360/n_bins| <= psi_half (t taken
+# modulo L) of that mirror; per particle the maximum over every admitted
+# (ref, m, s, t), ties to the lower ref, then the table order (m, s, t).
+# Pin: when a window holds the exhaustive optimum it returns multi_ref_align's
+# result (tests/test_oracle.py).
+
+def proposal_mask(prop_p, n_ref, nx, ny, n_bins, psi_half, hw, ring_len=RING_LEN):
+    """Admitted samples of one particle's proposals: bool [N, 2, S, L]."""
+    S, L = nx * ny, ring_len
+    mask = np.zeros((n_ref, 2, S, L), bool)
+    step = 360.0 / L
+    bw = 360.0 / n_bins
+    for ref, m, b, s0 in np.asarray(prop_p).reshape(-1, 4):
+        if not (0 <= ref < n_ref and 0 <= m <= 1 and 0 <= b < n_bins and 0 <= s0 < S):
+            continue
+        c = (b + 0.5) * bw
+        half = float(np.float32(psi_half))
+        t0 = int(np.ceil((c - half) / step))
+        t1 = int(np.floor((c + half) / step))
+        ts = np.arange(t0, min(t1, t0 + L - 1) + 1) % L
+        ix0, iy0 = divmod(int(s0), ny)
+        for ix in range(max(0, ix0 - hw), min(nx - 1, ix0 + hw) + 1):
+            for iy in range(max(0, iy0 - hw), min(ny - 1, iy0 + hw) + 1):
+                mask[ref, m, ix * ny + iy, ts] = True
+    return mask
+
+
+def windowed_best(sbj, ref, acc_xy, ring_num, shift_rng_x, shift_rng_y, shift_step, prop,
+                  n_bins=72, psi_half=10.0, hw=1, ring_len=RING_LEN):
+    """Best (ref, value, angle, sx, sy, mirror, s, t) per particle over the
+    proposals' windows; ref = -1 when nothing is admitted."""
+    sbj = np.asarray(sbj)
+    acc_xy = np.asarray(acc_xy, dtype=np.float64).reshape(-1, 2)
+    P, N = sbj.shape[0], ref.shape[0]
+    shifts = shift_grid(shift_rng_x, shift_rng_y, shift_step)
+    S, L = shifts.shape[0], ring_len
+    ny = int(np.sum(shifts[:, 0] == shifts[0, 0]))
+    nx = S // ny
+    rspec = reference_spectra(np.asarray(ref), ring_num, ring_len)
+    out = {k: np.zeros(P) for k in ("d", "angle", "sx", "sy", "mirror", "shift_idx", "t")}
+    out["ref"] = np.full(P, -1, np.int64)
+    for p in range(P):
+        mask = proposal_mask(prop[p], N, nx, ny, n_bins, psi_half, hw, ring_len)
+        refs = np.nonzero(mask.reshape(N, -1).any(1))[0]
+        if refs.size == 0:
+            out["d"][p] = -np.inf
+            continue
+        sspec = particle_spectra(sbj[p:p + 1], acc_xy[p:p + 1], shifts, ring_num, ring_len)
+        tab = ccf_table(sspec, rspec[refs], ring_len)[0]       # (n, 2, S, L)
+        vals = np.where(mask[refs], tab, -np.inf).reshape(len(refs), -1)
+        idx = np.argmax(vals, axis=1)                           # first max in (m, s, t) order
+        best = vals[np.arange(len(refs)), idx]
+        j = int(np.argmax(best))                                # ties: lower reference
+        m, s, t = idx[j] // (S * L), (idx[j] // L) % S, idx[j] % L
+        x7 = neighbours7(tab[j, m, s][None], np.array([t]))
+        delta = parabola_delta(x7)
+        out["ref"][p] = refs[j]
+        out["d"][p] = best[j]
+        out["angle"][p] = angle_multiref(np.array([t]), delta, np.array([m]), L)[0]
+        out["sx"][p] = acc_xy[p, 0] + shifts[s, 0]
+        out["sy"][p] = acc_xy[p, 1] + shifts[s, 1]
+        out["mirror"][p], out["shift_idx"][p], out["t"][p] = m, s, t
+    return out

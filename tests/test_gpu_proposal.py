@@ -1,0 +1,105 @@
+"""Narrowed search around proposals: exactness when a window holds the
+exhaustive optimum, the exhaustive fallback, and the oracle of the windows."""
+import numpy as np
+import pytest
+
+from isac_amd import binding, synth
+from oracle import isac_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+R, RNG = 35, 3.0
+
+
+def _data(P=300, N=30, seed=1):
+    import torch
+    sbj, ref, _ = synth.make_dataset_torch(P, N, 76, 34.0, 0.05, 3.0, False, seed_refs=seed,
+                                           seed_particles=seed + 1, seed_noise=seed + 2)
+    return sbj, ref
+
+
+def _exhaustive(sbj, ref, acc=None):
+    peak, d = binding.mref_align_device(sbj, ref, R, 1.0, RNG, RNG, acc_xy=acc)
+    return d, peak
+
+
+def _labels_prop(d, peak):
+    from isac_amd import posenet
+    lab = posenet.labels_from_search(d, peak, RNG)
+    m = lab[:, 1] // 72
+    b = lab[:, 1] % 72
+    import torch
+    return torch.stack([lab[:, 0], m, b, lab[:, 2]], 1).to(torch.int32)
+
+
+def test_window_on_the_optimum_is_bit_identical(gpu):
+    import torch
+    sbj, ref = _data()
+    acc = torch.from_numpy(np.random.default_rng(0).uniform(-1, 1, (300, 2)).astype(np.float32)).cuda()
+    d, peak = _exhaustive(sbj, ref, acc)
+    bref, bval, bpeak = binding.best_of_exhaustive(d, peak)
+    prop = _labels_prop(d, peak)
+    # accumulated shifts move the grid: rebuild the shift index from the peak shift
+    sx = bpeak[:, 1] - acc[:, 0]
+    sy = bpeak[:, 2] - acc[:, 1]
+    prop[:, 3] = (torch.round(sx + RNG) * 7 + torch.round(sy + RNG)).to(torch.int32)
+    # pad with junk proposals that must not matter (out-of-range and duplicate)
+    junk = torch.tensor([-1, 0, 0, 0], dtype=torch.int32, device="cuda").expand(300, 1, 4)
+    props = torch.cat([prop[:, None, :], junk, prop[:, None, :]], 1).contiguous()
+    o = binding.mref_proposal_device(sbj, ref, R, 1.0, RNG, RNG, props, threshold=-3.0e38,
+                                     acc_xy=acc)
+    torch.cuda.synchronize()
+    assert o["n_fallback"] == 0
+    assert torch.equal(o["best_ref"], bref)
+    assert torch.equal(o["best_val"], bval)
+    assert torch.equal(o["best_peak"], bpeak)
+
+
+def test_fallback_is_the_exhaustive_search(gpu):
+    import torch
+    sbj, ref = _data(P=200, N=20, seed=5)
+    d, peak = _exhaustive(sbj, ref)
+    bref, bval, bpeak = binding.best_of_exhaustive(d, peak)
+    prop = _labels_prop(d, peak)
+    prop[:, 0] = (prop[:, 0] + 1) % 20        # wrong references everywhere
+    o = binding.mref_proposal_device(sbj, ref, R, 1.0, RNG, RNG, prop[:, None, :].contiguous(),
+                                     threshold=3.0e38)
+    torch.cuda.synchronize()
+    assert o["n_fallback"] == 200 and int(o["fallback"].sum()) == 200
+    assert torch.equal(o["best_ref"], bref)
+    assert torch.equal(o["best_val"], bval)
+    assert torch.equal(o["best_peak"], bpeak)
+    # a calibrated threshold sends exactly the particles below it
+    thr = float(torch.quantile(bval, 0.5))
+    o2 = binding.mref_proposal_device(sbj, ref, R, 1.0, RNG, RNG, prop[:, None, :].contiguous(),
+                                      threshold=thr)
+    torch.cuda.synchronize()
+    fb = o2["fallback"].bool()
+    assert o2["n_fallback"] == int(fb.sum())
+    assert torch.equal(o2["best_val"][fb], bval[fb])
+    assert bool(torch.all(o2["best_val"][~fb] >= thr))
+
+
+def test_narrowed_search_matches_window_oracle(gpu):
+    """Random proposals (near and far from the optimum) against the FP64 oracle
+    of the windows: same reference, mirror and shift, psi within 0.5 degrees,
+    value within 1e-4 relative."""
+    import torch
+    P, N = 24, 6
+    ds = synth.make_dataset(P, N, 76, 34.0, 0.1, 3.0, False, seed_refs=3)
+    rng = np.random.default_rng(9)
+    prop = np.stack([rng.integers(0, N, (P, 4)), rng.integers(0, 2, (P, 4)),
+                     rng.integers(0, 72, (P, 4)), rng.integers(0, 49, (P, 4))], 2).astype(np.int32)
+    acc = np.zeros((P, 2), np.float32)
+    w = O.windowed_best(ds.sbj, ds.ref, acc, R, RNG, RNG, 1.0, prop)
+    s, r = torch.from_numpy(ds.sbj).cuda(), torch.from_numpy(ds.ref).cuda()
+    o = binding.mref_proposal_device(s, r, R, 1.0, RNG, RNG, torch.from_numpy(prop).cuda(),
+                                     threshold=-3.0e38)
+    torch.cuda.synchronize()
+    g = {k: v.cpu().numpy() for k, v in o.items() if k != "n_fallback"}
+    np.testing.assert_array_equal(g["best_ref"], w["ref"])
+    np.testing.assert_allclose(g["best_val"], w["d"], rtol=1e-4)
+    np.testing.assert_array_equal(g["best_peak"][:, 3], w["mirror"])
+    np.testing.assert_allclose(g["best_peak"][:, 1:3], np.stack([w["sx"], w["sy"]], 1), atol=1e-5)
+    dpsi = np.abs((g["best_peak"][:, 0] - w["angle"] + 180.0) % 360.0 - 180.0)
+    assert dpsi.max() <= 0.5, dpsi.max()

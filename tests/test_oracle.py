@@ -323,3 +323,39 @@ def test_ref_free_iteration_converges_noise_free():
         acc = np.stack([prm["shift_x"], prm["shift_y"]], 1)
     c_end = best_corr(avg[0])
     assert c_end > 0.97 and c_end > c_start
+
+
+def test_windowed_search_holding_the_optimum_equals_exhaustive():
+    """A proposal whose window holds the exhaustive optimum returns it exactly;
+    a proposal 180 degrees away returns a lower value inside its own window."""
+    from isac_amd import synth
+    ds = synth.make_dataset(6, 5, 48, 20.0, 0.2, 2.0, True, seed_refs=7)
+    acc = np.zeros((6, 2))
+    full = O.multi_ref_align(ds.sbj, ds.ref, acc, 19, 2.0, 2.0, 1.0)
+    best = np.argmax(full["d"], axis=0)
+    cols = np.arange(6)
+    m = full["mirror"][best, cols].astype(int)
+    s = full["shift_idx"][best, cols].astype(int)
+    t = full["t"][best, cols].astype(int)
+    b = (t * 360.0 / 256 // 5).astype(int)
+    prop = np.stack([best, m, b, s], 1)[:, None, :]
+    w = O.windowed_best(ds.sbj, ds.ref, acc, 19, 2.0, 2.0, 1.0, prop)
+    np.testing.assert_array_equal(w["ref"], best)
+    np.testing.assert_array_equal(w["d"], full["d"][best, cols])
+    np.testing.assert_array_equal(w["angle"], full["angle"][best, cols])
+    far = prop.copy()
+    far[:, 0, 2] = (b + 36) % 72
+    w2 = O.windowed_best(ds.sbj, ds.ref, acc, 19, 2.0, 2.0, 1.0, far)
+    assert np.all(w2["d"] < full["d"][best, cols])
+    centre = (far[:, 0, 2] + 0.5) * 5.0
+    dist = np.abs((w2["t"] * 360.0 / 256 - centre + 180.0) % 360.0 - 180.0)
+    assert np.all(dist <= 10.0)
+
+
+def test_proposal_mask_window_shape():
+    mk = O.proposal_mask(np.array([[1, 1, 0, 0], [1, 1, 0, 0], [9, 0, 0, 0]]), 3, 7, 7, 72, 10.0, 1)
+    assert mk[1].sum() == 4 * mk[1, 1, 0].sum()                 # corner window: 2 x 2 shifts
+    ts = np.nonzero(mk[1, 1, 0])[0]
+    # bin 0 centre 2.5 deg, +-10 deg -> [-7.5, 12.5] deg -> t in {-5..8} mod 256
+    assert set(ts) == set(list(range(0, 9)) + list(range(251, 256)))
+    assert not mk[0].any() and not mk[2].any()                  # ref 9 is out of range: ignored
