@@ -182,6 +182,102 @@ def batched_report(steps=3, warmup=1, n_subsets=64):
     return out
 
 
+def learned_report(n_train=200000, n_cal=10000, n_test=100000, epochs=16, oracle_sample=1000,
+                   target=0.997, snr=0.1, integer_shifts=True, log=None):
+    """Config 5: the pose-proposal network trained on exhaustive-search labels of
+    200,000 config-1-style particles (blob references, uniform psi and mirror,
+    integer shifts, SNR 0.1) at 76 px, 100 references, +-3 px (49 shifts);
+    threshold calibrated on 10,000 more; evaluated on 100,000 held-out
+    particles: agreement with the exhaustive FP32 search and with the FP64
+    oracle (sample), fallback rate, end-to-end time against the plain FP32
+    search and the MFMA form, and the same search with an untrained network."""
+    import numpy as np
+    import torch
+    from isac_amd import binding, learned, posenet, synth
+    from oracle import coracle
+    log = log or (lambda m: print(m, file=sys.stderr, flush=True))
+    N, D, R, rng = 100, 76, 35, 3.0
+
+    def make(P, seed):
+        s, r, _ = synth.make_dataset_torch(P, N, D, 34.0, snr, rng, integer_shifts, seed_refs=1,
+                                           seed_particles=seed, seed_noise=seed + 1)
+        return s, r
+
+    out = {"report": "learned proposal (config 5)", "refs": N, "box": D, "shifts": 49,
+           "snr": snr, "integer_shifts": integer_shifts, "train": n_train,
+           "calibration": n_cal, "held_out": n_test, "epochs": epochs}
+    tr, ref = make(n_train, 11)
+    t0 = time.perf_counter()
+    lab, _, _ = learned.labels(tr, ref, R, rng)
+    torch.cuda.synchronize()
+    out["label_s"] = time.perf_counter() - t0
+    net = posenet.PoseNet(N, 49, D=D, seed=0, **learned.NET)
+    out["params"] = net.n_params
+    t0 = time.perf_counter()
+    hist = net.train(tr, lab, epochs=epochs, log=log, **learned.TRAIN)
+    torch.cuda.synchronize()
+    out["train_s"] = time.perf_counter() - t0
+    out["train_loss_per_epoch"] = [round(h, 4) for h in hist]
+    del tr, lab
+    cal, _ = make(n_cal, 21)
+    _, dc, pc = learned.labels(cal, ref, R, rng)
+    pr, raw = learned.calibrate(net, cal, ref, dc, pc, R, rng, target=target)
+    out["calibration_set"] = {"narrowed_only_agreement": raw, "threshold": pr.threshold,
+                              "target": target}
+    del cal, dc, pc
+    te, _ = make(n_test, 31)
+    d, peak = learned.exhaustive(te, ref, R, rng)
+
+    def timed(fn, reps=2):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            r = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps, r
+
+    t_plain, _ = timed(lambda: learned.exhaustive(te, ref, R, rng, chunk=n_test))
+    t_mfma, _ = timed(lambda: binding.mref_best_mixed_device(te, ref, R, 1.0, rng, rng, "fp16"))
+    t_learn, o = timed(lambda: pr.run(te, ref))
+    t_cnn, prop = timed(lambda: net.propose(te, 8))
+    t_narrow, on = timed(lambda: binding.mref_proposal_device(te, ref, R, 1.0, rng, rng, prop,
+                                                              threshold=-3.0e38))
+    agree = learned.agreement(o["best_ref"], o["best_peak"], d, peak)
+    agree_n = learned.agreement(on["best_ref"], on["best_peak"], d, peak)
+    un = learned.Proposer(posenet.PoseNet(N, 49, D=D, seed=1, **learned.NET), pr.threshold, R, rng)
+    t_un, ou = timed(lambda: un.run(te, ref), reps=1)
+    agree_un = learned.agreement(ou["best_ref"], ou["best_peak"], d, peak)
+    out["held_out_set"] = {
+        "agreement_with_exhaustive_fp32": float(agree.double().mean()),
+        "narrowed_only_agreement": float(agree_n.double().mean()),
+        "fallback_fraction": o["n_fallback"] / n_test,
+        "plain_fp32_s": t_plain, "mfma_fp16_best_s": t_mfma, "learned_s": t_learn,
+        "cnn_proposals_s": t_cnn, "narrowed_without_fallback_s": t_narrow,
+        "speedup_vs_plain_fp32": t_plain / t_learn, "speedup_vs_mfma": t_mfma / t_learn,
+        "learned_particles_per_s": n_test / t_learn,
+        "plain_fp32_particles_per_s": n_test / t_plain}
+    out["without_training"] = {"agreement": float(agree_un.double().mean()),
+                               "fallback_fraction": ou["n_fallback"] / n_test, "time_s": t_un,
+                               "speedup_vs_plain_fp32": t_plain / t_un}
+    # FP64 oracle on a sample of the held-out particles
+    k = oracle_sample
+    s_h, r_h = te[:k].cpu().numpy(), ref.cpu().numpy()
+    t0 = time.perf_counter()
+    orc = coracle.mref(s_h, r_h, np.zeros((k, 2), np.float32), R, rng, rng, 1.0,
+                       nthreads=min(16, len(os.sched_getaffinity(0))), precision="f64")
+    out["oracle_s"] = time.perf_counter() - t0
+    od = torch.from_numpy(orc["d"].astype(np.float32)).cuda()
+    opk = torch.from_numpy(orc["peak_list"]).cuda()
+    a_l = learned.agreement(o["best_ref"][:k], o["best_peak"][:k], od, opk)
+    bref, _, bpk = binding.best_of_exhaustive(d[:, :k].contiguous(),
+                                              peak.view(N, -1, 4)[:, :k].reshape(N, 4 * k))
+    a_p = learned.agreement(bref, bpk, od, opk)
+    out["fp64_oracle_sample"] = {"particles": k, "learned_agreement": float(a_l.double().mean()),
+                                 "plain_fp32_agreement": float(a_p.double().mean())}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,7 +288,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the mixed-precision forms")
-    ap.add_argument("--report", choices=["precision", "batched"], default=None,
+    ap.add_argument("--report", choices=["precision", "batched", "learned"], default=None,
                     help="precision: config-3 accuracy report; batched: config-4 uneven-subset "
                          "throughput, graph vs one launch per subset")
     args = ap.parse_args()
@@ -208,6 +304,9 @@ def main():
 
     if args.report == "precision":
         print(json.dumps(precision_report()))
+        return
+    if args.report == "learned":
+        print(json.dumps(learned_report()))
         return
     if args.report == "batched":
         print(json.dumps(batched_report(steps=args.steps if args.steps != 10 else 3,

@@ -451,7 +451,7 @@ int run_mref_mixed(const float *d_sbj, const float *d_ref, ShiftView acc, int P,
 // whose best value is below `threshold` are searched exhaustively.
 int run_proposal(const float *d_sbj, const float *d_ref, ShiftView acc, int P, int N, int D, int R,
                  const std::vector<float2> &shifts, int nx, int ny, const int *d_prop, int K,
-                 int n_bins, float psi_half, int hw, float threshold, int *best_ref,
+                 int n_bins, float psi_half, int hw, float threshold, int edge_fb, int *best_ref,
                  float *best_val, float *best_peak, int *fallback, unsigned *n_fb_out,
                  hipStream_t st) {
     const int S = (int)shifts.size();
@@ -496,9 +496,9 @@ int run_proposal(const float *d_sbj, const float *d_ref, ShiftView acc, int P, i
                                          ws.twiddle.as<float2>(), ws.rec.as<PeakRec>(), st));
             if (g.prof) { g.prof_ccf_ms += ps.finish(); g.prof_ccf_n++; }
         }
-        HIP_OK(isac::launch_best_prop(ws.rec.as<PeakRec>(), ws.items.as<isac::PropItem>(), n, K, a,
-                                      ws.shifts_f.as<float2>(), S, threshold, p0, best_ref, best_val,
-                                      best_peak, fallback, st));
+        HIP_OK(isac::launch_best_prop(ws.rec.as<PeakRec>(), ws.items.as<isac::PropItem>(), n, K, nx,
+                                      ny, hw, edge_fb, a, ws.shifts_f.as<float2>(), S, threshold, p0,
+                                      best_ref, best_val, best_peak, fallback, st));
     }
     // exhaustive fallback for the flagged particles
     HIP_OK(isac::launch_compact(fallback, P, ws.fb_idx.as<int>(), ws.fb_count.as<int>(), st));
@@ -1206,7 +1206,7 @@ int isac_mref_align_device(const float *d_sbj, const float *d_ref, const float *
         set_error(kErrArg, "isac_mref_align_device: bad pointers or d_pitch < n_sbj");
         return kErrArg;
     }
-    hipStream_t st = stream ? (hipStream_t)stream : ws.stream;
+    hipStream_t st = (hipStream_t)stream;   // NULL: the null stream, ordered with the caller's work
     ShiftView acc{d_acc_xy, 2};
     return run_mref(d_sbj, d_ref, acc, (int)n_sbj, (int)n_ref, (int)img_dim, (int)ring_num, sh,
                     d_peak_list, 4 * (int)n_sbj, d_d, (int)d_pitch, 0.9f, st);
@@ -1225,7 +1225,7 @@ int isac_mref_align_mixed_device(const float *d_sbj, const float *d_ref, const f
         set_error(kErrArg, "isac_mref_align_mixed_device: bad pointers or d_pitch < n_sbj");
         return kErrArg;
     }
-    hipStream_t st = stream ? (hipStream_t)stream : ws.stream;
+    hipStream_t st = (hipStream_t)stream;   // NULL: the null stream, ordered with the caller's work
     ShiftView acc{d_acc_xy, 2};
     return run_mref_mixed(d_sbj, d_ref, acc, (int)n_sbj, (int)n_ref, (int)img_dim, (int)ring_num,
                           sh, bf16 != 0, d_peak_list, 4 * (int)n_sbj, d_d, (int)d_pitch, 0.9f, st);
@@ -1246,7 +1246,7 @@ int isac_mref_best_mixed_device(const float *d_sbj, const float *d_ref, const fl
         set_error(kErrArg, "isac_mref_best_mixed_device: NULL pointer");
         return kErrArg;
     }
-    hipStream_t st = stream ? (hipStream_t)stream : ws.stream;
+    hipStream_t st = (hipStream_t)stream;   // NULL: the null stream, ordered with the caller's work
     ShiftView acc{d_acc_xy, 2};
     BestOut bo;
     bo.ref = d_best_ref;
@@ -1264,9 +1264,9 @@ int isac_mref_proposal_device(const float *d_sbj, const float *d_ref, const floa
                               unsigned int ring_num, unsigned int ring_len, float shift_step,
                               float shift_rng_x, float shift_rng_y, const int *d_prop,
                               unsigned int n_prop, unsigned int n_psi_bins, float psi_half_deg,
-                              unsigned int shift_half, float threshold, int *d_best_ref,
-                              float *d_best_val, float *d_best_peak, int *d_fallback,
-                              unsigned int *n_fallback, void *stream) {
+                              unsigned int shift_half, float threshold, int edge_fallback,
+                              int *d_best_ref, float *d_best_val, float *d_best_peak,
+                              int *d_fallback, unsigned int *n_fallback, void *stream) {
     if (int e = ensure_device_ws(-1)) return e;
     std::vector<float2> sh = make_shift_grid(shift_rng_x, shift_rng_y, shift_step);
     if (int e = check_shape((int)img_dim, (int)ring_num, (int)ring_len, sh.size())) return e;
@@ -1283,11 +1283,11 @@ int isac_mref_proposal_device(const float *d_sbj, const float *d_ref, const floa
     int ny = 0;
     while (ny < (int)sh.size() && sh[ny].x == sh[0].x) ++ny;
     const int nx = ny ? (int)sh.size() / ny : 0;
-    hipStream_t st = stream ? (hipStream_t)stream : ws.stream;
+    hipStream_t st = (hipStream_t)stream;   // NULL: the null stream, ordered with the caller's work
     return run_proposal(d_sbj, d_ref, ShiftView{d_acc_xy, 2}, (int)n_sbj, (int)n_ref, (int)img_dim,
                         (int)ring_num, sh, nx, ny, d_prop, (int)n_prop, (int)n_psi_bins, psi_half_deg,
-                        (int)shift_half, threshold, d_best_ref, d_best_val, d_best_peak, d_fallback,
-                        n_fallback, st);
+                        (int)shift_half, threshold, edge_fallback != 0, d_best_ref, d_best_val,
+                        d_best_peak, d_fallback, n_fallback, st);
 }
 
 int isac_mref_align_batched_device(const IsacMrefSubset *subsets, unsigned int n_subsets,
@@ -1298,7 +1298,7 @@ int isac_mref_align_batched_device(const IsacMrefSubset *subsets, unsigned int n
         set_error(kErrArg, "isac_mref_align_batched_device: NULL descriptor array");
         return kErrArg;
     }
-    hipStream_t st = stream ? (hipStream_t)stream : ws.stream;
+    hipStream_t st = (hipStream_t)stream;   // NULL: the null stream, ordered with the caller's work
     if (mode == ISAC_BATCH_GRAPH) {
         HIP_OK(hipStreamSynchronize(st));   // inputs written on the caller's stream
         return run_batched_graph(subsets, n_subsets, st);

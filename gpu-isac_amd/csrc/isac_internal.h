@@ -92,10 +92,11 @@ hipError_t launch_ccf_prop(const float2 *spec, const float2 *norm, const float2 
                            const float *rsum0, const PropItem *items, int K, int n_img, int n_shift,
                            int n_ring, int n_ref, int nx, int ny, int hw, const float2 *twiddle,
                            PeakRec *rec, hipStream_t st);
-hipError_t launch_best_prop(const PeakRec *rec, const PropItem *items, int n_img, int K,
-                            ShiftView acc, const float2 *shifts_f, int n_shift, float threshold,
-                            int p_off, int *best_ref, float *best_val, float *best_peak,
-                            int *fallback, hipStream_t st);
+hipError_t launch_best_prop(const PeakRec *rec, const PropItem *items, int n_img, int K, int nx,
+                            int ny, int hw, int edge_fallback, ShiftView acc,
+                            const float2 *shifts_f, int n_shift, float threshold, int p_off,
+                            int *best_ref, float *best_val, float *best_peak, int *fallback,
+                            hipStream_t st);
 // Exhaustive fallback: compact flagged particles, gather them, scatter the best
 // reference of each back to its index.
 hipError_t launch_compact(const int *flag, int n, int *idx, int *count, hipStream_t st);

@@ -915,12 +915,30 @@ __global__ __launch_bounds__(256) void k_ccf_prop(
     write_rec(rec + item, bv, bf, nbuf[wave], lane);
 }
 
+// Is (m, s, t) of reference `ref` admitted by one of the particle's items?
+__device__ __forceinline__ bool admitted(const PropItem *it, int K, int ref, int m, int s, int t,
+                                         int ny, int hw) {
+    const int ix = s / ny, iy = s % ny;
+    const int bit = m * kRingLen + t;
+    for (int k = 0; k < K; ++k) {
+        if (it[k].ref != ref) continue;
+        const int cx = it[k].s / ny, cy = it[k].s % ny;
+        if (abs(ix - cx) <= hw && abs(iy - cy) <= hw && ((it[k].mask[bit >> 5] >> (bit & 31)) & 1u))
+            return true;
+    }
+    return false;
+}
+
 // Best item per particle (value, then lower reference, mirror, shift, psi
-// sample), decoded like the multi-reference wizard; fallback[p] = 1 when the
-// best value is below the threshold (or no item had a valid sample).
+// sample), decoded like the multi-reference wizard.  fallback[p] = 1 when the
+// best value is below the threshold, when no item had a valid sample, or when
+// the maximum lies on the edge of the admitted region: one of its neighbours
+// (psi sample t +- 1, or the next shift in x or y inside the grid) is admitted
+// by no item of that reference, so the search may have been cut short.
 __global__ void k_best_prop(const PeakRec *__restrict__ rec, const PropItem *__restrict__ items,
-                            int n_img, int K, ShiftView acc, const float2 *__restrict__ shifts_f,
-                            int n_shift, float threshold, int p_off, int *__restrict__ best_ref,
+                            int n_img, int K, int nx, int ny, int hw, int edge_fallback,
+                            ShiftView acc, const float2 *__restrict__ shifts_f, int n_shift,
+                            float threshold, int p_off, int *__restrict__ best_ref,
                             float *__restrict__ best_val, float *__restrict__ best_peak,
                             int *__restrict__ fallback) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -957,13 +975,23 @@ __global__ void k_best_prop(const PeakRec *__restrict__ rec, const PropItem *__r
         ay = acc.base[(size_t)p * acc.stride + 1];
     }
     const float2 sh = shifts_f[min(max(r.s, 0), n_shift - 1)];
-    best_ref[pp] = items[(size_t)p * K + bk].ref;
+    const PropItem *it = items + (size_t)p * K;
+    const int ref = it[bk].ref;
+    const int m = min(max(r.m, 0), 1), s = min(max(r.s, 0), n_shift - 1), t = r.t & (kRingLen - 1);
+    const int ix = s / ny, iy = s % ny;
+    bool interior = admitted(it, K, ref, m, s, (t + 1) & (kRingLen - 1), ny, hw) &&
+                    admitted(it, K, ref, m, s, (t + kRingLen - 1) & (kRingLen - 1), ny, hw);
+    if (ix > 0) interior = interior && admitted(it, K, ref, m, s - ny, t, ny, hw);
+    if (ix < nx - 1) interior = interior && admitted(it, K, ref, m, s + ny, t, ny, hw);
+    if (iy > 0) interior = interior && admitted(it, K, ref, m, s - 1, t, ny, hw);
+    if (iy < ny - 1) interior = interior && admitted(it, K, ref, m, s + 1, t, ny, hw);
+    best_ref[pp] = ref;
     best_val[pp] = r.val;
     best_peak[4 * pp + 0] = wizard_angle(r);
     best_peak[4 * pp + 1] = ax + sh.x;
     best_peak[4 * pp + 2] = ay + sh.y;
     best_peak[4 * pp + 3] = r.m ? 1.0f : 0.0f;
-    fallback[pp] = (r.val >= threshold) ? 0 : 1;
+    fallback[pp] = ((interior || !edge_fallback) && r.val >= threshold) ? 0 : 1;
 }
 
 // Indices of the particles flagged for the exhaustive fallback (any order).
@@ -1454,13 +1482,16 @@ hipError_t launch_ccf_prop(const float2 *spec, const float2 *norm, const float2 
     return hipGetLastError();
 }
 
-hipError_t launch_best_prop(const PeakRec *rec, const PropItem *items, int n_img, int K,
-                            ShiftView acc, const float2 *shifts_f, int n_shift, float threshold,
-                            int p_off, int *best_ref, float *best_val, float *best_peak,
-                            int *fallback, hipStream_t st) {
+hipError_t launch_best_prop(const PeakRec *rec, const PropItem *items, int n_img, int K, int nx,
+                            int ny, int hw, int edge_fallback, ShiftView acc,
+                            const float2 *shifts_f, int n_shift, float threshold, int p_off,
+                            int *best_ref, float *best_val, float *best_peak, int *fallback,
+                            hipStream_t st) {
     if (n_img <= 0) return hipSuccess;
+    if (nx * ny != n_shift || ny < 1) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_best_prop, dim3((n_img + 127) / 128), dim3(128), 0, st, rec, items, n_img, K,
-                       acc, shifts_f, n_shift, threshold, p_off, best_ref, best_val, best_peak, fallback);
+                       nx, ny, hw, edge_fallback, acc, shifts_f, n_shift, threshold, p_off,
+                       best_ref, best_val, best_peak, fallback);
     return hipGetLastError();
 }
 

@@ -406,10 +406,30 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
+// Soft target of class c of head h given the label: psi classes (mirror-major,
+// n_psi bins) get a circular Gaussian over bins of the labelled mirror, shift
+// classes (x-major grid, ny per row) a 2-D Gaussian over grid steps; sigma 0
+// gives the one-hot target.  Unnormalised.
+__device__ __forceinline__ float soft_target(int h, int c, int lab, int h1, float sp, float ss,
+                                             int ny) {
+    if (h == 1 && sp > 0.0f) {
+        const int np = h1 / 2;
+        if (c / np != lab / np) return 0.0f;
+        int d = abs(c % np - lab % np);
+        d = min(d, np - d);
+        return expf(-0.5f * (float)(d * d) / (sp * sp));
+    }
+    if (h == 2 && ss > 0.0f && ny > 0) {
+        const float dx = (float)(c / ny - lab / ny), dy = (float)(c % ny - lab % ny);
+        return expf(-0.5f * (dx * dx + dy * dy) / (ss * ss));
+    }
+    return c == lab ? 1.0f : 0.0f;
+}
+
 __global__ __launch_bounds__(256) void k_softmax_xent(const float *__restrict__ logits, int n, int ld,
                                                       int h0, int h1, int h2,
-                                                      const int *__restrict__ labels,
-                                                      float *__restrict__ loss,
+                                                      const int *__restrict__ labels, float sp,
+                                                      float ss, int ny, float *__restrict__ loss,
                                                       unsigned short *__restrict__ dlogits) {
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -421,18 +441,25 @@ __global__ __launch_bounds__(256) void k_softmax_xent(const float *__restrict__ 
     float total = 0.0f;
     for (int h = 0; h < 3; ++h) {
         const int lab = min(max(labels[(size_t)i * 3 + h], 0), sz[h] - 1);
-        float m = -FLT_MAX;
-        for (int c = lane; c < sz[h]; c += 64) m = fmaxf(m, z[off[h] + c]);
-        m = wave_max(m);
-        float s = 0.0f;
-        for (int c = lane; c < sz[h]; c += 64) s += expf(z[off[h] + c] - m);
-        s = wave_sum(s);
-        const float lse = m + logf(s);
-        total += lse - z[off[h] + lab];
+        float m = -FLT_MAX, tsum = 0.0f;
         for (int c = lane; c < sz[h]; c += 64) {
-            const float pr = expf(z[off[h] + c] - lse);
-            dz[off[h] + c] = f2bf((pr - (c == lab ? 1.0f : 0.0f)) * inv_n);
+            m = fmaxf(m, z[off[h] + c]);
+            tsum += soft_target(h, c, lab, h1, sp, ss, ny);
         }
+        m = wave_max(m);
+        tsum = wave_sum(tsum);
+        float se = 0.0f;
+        for (int c = lane; c < sz[h]; c += 64) se += expf(z[off[h] + c] - m);
+        se = wave_sum(se);
+        const float lse = m + logf(se);
+        float l = 0.0f;
+        for (int c = lane; c < sz[h]; c += 64) {
+            const float t = soft_target(h, c, lab, h1, sp, ss, ny) / tsum;
+            const float pr = expf(z[off[h] + c] - lse);
+            l += t * (lse - z[off[h] + c]);
+            dz[off[h] + c] = f2bf((pr - t) * inv_n);
+        }
+        total += wave_sum(l);
     }
     for (int c = h0 + h1 + h2 + lane; c < ld; c += 64) dz[c] = 0;
     if (lane == 0) loss[i] = total;
@@ -470,14 +497,14 @@ __global__ void k_pack_weights(const float *__restrict__ master, int Cout, int C
 }
 
 // ------------------------------------------------------------ proposals
-// Wave top-8 of one head: values v[j] at index lane + 64 j (j < J), -inf padded.
-// Returns (in every lane) the k-th best value and index for k < 8, in lane k's
-// registers: top_v / top_i valid in lanes 0..7.
-template <int J>
-__device__ __forceinline__ void head_top8(float (&v)[J], int lane, float &top_v, int &top_i) {
+// Wave top-KT of one head: values v[j] at index lane + 64 j (j < J), -inf padded.
+// Lane k (k < KT) receives the k-th best value and index (value descending,
+// index ascending); index -1 when the head has fewer than k+1 entries.
+template <int J, int KT>
+__device__ __forceinline__ void head_top(float (&v)[J], int lane, float &top_v, int &top_i) {
     top_v = -FLT_MAX;
     top_i = -1;
-    for (int r = 0; r < 8; ++r) {
+    for (int r = 0; r < KT; ++r) {
         float bv = -FLT_MAX;
         int bi = INT_MAX;
 #pragma unroll
@@ -507,11 +534,17 @@ __device__ __forceinline__ void head_top8(float (&v)[J], int lane, float &top_v,
     }
 }
 
-// One wavefront per particle; heads up to 512 (references), 256 (psi classes)
-// and 64 (shifts) entries.
+// One wavefront per particle.  Candidate triples: top-8 references x top-16 psi
+// classes x top-16 shifts (heads up to 512 / 256 / 64 entries); score = sum of
+// the three logits.  Greedy selection of k triples, best first (ties to the
+// lower (ref rank, psi rank, shift rank)); with psi_r >= 0 every selection
+// suppresses the candidates of the same reference and mirror within psi_r psi
+// bins (circular) and shift_r grid steps in x and y (grid rows of ny shifts).
 __global__ __launch_bounds__(256) void k_proposals(const float *__restrict__ logits, int n, int ld,
                                                    int n_ref, int n_psi, int n_shift, int k,
+                                                   int psi_r, int shift_r, int ny,
                                                    int4 *__restrict__ prop) {
+    constexpr int KR = 8, KP = 16, KS = 16, PER = KR * KP * KS / 64;   // 32 per lane
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= n) return;
@@ -524,27 +557,32 @@ __global__ __launch_bounds__(256) void k_proposals(const float *__restrict__ log
     sv[0] = lane < n_shift ? z[n_ref + 2 * n_psi + lane] : -FLT_MAX;
     float tr, tp, ts;
     int ir, ip, is;
-    head_top8<8>(rv, lane, tr, ir);
-    head_top8<4>(pv, lane, tp, ip);
-    head_top8<1>(sv, lane, ts, is);
-    // lane l scores triples (ref rank l/8, psi rank l%8, shift rank c), c = 0..7
-    const float r_l = __shfl(tr, lane >> 3, kWave), p_l = __shfl(tp, lane & 7, kWave);
-    const bool ab_ok = __shfl(ir, lane >> 3, kWave) >= 0 && __shfl(ip, lane & 7, kWave) >= 0;
-    float sc[8];
+    head_top<8, KR>(rv, lane, tr, ir);
+    head_top<4, KP>(pv, lane, tp, ip);
+    head_top<1, KS>(sv, lane, ts, is);
+    // lane l holds candidates c = l * PER + j: ref rank c / 256, psi rank (c / 16) % 16,
+    // shift rank c % 16 (so c orders candidates lexicographically by rank)
+    float sc[PER];
+    int cr[PER], cp[PER], cs[PER];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        const bool ok = ab_ok && __shfl(is, c, kWave) >= 0;
-        sc[c] = ok ? r_l + p_l + __shfl(ts, c, kWave) : -INFINITY;
+    for (int j = 0; j < PER; ++j) {
+        const int c = lane * PER + j;
+        const int a = c / (KP * KS), b = (c / KS) % KP, d = c % KS;
+        cr[j] = __shfl(ir, a, kWave);
+        cp[j] = __shfl(ip, b, kWave);
+        cs[j] = __shfl(is, d, kWave);
+        const float v = __shfl(tr, a, kWave) + __shfl(tp, b, kWave) + __shfl(ts, d, kWave);
+        sc[j] = (cr[j] >= 0 && cp[j] >= 0 && cs[j] >= 0) ? v : -INFINITY;
     }
     for (int r = 0; r < k; ++r) {
         float bv = -FLT_MAX;
         int bi = INT_MAX;
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int idx = lane * 8 + c;   // (ref rank, psi rank, shift rank) in lexicographic order
-            if (sc[c] > bv || (sc[c] == bv && idx < bi)) {
-                bv = sc[c];
-                bi = idx;
+        for (int j = 0; j < PER; ++j) {
+            const int c = lane * PER + j;
+            if (sc[j] > bv || (sc[j] == bv && c < bi)) {
+                bv = sc[j];
+                bi = c;
             }
         }
 #pragma unroll
@@ -560,10 +598,21 @@ __global__ __launch_bounds__(256) void k_proposals(const float *__restrict__ log
             if (lane == 0) prop[(size_t)i * k + r] = make_int4(-1, 0, 0, 0);
             continue;
         }
-        const int a = bi >> 6, b = (bi >> 3) & 7, c = bi & 7;
-        const int ref = __shfl(ir, a, kWave), pc = __shfl(ip, b, kWave), sh = __shfl(is, c, kWave);
+        const int sa = bi / (KP * KS), sb = (bi / KS) % KP, sd = bi % KS;
+        const int ref = __shfl(ir, sa, kWave), pc = __shfl(ip, sb, kWave), sh = __shfl(is, sd, kWave);
         if (lane == 0) prop[(size_t)i * k + r] = make_int4(ref, pc / n_psi, pc % n_psi, sh);
-        if (lane == (bi >> 3)) sc[c] = -INFINITY;
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int c = lane * PER + j;
+            bool sup = c == bi;
+            if (psi_r >= 0 && cr[j] == ref && cp[j] / n_psi == pc / n_psi) {
+                int db = abs(cp[j] % n_psi - pc % n_psi);
+                db = min(db, n_psi - db);
+                const bool near_s = abs(cs[j] / ny - sh / ny) <= shift_r && abs(cs[j] % ny - sh % ny) <= shift_r;
+                sup = sup || (db <= psi_r && near_s);
+            }
+            if (sup) sc[j] = -INFINITY;
+        }
     }
 }
 
@@ -685,13 +734,17 @@ int isac_nn_conv_wgrad(const uint16_t *x, const uint16_t *dz, int n, int H, int 
 }
 
 int isac_nn_softmax_xent(const float *logits, int n, int ld, int h0, int h1, int h2,
-                         const int *labels, float *loss, uint16_t *dlogits, void *stream) {
-    if (n < 0 || h0 < 1 || h1 < 1 || h2 < 1 || ld < h0 + h1 + h2 ||
+                         const int *labels, float sigma_psi, float sigma_shift, int shift_ny,
+                         float *loss, uint16_t *dlogits, void *stream) {
+    if (n < 0 || h0 < 1 || h1 < 1 || h2 < 1 || ld < h0 + h1 + h2 || sigma_psi < 0.0f ||
+        sigma_shift < 0.0f || (sigma_psi > 0.0f && h1 % 2) ||
+        (sigma_shift > 0.0f && (shift_ny < 1 || h2 % shift_ny)) ||
         (n > 0 && (!logits || !labels || !loss || !dlogits)))
         return fail(2, "isac_nn_softmax_xent: bad arguments");
     if (n == 0) return 0;
     hipLaunchKernelGGL(k_softmax_xent, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, (hipStream_t)stream,
-                       logits, n, ld, h0, h1, h2, labels, loss, dlogits);
+                       logits, n, ld, h0, h1, h2, labels, sigma_psi, sigma_shift, shift_ny, loss,
+                       dlogits);
     return launched("isac_nn_softmax_xent");
 }
 
@@ -716,14 +769,16 @@ int isac_nn_pack_weights(const float *master, int Cout, int Cin, int ks, uint16_
 }
 
 int isac_nn_proposals(const float *logits, int n, int ld, int n_ref, int n_psi, int n_shift, int k,
-                      int *prop, void *stream) {
+                      int psi_r, int shift_r, int shift_ny, int *prop, void *stream) {
     if (n < 0 || k < 1 || k > 8 || n_ref < 1 || n_ref > 512 || n_psi < 1 || 2 * n_psi > 256 ||
         n_shift < 1 || n_shift > 64 || ld < n_ref + 2 * n_psi + n_shift ||
+        (psi_r >= 0 && (shift_r < 0 || shift_ny < 1 || n_shift % shift_ny)) ||
         (n > 0 && (!logits || !prop)))
         return fail(2, "isac_nn_proposals: bad arguments (refs <= 512, psi classes <= 256, shifts <= 64, k <= 8)");
     if (n == 0) return 0;
     hipLaunchKernelGGL(k_proposals, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, (hipStream_t)stream,
-                       logits, n, ld, n_ref, n_psi, n_shift, k, reinterpret_cast<int4 *>(prop));
+                       logits, n, ld, n_ref, n_psi, n_shift, k, psi_r, shift_r, shift_ny > 0 ? shift_ny : 1,
+                       reinterpret_cast<int4 *>(prop));
     return launched("isac_nn_proposals");
 }
 

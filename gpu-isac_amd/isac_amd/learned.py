@@ -1,0 +1,108 @@
+"""Learned-proposal form of the multi-reference search.
+
+Pipeline: the exhaustive FP32 search (isac_mref_align_device) labels a
+training set; PoseNet is trained on those labels with its own kernels; a
+threshold on the narrowed search's best CCF is calibrated on held-out
+particles so that the particles below it, sent to the exhaustive search,
+bring the agreement with the exhaustive search to a target; inference is
+PoseNet proposals -> isac_mref_proposal_device.  No counterpart in the
+reference, whose search is always exhaustive (cuda/gpu_aln_noref.cu:326-554).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import binding, posenet
+
+TIE_REL = 1e-5
+
+# Network and training settings of the learned form: the last conv layer keeps
+# its 9x9 map (128 -> 32 channels, no pooling) so the shift head sees position;
+# Gaussian targets over neighbouring psi bins (sigma 1 bin) and shifts (0.7 step).
+NET = dict(channels=(32, 64, 128, 32), hidden=384, pool4=False)
+TRAIN = dict(batch=256, lr=2e-3, wd=1e-4, warmup=1000, sigma_psi=1.0, sigma_shift=0.7)
+
+
+def exhaustive(sbj, ref, ring_num, shift_rng, shift_step=1.0, chunk=20000):
+    """Exhaustive FP32 search in particle chunks: (d [N, P], peak_list [N, 4P])."""
+    import torch
+    P, N = sbj.shape[0], ref.shape[0]
+    d = torch.empty((N, P), dtype=torch.float32, device=sbj.device)
+    peak = torch.empty((N, P, 4), dtype=torch.float32, device=sbj.device)
+    for p0 in range(0, P, chunk):
+        p1 = min(P, p0 + chunk)
+        pk, dd = binding.mref_align_device(sbj[p0:p1], ref, ring_num, shift_step, shift_rng,
+                                           shift_rng)
+        d[:, p0:p1] = dd
+        peak[:, p0:p1] = pk.view(N, p1 - p0, 4)
+    return d, peak.view(N, 4 * P)
+
+
+def labels(sbj, ref, ring_num, shift_rng, shift_step=1.0, n_psi=72):
+    d, peak = exhaustive(sbj, ref, ring_num, shift_rng, shift_step)
+    return posenet.labels_from_search(d, peak, shift_rng, shift_step, n_psi), d, peak
+
+
+def agreement(best_ref, best_peak, d, peak, psi_tol=0.5, tie_rel=TIE_REL):
+    """Per particle: does (best_ref, best_peak) match the exhaustive search?
+    Same reference (either of the top two when they tie within tie_rel), same
+    mirror and shift, psi within psi_tol degrees of that reference's angle."""
+    import torch
+    N, P = d.shape
+    cols = torch.arange(P, device=d.device)
+    top = torch.topk(d, 2, dim=0) if N > 1 else None
+    r1 = torch.argmax(d, dim=0)
+    ok_ref = best_ref.long() == r1
+    if top is not None:
+        v1, v2 = top.values[0], top.values[1]
+        tie = (v1 - v2) <= tie_rel * v1.abs()
+        ok_ref = ok_ref | (tie & (best_ref.long() == top.indices[1]))
+    rr = torch.where(ok_ref, best_ref.long().clamp(0, N - 1), r1)
+    pk = peak.view(N, P, 4)[rr, cols]
+    dpsi = torch.remainder(best_peak[:, 0] - pk[:, 0] + 180.0, 360.0) - 180.0
+    same = ok_ref & (best_peak[:, 3] == pk[:, 3]) & ((best_peak[:, 1] - pk[:, 1]).abs() < 1e-4) & \
+        ((best_peak[:, 2] - pk[:, 2]).abs() < 1e-4) & (dpsi.abs() <= psi_tol)
+    return same
+
+
+def calibrate_threshold(val, agree, target: float = 0.997):
+    """Smallest threshold t such that, with the particles below t sent to the
+    exhaustive search (which then agree), at most (1 - target) of the
+    calibration particles disagree."""
+    val = np.asarray(val, np.float64)
+    bad = np.sort(val[~np.asarray(agree, bool)])[::-1]
+    allowed = int(math.floor((1.0 - target) * val.size))
+    if bad.size <= allowed:
+        return -3.0e38
+    return float(np.nextafter(np.float32(bad[allowed]), np.float32(np.inf)))
+
+
+@dataclass
+class Proposer:
+    net: posenet.PoseNet
+    threshold: float
+    ring_num: int
+    shift_rng: float
+    shift_step: float = 1.0
+    k: int = 8
+    psi_half: float = 10.0
+    shift_half: int = 1
+
+    def run(self, sbj, ref, acc_xy=None):
+        prop = self.net.propose(sbj, self.k)
+        return binding.mref_proposal_device(sbj, ref, self.ring_num, self.shift_step,
+                                            self.shift_rng, self.shift_rng, prop, self.threshold,
+                                            n_psi_bins=self.net.n_psi, psi_half=self.psi_half,
+                                            shift_half=self.shift_half, acc_xy=acc_xy)
+
+
+def calibrate(net, sbj, ref, d, peak, ring_num, shift_rng, target=0.997, **kw):
+    """Proposer whose threshold is calibrated on (sbj, exhaustive d / peak)."""
+    pr = Proposer(net, -3.0e38, ring_num, shift_rng, **kw)
+    o = pr.run(sbj, ref)
+    agree = agreement(o["best_ref"], o["best_peak"], d, peak)
+    pr.threshold = calibrate_threshold(o["best_val"].cpu().numpy(), agree.cpu().numpy(), target)
+    return pr, float(agree.double().mean())

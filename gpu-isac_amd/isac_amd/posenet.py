@@ -37,10 +37,10 @@ def _lib():
         "isac_nn_maxpool2_bwd": [vp, vp, ci, ci, ci, ci, vp, vp],
         "isac_nn_relu_bwd": [vp, vp, sz, vp, vp],
         "isac_nn_conv_wgrad": [vp, vp, ci, ci, ci, ci, ci, ci, vp, vp, sz, vp],
-        "isac_nn_softmax_xent": [vp, ci, ci, ci, ci, ci, vp, vp, vp, vp],
+        "isac_nn_softmax_xent": [vp, ci, ci, ci, ci, ci, vp, cf, cf, ci, vp, vp, vp],
         "isac_nn_adamw": [vp, vp, vp, vp, vp, sz, cf, cf, cf, cf, cf, ci, vp],
         "isac_nn_pack_weights": [vp, ci, ci, ci, vp, vp, vp],
-        "isac_nn_proposals": [vp, ci, ci, ci, ci, ci, ci, vp, vp],
+        "isac_nn_proposals": [vp, ci, ci, ci, ci, ci, ci, ci, ci, ci, vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -84,7 +84,7 @@ class PoseNet:
 
     def __init__(self, n_ref: int, n_shift: int, D: int = 76, n_psi: int = 72,
                  channels=(32, 64, 128, 128), hidden: int = 384, seed: int = 0,
-                 device="cuda"):
+                 pool4: bool = True, device="cuda"):
         import torch
         if channels[0] != 32 or any(c % 16 for c in channels[1:]) or hidden % 16:
             raise ValueError("channels must be (32, 16k, 16k, 16k) and hidden a multiple of 16")
@@ -100,6 +100,9 @@ class PoseNet:
             s //= 2
         if s < 1:
             raise ValueError("image too small for four 2x2 poolings")
+        self.pool4 = pool4
+        if not pool4:
+            s = self.sizes[3]
         self.feat = channels[3] * s * s
         self.n_out = sum(self.heads)
         self.ld = (self.n_out + 15) // 16 * 16
@@ -182,8 +185,11 @@ class PoseNet:
                 b, bs = self._bias(li)
                 _call(lib, "isac_nn_conv_fwd", _p(a), n, H, H, cin, _p(self.w[li]), b, bs, cout, 3,
                       1, 0, _p(y), st)
-            a = e(n, H // 2, H // 2, cout)
-            _call(lib, "isac_nn_maxpool2", _p(y), n, H, H, cout, _p(a), st)
+            if li == 3 and not self.pool4:
+                a = y
+            else:
+                a = e(n, H // 2, H // 2, cout)
+                _call(lib, "isac_nn_maxpool2", _p(y), n, H, H, cout, _p(a), st)
             sv[f"y{li + 1}"], sv[f"a{li + 1}"] = y, a
         h = e(n, self.hidden)
         b, bs = self._bias(4)
@@ -198,9 +204,11 @@ class PoseNet:
         return logits
 
     # ----------------------------------------------------------------- backward
-    def backward(self, labels):
+    def backward(self, labels, sigma_psi: float = 0.0, sigma_shift: float = 0.0):
         """Gradients of the mean loss of the last training forward into the
-        gradient buffer; returns the per-particle loss (device tensor)."""
+        gradient buffer; returns the per-particle loss (device tensor).
+        sigma_psi / sigma_shift > 0 use Gaussian targets over neighbouring psi
+        bins / shifts (the shift grid is taken as square)."""
         import torch
         sv = self.saved
         if sv is None:
@@ -212,8 +220,9 @@ class PoseNet:
         loss = torch.empty(n, dtype=torch.float32, device=self.dev)
         dl = e(n, self.ld)
         h0, h1, h2 = self.heads
+        ny = int(round(math.sqrt(h2)))
         _call(lib, "isac_nn_softmax_xent", _p(sv["logits"]), n, self.ld, h0, h1, h2, _p(labels),
-              _p(loss), _p(dl), st)
+              sigma_psi, sigma_shift, ny if ny * ny == h2 else 0, _p(loss), _p(dl), st)
         pb = self.part.numel() * 4
         _call(lib, "isac_nn_conv_wgrad", _p(sv["h"]), _p(dl), n, 1, 1, self.hidden, self.ld, 1,
               _p(self.grad(5)), _p(self.part), pb, st)
@@ -231,8 +240,11 @@ class PoseNet:
             H = self.sizes[li]
             cin, cout, _ = self.layers[li]
             dz = e(n, H, H, cout)
-            _call(lib, "isac_nn_maxpool2_bwd", _p(sv[f"y{li + 1}"]), _p(da), n, H, H, cout,
-                  _p(dz), st)
+            if li == 3 and not self.pool4:
+                _call(lib, "isac_nn_relu_bwd", _p(sv["y4"]), _p(da), n * H * H * cout, _p(dz), st)
+            else:
+                _call(lib, "isac_nn_maxpool2_bwd", _p(sv[f"y{li + 1}"]), _p(da), n, H, H, cout,
+                      _p(dz), st)
             xin = sv["x0"] if li == 0 else sv[f"a{li}"]
             _call(lib, "isac_nn_conv_wgrad", _p(xin), _p(dz), n, H, H, cin, cout, 3,
                   _p(self.grad(li)), _p(self.part), pb, st)
@@ -252,7 +264,8 @@ class PoseNet:
 
     # ---------------------------------------------------------------- training
     def train(self, imgs, labels, epochs: int, batch: int = 256, lr: float = 2e-3,
-              wd: float = 1e-4, warmup: int = 200, seed: int = 0, log=None):
+              wd: float = 1e-4, warmup: int = 200, seed: int = 0, sigma_psi: float = 0.0,
+              sigma_shift: float = 0.0, log=None):
         """Mini-batch AdamW with linear warm-up and cosine decay over ``epochs``
         passes of (imgs [N, D, D], labels [N, 3] int32), both on the device."""
         import torch
@@ -271,7 +284,7 @@ class PoseNet:
                 x = imgs.index_select(0, idx).contiguous()
                 y = labels.index_select(0, idx).contiguous()
                 self.forward(x, train=True)
-                loss = self.backward(y)
+                loss = self.backward(y, sigma_psi, sigma_shift)
                 w = min(1.0, (it + 1) / warmup)
                 cur = lr * w * 0.5 * (1.0 + math.cos(math.pi * it / total))
                 self.step(cur, wd)
@@ -285,16 +298,21 @@ class PoseNet:
         return hist
 
     # --------------------------------------------------------------- inference
-    def propose(self, imgs, k: int = 8, chunk: int = 4096):
-        """Top-k proposals [n, k, 4] int32 (ref, mirror, psi bin, shift index)."""
+    def propose(self, imgs, k: int = 8, chunk: int = 4096, psi_r: int = 2, shift_r: int = 1):
+        """Top-k proposals [n, k, 4] int32 (ref, mirror, psi bin, shift index),
+        spread by suppressing neighbours within psi_r bins and shift_r steps
+        (psi_r < 0: plain top-k of the summed logits)."""
         import torch
         n = imgs.shape[0]
+        ny = int(round(math.sqrt(self.n_shift)))
+        if ny * ny != self.n_shift:
+            psi_r, ny = -1, 1
         prop = torch.empty((n, k, 4), dtype=torch.int32, device=self.dev)
         for i0 in range(0, n, chunk):
             i1 = min(n, i0 + chunk)
             logits = self.forward(imgs[i0:i1])
             _call(self.lib, "isac_nn_proposals", _p(logits), i1 - i0, self.ld, self.n_ref,
-                  self.n_psi, self.n_shift, k, _p(prop[i0:i1]), self._st())
+                  self.n_psi, self.n_shift, k, psi_r, shift_r, ny, _p(prop[i0:i1]), self._st())
         return prop
 
 

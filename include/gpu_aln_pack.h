@@ -149,8 +149,9 @@ void gpu_clear_error(void);
  *   d_sbj [n_sbj][img_dim][img_dim], d_ref [n_ref][img_dim][img_dim] float32,
  *   d_acc_xy [n_sbj][2] accumulated shifts (NULL = zero),
  *   d_peak_list [n_ref][4*n_sbj], d_d [n_ref][d_pitch].
- * `stream` is a hipStream_t (NULL = the library's own stream); the call
- * returns after the work on it has completed.  Returns 0 or an error code. */
+ * `stream` is a hipStream_t; the work is queued on it after whatever the
+ * caller queued there before (NULL = the null stream), and the call returns
+ * after it has completed.  Returns 0 or an error code. */
 int isac_mref_align_device(const float *d_sbj, const float *d_ref, const float *d_acc_xy,
                            unsigned int n_sbj, unsigned int n_ref, unsigned int img_dim,
                            unsigned int ring_num, unsigned int ring_len,
@@ -194,8 +195,11 @@ int isac_mref_best_mixed_device(const float *d_sbj, const float *d_ref, const fl
  * ignored).  Each proposal is searched over the shifts within shift_half grid
  * steps and the psi samples within psi_half_deg of its bin centre, with the
  * arithmetic of isac_mref_align_device, so a window holding the exhaustive
- * optimum returns it bit for bit.  Particles whose best value is below
- * `threshold` are searched exhaustively (fallback = 1).  Outputs per particle:
+ * optimum returns it bit for bit.  Particles are searched exhaustively
+ * (fallback = 1) when their best value is below `threshold`, or, with
+ * edge_fallback != 0, when that maximum lies on the edge of the admitted
+ * region (a psi or shift neighbour that no window of its reference admits),
+ * where the windows may have cut the search short.  Outputs per particle:
  * best reference, its peak value, {angle, shift_x, shift_y, mirror} in
  * d_best_peak [n_sbj][4], the fallback flag; *n_fallback (host) = number of
  * fallbacks.  Returns after the work has completed. */
@@ -204,9 +208,9 @@ int isac_mref_proposal_device(const float *d_sbj, const float *d_ref, const floa
                               unsigned int ring_num, unsigned int ring_len, float shift_step,
                               float shift_rng_x, float shift_rng_y, const int *d_prop,
                               unsigned int n_prop, unsigned int n_psi_bins, float psi_half_deg,
-                              unsigned int shift_half, float threshold, int *d_best_ref,
-                              float *d_best_val, float *d_best_peak, int *d_fallback,
-                              unsigned int *n_fallback, void *stream);
+                              unsigned int shift_half, float threshold, int edge_fallback,
+                              int *d_best_ref, float *d_best_val, float *d_best_peak,
+                              int *d_fallback, unsigned int *n_fallback, void *stream);
 
 /* No counterpart in the reference (the reference aligns one subset per
  * multi_ref_align call, cuda/gpu_aln_noref.cu:326-554, driven once per ISAC

@@ -66,11 +66,16 @@ int isac_nn_conv_wgrad(const uint16_t *x, const uint16_t *dz, int n, int H, int 
                        int Cout, int ks, float *g, float *part, size_t part_bytes, void *stream);
 
 /* Softmax cross-entropy over three heads of logits [n][ld] (fp32): columns
- * [0, h0), [h0, h0+h1), [h0+h1, h0+h1+h2); labels [n][3].  loss[i] = sum of
- * the three head losses; dlogits [n][ld] bf16 = (softmax - onehot) / n, zero
- * in the columns after the heads. */
+ * [0, h0) references, [h0, h0+h1) psi classes (h1 = 2 n_psi, mirror-major),
+ * [h0+h1, h0+h1+h2) shifts (x-major grid, shift_ny per row); labels [n][3].
+ * Targets are one-hot, or with sigma_psi > 0 a circular Gaussian (sigma in
+ * bins) over the psi bins of the labelled mirror and with sigma_shift > 0 a
+ * 2-D Gaussian (sigma in grid steps) over the shifts, normalised per head.
+ * loss[i] = sum of the three head cross-entropies; dlogits [n][ld] bf16 =
+ * (softmax - target) / n, zero in the columns after the heads. */
 int isac_nn_softmax_xent(const float *logits, int n, int ld, int h0, int h1, int h2,
-                         const int *labels, float *loss, uint16_t *dlogits, void *stream);
+                         const int *labels, float sigma_psi, float sigma_shift, int shift_ny,
+                         float *loss, uint16_t *dlogits, void *stream);
 
 /* AdamW step t (1-based) over count parameters: p -= lr * (mhat / (sqrt(vhat) +
  * eps) + wd * p); mask (or NULL) zeroes the update where mask[i] == 0. */
@@ -85,11 +90,16 @@ int isac_nn_pack_weights(const float *master, int Cout, int Cin, int ks, uint16_
                          uint16_t *wflip, void *stream);
 
 /* Top-k (k <= 8) proposals per particle from logits [n][ld]: heads of n_ref
- * references, 2*n_psi (mirror-major) psi classes and n_shift shifts.  Scores
- * add the three logits; prop[i][j] = {ref, mirror, psi bin, shift index},
- * best first (ties to the lower ranks). */
+ * references, 2*n_psi (mirror-major) psi classes and n_shift shifts.  The
+ * candidates are the triples of the top-8 references, top-16 psi classes and
+ * top-16 shifts, scored by the sum of their three logits and taken greedily,
+ * best first (ties to the lower ranks).  With psi_r >= 0 each choice removes
+ * the candidates of the same reference and mirror within psi_r bins and
+ * shift_r grid steps (x-major grid of shift_ny per row), so that the narrowed
+ * windows spread out.  prop[i][j] = {ref, mirror, psi bin, shift index};
+ * {-1, 0, 0, 0} when fewer than k candidates remain. */
 int isac_nn_proposals(const float *logits, int n, int ld, int n_ref, int n_psi, int n_shift,
-                      int k, int *prop, void *stream);
+                      int k, int psi_r, int shift_r, int shift_ny, int *prop, void *stream);
 
 #ifdef __cplusplus
 }

@@ -506,7 +506,9 @@ def proposal_mask(prop_p, n_ref, nx, ny, n_bins, psi_half, hw, ring_len=RING_LEN
 def windowed_best(sbj, ref, acc_xy, ring_num, shift_rng_x, shift_rng_y, shift_step, prop,
                   n_bins=72, psi_half=10.0, hw=1, ring_len=RING_LEN):
     """Best (ref, value, angle, sx, sy, mirror, s, t) per particle over the
-    proposals' windows; ref = -1 when nothing is admitted."""
+    proposals' windows; ref = -1 when nothing is admitted.  ``interior``: every
+    neighbour of the maximum (psi t +- 1, shift +- 1 in x and y inside the grid)
+    is admitted for its reference and mirror."""
     sbj = np.asarray(sbj)
     acc_xy = np.asarray(acc_xy, dtype=np.float64).reshape(-1, 2)
     P, N = sbj.shape[0], ref.shape[0]
@@ -517,6 +519,7 @@ def windowed_best(sbj, ref, acc_xy, ring_num, shift_rng_x, shift_rng_y, shift_st
     rspec = reference_spectra(np.asarray(ref), ring_num, ring_len)
     out = {k: np.zeros(P) for k in ("d", "angle", "sx", "sy", "mirror", "shift_idx", "t")}
     out["ref"] = np.full(P, -1, np.int64)
+    out["interior"] = np.zeros(P, bool)
     for p in range(P):
         mask = proposal_mask(prop[p], N, nx, ny, n_bins, psi_half, hw, ring_len)
         refs = np.nonzero(mask.reshape(N, -1).any(1))[0]
@@ -538,4 +541,9 @@ def windowed_best(sbj, ref, acc_xy, ring_num, shift_rng_x, shift_rng_y, shift_st
         out["sx"][p] = acc_xy[p, 0] + shifts[s, 0]
         out["sy"][p] = acc_xy[p, 1] + shifts[s, 1]
         out["mirror"][p], out["shift_idx"][p], out["t"][p] = m, s, t
+        ix, iy = divmod(int(s), ny)
+        nb = [(s, (t + 1) % L), (s, (t - 1) % L)]
+        nb += [(s + dx * ny + dy, t) for dx, dy in ((-1, 0), (1, 0), (0, -1), (0, 1))
+               if 0 <= ix + dx < nx and 0 <= iy + dy < ny]
+        out["interior"][p] = all(mask[refs[j], m, s2, t2] for s2, t2 in nb)
     return out

@@ -44,6 +44,7 @@ class NetSpec:
     channels: tuple = (32, 64, 128, 128)
     hidden: int = 384
     heads: tuple = (100, 144, 49)
+    pool4: bool = True                           # 2x2 max pool after the last conv layer
     sizes: list = field(default_factory=list)   # spatial size entering conv l (l = 1..4)
 
     def __post_init__(self):
@@ -51,6 +52,8 @@ class NetSpec:
         for _ in range(4):
             self.sizes.append(s)
             s //= 2
+        if not self.pool4:
+            s = self.sizes[3]
         self.feat = self.channels[3] * s * s
         self.n_out = sum(self.heads)
         self.ld = (self.n_out + 15) // 16 * 16
@@ -158,7 +161,10 @@ def forward(spec: NetSpec, params, img, quant=True):
         cin, cout, ks = spec.layers()[l]
         P = params[l]
         y = q(np.maximum(conv(a, q(P[:, :-1]), P[:, -1].astype(np.float64), 3), 0.0))
-        a, arg = pool(y)
+        if l == 3 and not spec.pool4:
+            a, arg = y, None
+        else:
+            a, arg = pool(y)
         act[f"y{l + 1}"], act[f"arg{l + 1}"], act[f"a{l + 1}"] = y, arg, a
     f = a.reshape(n, -1)
     act["f"] = f
@@ -169,28 +175,52 @@ def forward(spec: NetSpec, params, img, quant=True):
     return act
 
 
-def xent(spec: NetSpec, logits, labels):
-    """Per-particle summed head losses and d(mean loss)/d logits."""
+def soft_targets(spec: NetSpec, labels, sigma_psi=0.0, sigma_shift=0.0, shift_ny=0):
+    """Per-head target distributions: one-hot, or Gaussian over psi bins of the
+    labelled mirror (circular, sigma in bins) and over the shift grid (x-major,
+    shift_ny per row, sigma in steps)."""
+    n = labels.shape[0]
+    out = []
+    for h, sz in enumerate(spec.heads):
+        c = np.arange(sz)
+        lab = labels[:, h][:, None]
+        if h == 1 and sigma_psi > 0:
+            npsi = sz // 2
+            d = np.abs(c[None] % npsi - lab % npsi)
+            d = np.minimum(d, npsi - d)
+            t = np.where(c[None] // npsi == lab // npsi, np.exp(-0.5 * d * d / sigma_psi ** 2), 0.0)
+        elif h == 2 and sigma_shift > 0:
+            dx = c[None] // shift_ny - lab // shift_ny
+            dy = c[None] % shift_ny - lab % shift_ny
+            t = np.exp(-0.5 * (dx * dx + dy * dy) / sigma_shift ** 2)
+        else:
+            t = (c[None] == lab).astype(np.float64)
+        out.append(t / t.sum(1, keepdims=True))
+    return out
+
+
+def xent(spec: NetSpec, logits, labels, sigma_psi=0.0, sigma_shift=0.0, shift_ny=0):
+    """Per-particle summed head cross-entropies and d(mean loss)/d logits."""
     n = logits.shape[0]
     loss = np.zeros(n)
     g = np.zeros_like(logits)
+    tg = soft_targets(spec, labels, sigma_psi, sigma_shift, shift_ny)
     o = 0
     for h, sz in enumerate(spec.heads):
         z = logits[:, o:o + sz]
         m = z.max(1, keepdims=True)
         lse = m[:, 0] + np.log(np.exp(z - m).sum(1))
-        loss += lse - z[np.arange(n), labels[:, h]]
+        loss += (tg[h] * (lse[:, None] - z)).sum(1)
         pr = np.exp(z - lse[:, None])
-        pr[np.arange(n), labels[:, h]] -= 1.0
-        g[:, o:o + sz] = pr / n
+        g[:, o:o + sz] = (pr - tg[h]) / n
         o += sz
     return loss, g
 
 
-def backward(spec: NetSpec, params, act, labels, quant=True):
-    """(per-particle loss, gradients [Cout][K+1] per layer)."""
+def backward(spec: NetSpec, params, act, labels, quant=True, **soft):
+    """(per-particle loss, gradients [Cout][K+1] per layer); ``soft`` as xent."""
     q = bf16 if quant else (lambda v: np.asarray(v, np.float64))
-    loss, g = xent(spec, act["logits"], labels)
+    loss, g = xent(spec, act["logits"], labels, **soft)
     dl = q(g)
     grads = [None] * 6
     h, f = act["h"], act["f"]
@@ -202,7 +232,7 @@ def backward(spec: NetSpec, params, act, labels, quant=True):
     for l in range(3, -1, -1):
         cin, cout, ks = spec.layers()[l]
         y, arg = act[f"y{l + 1}"], act[f"arg{l + 1}"]
-        dz = pool_bwd(y, arg, da)
+        dz = np.where(y > 0, da, 0.0) if arg is None else pool_bwd(y, arg, da)
         xin = act["x0"] if l == 0 else act[f"a{l}"]
         n, H, W, _ = xin.shape
         cols = im2col(xin, 3).reshape(n * H * W, 9 * cin)
@@ -225,26 +255,45 @@ def adamw(p, g, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
     return p, m, v
 
 
-def proposals(spec_heads, logits, k=8):
-    """Top-k (ref, mirror, psi bin, shift) triples by summed head logits
-    (brute force over every triple; ties to the lower (ref, psi, shift) ranks)."""
+def proposals(spec_heads, logits, k=8, psi_r=-1, shift_r=1, shift_ny=1, pool=(8, 16, 16)):
+    """Greedy top-k (ref, mirror, psi bin, shift) triples by summed head logits
+    over the candidate pool (top pool[0] references x top pool[1] psi classes x
+    top pool[2] shifts; ranks by value, ties to the lower index), best first,
+    ties to the lower (ref, psi, shift) ranks.  psi_r >= 0 suppresses, after each
+    choice, the candidates of the same reference and mirror within psi_r bins
+    (circular) and shift_r steps in x and y.  Brute force: small heads only."""
     n_ref, n_psi2, n_shift = spec_heads
     n_psi = n_psi2 // 2
-    out = np.zeros((logits.shape[0], k, 4), np.int64)
+    out = np.full((logits.shape[0], k, 4), 0, np.int64)
     for i, z in enumerate(np.asarray(logits, np.float32)):
         zr, zp, zs = z[:n_ref], z[n_ref:n_ref + n_psi2], z[n_ref + n_psi2:n_ref + n_psi2 + n_shift]
-        # rank order of each head: value descending, index ascending
-        rr = sorted(range(n_ref), key=lambda j: (-zr[j], j))
-        rp = sorted(range(n_psi2), key=lambda j: (-zp[j], j))
-        rs = sorted(range(n_shift), key=lambda j: (-zs[j], j))
-        trip = []
+        rr = sorted(range(n_ref), key=lambda j: (-zr[j], j))[:pool[0]]
+        rp = sorted(range(n_psi2), key=lambda j: (-zp[j], j))[:pool[1]]
+        rs = sorted(range(n_shift), key=lambda j: (-zs[j], j))[:pool[2]]
+        cand = []
         for a, ri in enumerate(rr):
             for b, pi in enumerate(rp):
                 for c, si in enumerate(rs):
-                    trip.append((-(np.float32(zr[ri]) + np.float32(zp[pi]) + np.float32(zs[si])),
+                    cand.append((-(np.float32(zr[ri]) + np.float32(zp[pi]) + np.float32(zs[si])),
                                  (a, b, c), ri, pi, si))
-        trip.sort(key=lambda t: (t[0], t[1]))
+        cand.sort(key=lambda t: (t[0], t[1]))
+        alive = [True] * len(cand)
         for j in range(k):
-            _, _, ri, pi, si = trip[j]
+            pick = next((q for q in range(len(cand)) if alive[q]), None)
+            if pick is None:
+                out[i, j] = (-1, 0, 0, 0)
+                continue
+            _, _, ri, pi, si = cand[pick]
             out[i, j] = (ri, pi // n_psi, pi % n_psi, si)
+            for q, (_, _, r2, p2, s2) in enumerate(cand):
+                if not alive[q]:
+                    continue
+                if q == pick:
+                    alive[q] = False
+                elif psi_r >= 0 and r2 == ri and p2 // n_psi == pi // n_psi:
+                    db = abs(p2 % n_psi - pi % n_psi)
+                    db = min(db, n_psi - db)
+                    if db <= psi_r and abs(s2 // shift_ny - si // shift_ny) <= shift_r and \
+                            abs(s2 % shift_ny - si % shift_ny) <= shift_r:
+                        alive[q] = False
     return out

@@ -13,7 +13,8 @@ def _net(n_ref=100, n_shift=49, D=76, seed=0):
 
 
 def _spec(net):
-    return N.NetSpec(D=net.D, channels=net.channels, hidden=net.hidden, heads=net.heads)
+    return N.NetSpec(D=net.D, channels=net.channels, hidden=net.hidden, heads=net.heads,
+                     pool4=net.pool4)
 
 
 def _perturbed_params(net, seed):
@@ -50,20 +51,24 @@ def test_forward_matches_oracle(gpu):
     assert np.all(logits[:, net.n_out:] == 0.0)
 
 
-def test_gradients_match_oracle(gpu):
+@pytest.mark.parametrize("pool4,sp,ss", [(True, 0.0, 0.0), (False, 1.0, 0.7)])
+def test_gradients_match_oracle(gpu, pool4, sp, ss):
     """Per-layer gradients (weights and biases) of the mean loss within 2e-2
-    (relative, Frobenius) of the oracle with the same bf16 rounding points."""
+    (relative, Frobenius) of the oracle with the same bf16 rounding points;
+    one-hot and Gaussian targets, with and without the last pooling."""
     import torch
-    net = _net()
+    from isac_amd import posenet
+    net = posenet.PoseNet(100, 49, D=76, seed=0, pool4=pool4,
+                          channels=(32, 64, 128, 128 if pool4 else 32))
     ps = _perturbed_params(net, 3)
     img = _images(8, 76, 4)
     rng = np.random.default_rng(5)
     lab = np.stack([rng.integers(0, 100, 8), rng.integers(0, 144, 8), rng.integers(0, 49, 8)], 1)
     net.forward(torch.from_numpy(img).cuda(), train=True)
-    loss = net.backward(torch.from_numpy(lab.astype(np.int32)).cuda()).cpu().numpy()
+    loss = net.backward(torch.from_numpy(lab.astype(np.int32)).cuda(), sp, ss).cpu().numpy()
     spec = _spec(net)
     act = N.forward(spec, ps, img, quant=True)
-    oloss, og = N.backward(spec, ps, act, lab, quant=True)
+    oloss, og = N.backward(spec, ps, act, lab, quant=True, sigma_psi=sp, sigma_shift=ss, shift_ny=7)
     assert _rel(loss, oloss) < 1e-3, (loss, oloss)
     for li in range(6):
         g = net.grad(li).cpu().numpy()
@@ -85,20 +90,22 @@ def test_adamw_step_matches_oracle(gpu):
     np.testing.assert_allclose(net.m.cpu().numpy(), m1, rtol=1e-6, atol=1e-9)
 
 
-def test_proposals_kernel_matches_brute_force(gpu):
+@pytest.mark.parametrize("psi_r", [-1, 1, 2])
+def test_proposals_kernel_matches_brute_force(gpu, psi_r):
     import torch
     from isac_amd import posenet
     lib = posenet._lib()
     rng = np.random.default_rng(8)
-    heads = (10, 12, 9)
-    n, ld = 37, 32
+    heads = (10, 24, 9)
+    n, ld = 37, 48
     z = rng.standard_normal((n, ld)).astype(np.float32)
     z[3, :10] = 0.5   # ties inside a head
+    z[5, 10:34] = 0.25
     zt = torch.from_numpy(z).cuda()
     prop = torch.empty((n, 8, 4), dtype=torch.int32, device="cuda")
-    posenet._call(lib, "isac_nn_proposals", zt.data_ptr(), n, ld, 10, 6, 9, 8, prop.data_ptr(),
-                  torch.cuda.current_stream().cuda_stream)
-    want = N.proposals(heads, z, k=8)
+    posenet._call(lib, "isac_nn_proposals", zt.data_ptr(), n, ld, 10, 12, 9, 8, psi_r, 1, 3,
+                  prop.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    want = N.proposals(heads, z, k=8, psi_r=psi_r, shift_r=1, shift_ny=3)
     np.testing.assert_array_equal(prop.cpu().numpy(), want)
 
 

@@ -343,6 +343,7 @@ def test_windowed_search_holding_the_optimum_equals_exhaustive():
     np.testing.assert_array_equal(w["ref"], best)
     np.testing.assert_array_equal(w["d"], full["d"][best, cols])
     np.testing.assert_array_equal(w["angle"], full["angle"][best, cols])
+    assert w["interior"].all()          # centred windows: every neighbour admitted
     far = prop.copy()
     far[:, 0, 2] = (b + 36) % 72
     w2 = O.windowed_best(ds.sbj, ds.ref, acc, 19, 2.0, 2.0, 1.0, far)

@@ -53,22 +53,24 @@ def test_pool_first_max_and_backward_routing():
     assert dz.sum() == 2.0      # windows whose maximum is 0 pass nothing (ReLU)
 
 
-def test_backward_matches_finite_differences():
-    spec = N.NetSpec(D=20, channels=(4, 6, 8, 8), hidden=10, heads=(3, 4, 5))
+@pytest.mark.parametrize("pool4,soft", [(True, {}),
+                                        (False, dict(sigma_psi=0.8, sigma_shift=0.6, shift_ny=3))])
+def test_backward_matches_finite_differences(pool4, soft):
+    spec = N.NetSpec(D=20, channels=(4, 6, 8, 8), hidden=10, heads=(3, 4, 9), pool4=pool4)
     params = N.init_params(spec, seed=3)
     params = [p.astype(np.float64) for p in params]
     rng = np.random.default_rng(4)
     for p in params:
         p[:, -1] = rng.standard_normal(p.shape[0]) * 0.1
     img = rng.standard_normal((3, 20, 20)).astype(np.float32)
-    lab = np.stack([rng.integers(0, 3, 3), rng.integers(0, 4, 3), rng.integers(0, 5, 3)], 1)
+    lab = np.stack([rng.integers(0, 3, 3), rng.integers(0, 4, 3), rng.integers(0, 9, 3)], 1)
 
     def loss_of(ps):
         act = N.forward(spec, ps, img, quant=False)
-        return N.xent(spec, act["logits"], lab)[0].mean()
+        return N.xent(spec, act["logits"], lab, **soft)[0].mean()
 
     act = N.forward(spec, params, img, quant=False)
-    _, grads = N.backward(spec, params, act, lab, quant=False)
+    _, grads = N.backward(spec, params, act, lab, quant=False, **soft)
     eps = 1e-6
     checked = 0
     for li, p in enumerate(params):
@@ -115,3 +117,29 @@ def test_proposals_brute_force_orders_by_summed_logits():
         r, pc, sh = np.unravel_index(order, s.shape)
         want = np.stack([r, pc // 3, pc % 3, sh], 1)
         np.testing.assert_array_equal(got[i], want)
+
+
+def test_soft_targets_shape():
+    spec = N.NetSpec(D=20, channels=(4, 6, 8, 8), hidden=10, heads=(3, 8, 9))
+    t = N.soft_targets(spec, np.array([[0, 5, 4]]), sigma_psi=1.0, sigma_shift=1.0, shift_ny=3)
+    assert np.allclose([x.sum() for x in t], 1.0)
+    assert np.all(t[1][0, :4] == 0.0)                      # other mirror: no weight
+    assert abs(t[1][0, 4] - t[1][0, 6]) < 1e-15            # circular neighbours of bin 1
+    assert t[2][0].argmax() == 4 and abs(t[2][0, 1] - t[2][0, 3]) < 1e-15
+
+
+def test_proposal_suppression_spreads_windows():
+    """With suppression no two proposals of one reference and mirror lie within
+    the suppression radius of each other; without it the top-k may."""
+    rng = np.random.default_rng(11)
+    z = rng.standard_normal((5, 3 + 24 + 9)).astype(np.float32)
+    z[:, 0] += 5.0                                   # one dominant reference
+    got = N.proposals((3, 24, 9), z, k=6, psi_r=2, shift_r=1, shift_ny=3)
+    for p in got:
+        for a in range(6):
+            for b in range(a):
+                if p[a, 0] == p[b, 0] and p[a, 1] == p[b, 1]:
+                    db = abs(p[a, 2] - p[b, 2])
+                    db = min(db, 12 - db)
+                    near = abs(p[a, 3] // 3 - p[b, 3] // 3) <= 1 and abs(p[a, 3] % 3 - p[b, 3] % 3) <= 1
+                    assert not (db <= 2 and near)
