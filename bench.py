@@ -137,6 +137,19 @@ def precision_report(n_particles=1000):
     return out
 
 
+def traffic_per_launch(pairs_per_launch):
+    """HBM bytes per k_ccf_mref launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the factor a 1 GiB copy
+    gives), as bytes per pair times the pairs of this launch; None without it."""
+    path = os.path.join(ROOT, "profiles", "k_ccf_mref_traffic_cfg2.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t["hbm_bytes_per_pair"] * pairs_per_launch
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def batched_report(steps=3, warmup=1, n_subsets=64):
     """Config 4: 64 uneven subsets (500..5000 particles, box 64/76/96/128, 20..200
     references, +-2..+-4 shifts, rings to box/2-4) in one captured graph around
@@ -242,7 +255,9 @@ def learned_report(n_train=200000, n_cal=10000, n_test=100000, epochs=16, oracle
     t_learn, o = timed(lambda: pr.run(te, ref))
     t_cnn, prop = timed(lambda: net.propose(te, 8))
     t_narrow, on = timed(lambda: binding.mref_proposal_device(te, ref, R, 1.0, rng, rng, prop,
-                                                              threshold=-3.0e38))
+                                                              threshold=-3.0e38,
+                                                              edge_fallback=False))
+    oe = binding.mref_proposal_device(te, ref, R, 1.0, rng, rng, prop, threshold=-3.0e38)
     agree = learned.agreement(o["best_ref"], o["best_peak"], d, peak)
     agree_n = learned.agreement(on["best_ref"], on["best_peak"], d, peak)
     un = learned.Proposer(posenet.PoseNet(N, 49, D=D, seed=1, **learned.NET), pr.threshold, R, rng)
@@ -252,6 +267,7 @@ def learned_report(n_train=200000, n_cal=10000, n_test=100000, epochs=16, oracle
         "agreement_with_exhaustive_fp32": float(agree.double().mean()),
         "narrowed_only_agreement": float(agree_n.double().mean()),
         "fallback_fraction": o["n_fallback"] / n_test,
+        "edge_fallback_fraction": oe["n_fallback"] / n_test,
         "plain_fp32_s": t_plain, "mfma_fp16_best_s": t_mfma, "learned_s": t_learn,
         "cnn_proposals_s": t_cnn, "narrowed_without_fallback_s": t_narrow,
         "speedup_vs_plain_fp32": t_plain / t_learn, "speedup_vs_mfma": t_mfma / t_learn,
@@ -288,6 +304,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the mixed-precision forms")
+    ap.add_argument("--probe", action="store_true",
+                    help="one step of the workload plus a 1 GiB device copy, nothing printed: the "
+                         "target of rocprofv3 --pmc runs (the copy calibrates the byte counters)")
     ap.add_argument("--report", choices=["precision", "batched", "learned"], default=None,
                     help="precision: config-3 accuracy report; batched: config-4 uneven-subset "
                          "throughput, graph vs one launch per subset")
@@ -330,6 +349,14 @@ def main():
     def step():
         binding.mref_align_device(sbj, ref, R, 1.0, rng, rng, acc_xy=acc, out=(peak, dval))
 
+    if args.probe:
+        step()
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)   # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        return
+
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -365,7 +392,8 @@ def main():
                    "particles": P, "refs": N, "img_dim": D, "rings": R, "shifts": S},
         "roofline": {"bound": "mfma", "kernel": "k_ccf_mref (ring contraction + psi IFFT + search)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": traffic_per_launch(pairs_per_launch),
                      "avg_launch_ms": avg_launch_ms, "launches": launches,
                      "flop_per_pair": flops_per_pair(S, R),
                      "polar_kernel_ms_per_step": ms_pol.value / args.steps},

@@ -129,6 +129,7 @@ struct Workspace {
     DevBuf mix_p, mix_pn, mix_r, mix_rn;   // 16-bit MFMA operands and row norms
     DevBuf cand, rec_c, best, second, tie, val64;   // candidate re-scoring
     DevBuf items, smask, fb_idx, fb_count, fb_img, fb_acc;   // narrowed search + fallback
+    DevBuf edge;                                             // edge-test flags
     std::vector<float2> shifts_uploaded;
     BatchWs batch;
     void release() {
@@ -136,7 +137,8 @@ struct Workspace {
         DevBuf *all[] = {&tmpl, &twiddle, &zero_shift, &shifts_i, &shifts_f, &spec, &norm,
                          &rspec, &rnorm, &rsum0, &rec, &peak, &dvals, &acc,
                          &mix_p, &mix_pn, &mix_r, &mix_rn, &cand, &rec_c, &best, &second,
-                         &tie, &val64, &items, &smask, &fb_idx, &fb_count, &fb_img, &fb_acc};
+                         &tie, &val64, &items, &smask, &fb_idx, &fb_count, &fb_img, &fb_acc,
+                         &edge};
         for (DevBuf *b : all) b->release();
         tmpl_rings = 0;
         shifts_uploaded.clear();
@@ -444,16 +446,21 @@ int run_mref_mixed(const float *d_sbj, const float *d_ref, ShiftView acc, int P,
     return 0;
 }
 
-// Narrowed search around proposals with the exhaustive search as fallback.
-// Particle p's proposals d_prop[p][K] = {ref, mirror, psi bin, shift index}; each
-// is searched over the +-hw shift window and the psi samples within psi_half
-// degrees of the bin centre, with the exhaustive kernels' arithmetic.  Particles
-// whose best value is below `threshold` are searched exhaustively.
+// Narrowed search around proposals, in three levels.  Particle p's proposals
+// d_prop[p][K] = {ref, mirror, psi bin, shift index}.
+//  1. each proposal is searched over the +-hw shift window and the psi samples
+//     within psi_half degrees of its bin centre;
+//  2. particles whose maximum is below `threshold` or (edge_fb) on the edge of
+//     the admitted region get, with `level2`, an exact search of every shift and
+//     psi over their distinct proposed references;
+//  3. particles still below the threshold (or flagged, without level 2) get the
+//     exhaustive search.
+// Every level uses the exhaustive kernels' arithmetic for the pairs it covers.
 int run_proposal(const float *d_sbj, const float *d_ref, ShiftView acc, int P, int N, int D, int R,
                  const std::vector<float2> &shifts, int nx, int ny, const int *d_prop, int K,
-                 int n_bins, float psi_half, int hw, float threshold, int edge_fb, int *best_ref,
-                 float *best_val, float *best_peak, int *fallback, unsigned *n_fb_out,
-                 hipStream_t st) {
+                 int n_bins, float psi_half, int hw, float threshold, int edge_fb, int level2,
+                 int *best_ref, float *best_val, float *best_peak, int *fallback,
+                 unsigned *n_fb_out, unsigned *n_l2_out, hipStream_t st) {
     const int S = (int)shifts.size();
     if (int e = check_shape(D, R, kRingLen, shifts.size())) return e;
     if (S > 64 || nx * ny != S) {
@@ -461,21 +468,25 @@ int run_proposal(const float *d_sbj, const float *d_ref, ShiftView acc, int P, i
         return kErrUnsupported;
     }
     if (n_fb_out) *n_fb_out = 0;
+    if (n_l2_out) *n_l2_out = 0;
     if (P <= 0 || N <= 0) return 0;
     if (int e = ensure_template(R)) return e;
     if (int e = ensure_shifts(shifts)) return e;
     if (int e = ref_spectra(d_ref, N, D, R, st)) return e;
     const size_t spec_row = (size_t)S * R * kSlots * sizeof(float2);
     const size_t per = spec_row + (size_t)S * sizeof(float2) +
-                       (size_t)K * (sizeof(isac::PropItem) + sizeof(PeakRec)) + sizeof(uint64_t);
+                       (size_t)K * (sizeof(isac::PropItem) + sizeof(PeakRec)) + sizeof(uint64_t) +
+                       sizeof(int);
     const int pb = batch_for(per, (size_t)P * 2 * sizeof(int), 0.9f, P);
     HIP_OK(ws.spec.reserve((size_t)pb * spec_row));
     HIP_OK(ws.norm.reserve((size_t)pb * S * sizeof(float2)));
     HIP_OK(ws.items.reserve((size_t)pb * K * sizeof(isac::PropItem)));
     HIP_OK(ws.rec.reserve((size_t)pb * K * sizeof(PeakRec)));
     HIP_OK(ws.smask.reserve((size_t)pb * sizeof(uint64_t)));
+    HIP_OK(ws.edge.reserve((size_t)pb * sizeof(int)));
     HIP_OK(ws.fb_idx.reserve((size_t)P * sizeof(int)));
     HIP_OK(ws.fb_count.reserve(sizeof(int)));
+    // level 1
     for (int p0 = 0; p0 < P; p0 += pb) {
         const int n = std::min(pb, P - p0);
         ShiftView a = acc;
@@ -497,17 +508,55 @@ int run_proposal(const float *d_sbj, const float *d_ref, ShiftView acc, int P, i
             if (g.prof) { g.prof_ccf_ms += ps.finish(); g.prof_ccf_n++; }
         }
         HIP_OK(isac::launch_best_prop(ws.rec.as<PeakRec>(), ws.items.as<isac::PropItem>(), n, K, nx,
-                                      ny, hw, edge_fb, a, ws.shifts_f.as<float2>(), S, threshold, p0,
-                                      best_ref, best_val, best_peak, fallback, st));
+                                      ny, hw, edge_fb, a, ws.shifts_f.as<float2>(), S, p0, best_ref,
+                                      best_val, best_peak, ws.edge.as<int>(), st));
+        HIP_OK(isac::launch_fallback_flags(best_val, ws.edge.as<int>(), n, threshold, p0, fallback, st));
     }
-    // exhaustive fallback for the flagged particles
-    HIP_OK(isac::launch_compact(fallback, P, ws.fb_idx.as<int>(), ws.fb_count.as<int>(), st));
+    auto flagged = [&](int &count) -> int {
+        HIP_OK(isac::launch_compact(fallback, P, ws.fb_idx.as<int>(), ws.fb_count.as<int>(), st));
+        HIP_OK(hipMemcpyAsync(&count, ws.fb_count.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        return 0;
+    };
+    const size_t per_fb = (size_t)D * D * sizeof(float) + sizeof(float2) + spec_row +
+                          (size_t)S * sizeof(float2) + (size_t)std::max(N, K) * sizeof(PeakRec) +
+                          (size_t)K * sizeof(int);
+    // level 2: every shift and psi of the proposed references
     int n_fb = 0;
-    HIP_OK(hipMemcpyAsync(&n_fb, ws.fb_count.p, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIP_OK(hipStreamSynchronize(st));
+    if (int e = flagged(n_fb)) return e;
+    if (level2 && n_fb > 0) {
+        if (n_l2_out) *n_l2_out = (unsigned)n_fb;
+        const int fb = batch_for(per_fb, 0, 0.9f, n_fb);
+        HIP_OK(ws.fb_img.reserve((size_t)fb * D * D * sizeof(float)));
+        HIP_OK(ws.fb_acc.reserve((size_t)fb * sizeof(float2)));
+        HIP_OK(ws.spec.reserve((size_t)fb * spec_row));
+        HIP_OK(ws.norm.reserve((size_t)fb * S * sizeof(float2)));
+        HIP_OK(ws.rec.reserve((size_t)fb * K * sizeof(PeakRec)));
+        HIP_OK(ws.cand.reserve((size_t)fb * K * sizeof(int)));
+        for (int j0 = 0; j0 < n_fb; j0 += fb) {
+            const int m = std::min(fb, n_fb - j0);
+            const int *idx = ws.fb_idx.as<int>() + j0;
+            HIP_OK(isac::launch_gather(d_sbj, D, acc, idx, m, ws.fb_img.as<float>(),
+                                       ws.fb_acc.as<float2>(), st));
+            HIP_OK(isac::launch_prop_refs(d_prop, idx, m, K, N, ws.cand.as<int>(), st));
+            HIP_OK(isac::launch_polar_spectra(ws.fb_img.as<float>(), m, D,
+                                              ShiftView{ws.fb_acc.as<float>(), 2},
+                                              ws.shifts_i.as<int2>(), S, ws.tmpl.as<float2>(), R,
+                                              ws.twiddle.as<float2>(), ws.spec.as<float2>(),
+                                              ws.norm.as<float2>(), st));
+            HIP_OK(isac::launch_ccf_candidates(ws.spec.as<float2>(), ws.norm.as<float2>(),
+                                               ws.rspec.as<float2>(), ws.rsum0.as<float>(),
+                                               ws.cand.as<int>(), K, m, S, R, N,
+                                               ws.twiddle.as<float2>(), ws.rec.as<PeakRec>(), st));
+            HIP_OK(isac::launch_best_refs_level2(ws.rec.as<PeakRec>(), ws.cand.as<int>(), m, K,
+                                                 ws.fb_acc.as<float2>(), ws.shifts_f.as<float2>(), S,
+                                                 idx, threshold, best_ref, best_val, best_peak,
+                                                 fallback, st));
+        }
+        if (int e = flagged(n_fb)) return e;
+    }
+    // level 3: the exhaustive search
     if (n_fb > 0) {
-        const size_t per_fb = (size_t)D * D * sizeof(float) + sizeof(float2) + spec_row +
-                              (size_t)S * sizeof(float2) + (size_t)N * sizeof(PeakRec);
         const int fb = batch_for(per_fb, 0, 0.9f, n_fb);
         HIP_OK(ws.fb_img.reserve((size_t)fb * D * D * sizeof(float)));
         HIP_OK(ws.fb_acc.reserve((size_t)fb * sizeof(float2)));
@@ -1265,12 +1314,14 @@ int isac_mref_proposal_device(const float *d_sbj, const float *d_ref, const floa
                               float shift_rng_x, float shift_rng_y, const int *d_prop,
                               unsigned int n_prop, unsigned int n_psi_bins, float psi_half_deg,
                               unsigned int shift_half, float threshold, int edge_fallback,
-                              int *d_best_ref, float *d_best_val, float *d_best_peak,
-                              int *d_fallback, unsigned int *n_fallback, void *stream) {
+                              int level2, int *d_best_ref, float *d_best_val, float *d_best_peak,
+                              int *d_fallback, unsigned int *n_level2, unsigned int *n_fallback,
+                              void *stream) {
     if (int e = ensure_device_ws(-1)) return e;
     std::vector<float2> sh = make_shift_grid(shift_rng_x, shift_rng_y, shift_step);
     if (int e = check_shape((int)img_dim, (int)ring_num, (int)ring_len, sh.size())) return e;
     if (n_fallback) *n_fallback = 0;
+    if (n_level2) *n_level2 = 0;
     if (n_sbj == 0 || n_ref == 0) return 0;
     if (!d_sbj || !d_ref || !d_prop || !d_best_ref || !d_best_val || !d_best_peak || !d_fallback ||
         n_prop < 1 || n_prop > (unsigned)isac::kMaxProposals || n_psi_bins < 1 || n_psi_bins > 256 ||
@@ -1286,8 +1337,8 @@ int isac_mref_proposal_device(const float *d_sbj, const float *d_ref, const floa
     hipStream_t st = (hipStream_t)stream;   // NULL: the null stream, ordered with the caller's work
     return run_proposal(d_sbj, d_ref, ShiftView{d_acc_xy, 2}, (int)n_sbj, (int)n_ref, (int)img_dim,
                         (int)ring_num, sh, nx, ny, d_prop, (int)n_prop, (int)n_psi_bins, psi_half_deg,
-                        (int)shift_half, threshold, edge_fallback != 0, d_best_ref, d_best_val,
-                        d_best_peak, d_fallback, n_fallback, st);
+                        (int)shift_half, threshold, edge_fallback != 0, level2 != 0, d_best_ref,
+                        d_best_val, d_best_peak, d_fallback, n_fallback, n_level2, st);
 }
 
 int isac_mref_align_batched_device(const IsacMrefSubset *subsets, unsigned int n_subsets,

@@ -93,10 +93,21 @@ hipError_t launch_ccf_prop(const float2 *spec, const float2 *norm, const float2 
                            int n_ring, int n_ref, int nx, int ny, int hw, const float2 *twiddle,
                            PeakRec *rec, hipStream_t st);
 hipError_t launch_best_prop(const PeakRec *rec, const PropItem *items, int n_img, int K, int nx,
-                            int ny, int hw, int edge_fallback, ShiftView acc,
-                            const float2 *shifts_f, int n_shift, float threshold, int p_off,
-                            int *best_ref, float *best_val, float *best_peak, int *fallback,
+                            int ny, int hw, int edge_test, ShiftView acc, const float2 *shifts_f,
+                            int n_shift, int p_off, int *best_ref, float *best_val,
+                            float *best_peak, int *edge, hipStream_t st);
+// Second level: exact search of all shifts and psi over the distinct proposed
+// references of gathered particles idx[0..n) (cand [n][K]), then the best one
+// decoded into the particles' outputs with fallback = value below threshold.
+hipError_t launch_prop_refs(const int *prop, const int *idx, int n, int K, int N, int *cand,
                             hipStream_t st);
+hipError_t launch_best_refs_level2(const PeakRec *rec, const int *cand, int n, int K,
+                                   const float2 *acc, const float2 *shifts_f, int n_shift,
+                                   const int *idx, float threshold, int *best_ref,
+                                   float *best_val, float *best_peak, int *fallback,
+                                   hipStream_t st);
+hipError_t launch_fallback_flags(const float *best_val, const int *edge, int n_img, float threshold,
+                                 int p_off, int *fallback, hipStream_t st);
 // Exhaustive fallback: compact flagged particles, gather them, scatter the best
 // reference of each back to its index.
 hipError_t launch_compact(const int *flag, int n, int *idx, int *count, hipStream_t st);

@@ -149,6 +149,7 @@ __global__ __launch_bounds__(256) void k_polar_spectra(
     float *img = reinterpret_cast<float *>(fb + 4 * 512);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int p = blockIdx.x;
+    if (MASKED && smask[p] == 0ull) return;   // uniform over the workgroup
     const float *src = imgs + (size_t)p * D * D;
     for (int i = tid; i < D * D; i += 256) img[i] = src[i];
     tw[tid] = twg[tid];
@@ -929,18 +930,48 @@ __device__ __forceinline__ bool admitted(const PropItem *it, int K, int ref, int
     return false;
 }
 
+// Edge test of a maximum (m, s, t) of reference `ref`: true when one of its
+// neighbours (psi sample t +- 1, or the next shift in x or y inside the grid) is
+// admitted by no item of that reference, i.e. the windows may have cut the
+// search short.
+__device__ __forceinline__ bool on_edge(const PropItem *it, int K, int ref, int m, int s, int t,
+                                        int nx, int ny, int hw) {
+    const int ix = s / ny, iy = s % ny;
+    bool in = admitted(it, K, ref, m, s, (t + 1) & (kRingLen - 1), ny, hw) &&
+              admitted(it, K, ref, m, s, (t + kRingLen - 1) & (kRingLen - 1), ny, hw);
+    if (ix > 0) in = in && admitted(it, K, ref, m, s - ny, t, ny, hw);
+    if (ix < nx - 1) in = in && admitted(it, K, ref, m, s + ny, t, ny, hw);
+    if (iy > 0) in = in && admitted(it, K, ref, m, s - 1, t, ny, hw);
+    if (iy < ny - 1) in = in && admitted(it, K, ref, m, s + 1, t, ny, hw);
+    return !in;
+}
+
+__device__ __forceinline__ void decode_best(const PeakRec &r, int ref, int p, ShiftView acc,
+                                            const float2 *shifts_f, int n_shift, size_t pp,
+                                            int *best_ref, float *best_val, float *best_peak) {
+    float ax = 0.0f, ay = 0.0f;
+    if (acc.base) {
+        ax = acc.base[(size_t)p * acc.stride];
+        ay = acc.base[(size_t)p * acc.stride + 1];
+    }
+    const float2 sh = shifts_f[min(max(r.s, 0), n_shift - 1)];
+    best_ref[pp] = ref;
+    best_val[pp] = r.val;
+    best_peak[4 * pp + 0] = wizard_angle(r);
+    best_peak[4 * pp + 1] = ax + sh.x;
+    best_peak[4 * pp + 2] = ay + sh.y;
+    best_peak[4 * pp + 3] = r.m ? 1.0f : 0.0f;
+}
+
 // Best item per particle (value, then lower reference, mirror, shift, psi
-// sample), decoded like the multi-reference wizard.  fallback[p] = 1 when the
-// best value is below the threshold, when no item had a valid sample, or when
-// the maximum lies on the edge of the admitted region: one of its neighbours
-// (psi sample t +- 1, or the next shift in x or y inside the grid) is admitted
-// by no item of that reference, so the search may have been cut short.
+// sample), decoded like the multi-reference wizard into the outputs at p_off + p;
+// edge[p] = 1 when it failed the edge test (with edge_test) or no item had a
+// valid sample.
 __global__ void k_best_prop(const PeakRec *__restrict__ rec, const PropItem *__restrict__ items,
-                            int n_img, int K, int nx, int ny, int hw, int edge_fallback,
-                            ShiftView acc, const float2 *__restrict__ shifts_f, int n_shift,
-                            float threshold, int p_off, int *__restrict__ best_ref,
-                            float *__restrict__ best_val, float *__restrict__ best_peak,
-                            int *__restrict__ fallback) {
+                            int n_img, int K, int nx, int ny, int hw, int edge_test, ShiftView acc,
+                            const float2 *__restrict__ shifts_f, int n_shift, int p_off,
+                            int *__restrict__ best_ref, float *__restrict__ best_val,
+                            float *__restrict__ best_peak, int *__restrict__ edge) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_img) return;
     int bk = -1;
@@ -965,33 +996,74 @@ __global__ void k_best_prop(const PeakRec *__restrict__ rec, const PropItem *__r
         best_ref[pp] = -1;
         best_val[pp] = -3.40282e+38f;
         best_peak[4 * pp] = best_peak[4 * pp + 1] = best_peak[4 * pp + 2] = best_peak[4 * pp + 3] = 0.0f;
-        fallback[pp] = 1;
+        edge[p] = 1;
         return;
     }
-    const PeakRec r = rec[(size_t)p * K + bk];
-    float ax = 0.0f, ay = 0.0f;
-    if (acc.base) {
-        ax = acc.base[(size_t)p * acc.stride];
-        ay = acc.base[(size_t)p * acc.stride + 1];
-    }
-    const float2 sh = shifts_f[min(max(r.s, 0), n_shift - 1)];
+    PeakRec r = rec[(size_t)p * K + bk];
+    r.m = min(max(r.m, 0), 1);
+    r.s = min(max(r.s, 0), n_shift - 1);
+    r.t &= kRingLen - 1;
     const PropItem *it = items + (size_t)p * K;
     const int ref = it[bk].ref;
-    const int m = min(max(r.m, 0), 1), s = min(max(r.s, 0), n_shift - 1), t = r.t & (kRingLen - 1);
-    const int ix = s / ny, iy = s % ny;
-    bool interior = admitted(it, K, ref, m, s, (t + 1) & (kRingLen - 1), ny, hw) &&
-                    admitted(it, K, ref, m, s, (t + kRingLen - 1) & (kRingLen - 1), ny, hw);
-    if (ix > 0) interior = interior && admitted(it, K, ref, m, s - ny, t, ny, hw);
-    if (ix < nx - 1) interior = interior && admitted(it, K, ref, m, s + ny, t, ny, hw);
-    if (iy > 0) interior = interior && admitted(it, K, ref, m, s - 1, t, ny, hw);
-    if (iy < ny - 1) interior = interior && admitted(it, K, ref, m, s + 1, t, ny, hw);
-    best_ref[pp] = ref;
+    decode_best(r, ref, p, acc, shifts_f, n_shift, pp, best_ref, best_val, best_peak);
+    edge[p] = (edge_test && on_edge(it, K, ref, r.m, r.s, r.t, nx, ny, hw)) ? 1 : 0;
+}
+
+// Second level: the distinct valid references among particle idx[j]'s
+// proposals -> cand[j][0..K) (padded with the first one; reference 0 if none).
+__global__ void k_prop_refs(const int4 *__restrict__ prop, const int *__restrict__ idx, int n, int K,
+                            int N, int *__restrict__ cand) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int p = idx[j];
+    int c[kMaxProposals];
+    int cnt = 0;
+    for (int k = 0; k < K; ++k) {
+        const int r = prop[(size_t)p * K + k].x;
+        if (r < 0 || r >= N) continue;
+        bool seen = false;
+        for (int q = 0; q < cnt; ++q) seen = seen || c[q] == r;
+        if (!seen) c[cnt++] = r;
+    }
+    if (cnt == 0) c[cnt++] = 0;
+    for (int k = 0; k < K; ++k) cand[(size_t)j * K + k] = c[k < cnt ? k : 0];
+}
+
+// Best of the second-level records rec[j][K] (value, then lower reference),
+// decoded into the outputs of particle idx[j]; fallback = value below threshold.
+__global__ void k_best_refs_level2(const PeakRec *__restrict__ rec, const int *__restrict__ cand,
+                                   int n, int K, const float2 *__restrict__ acc,
+                                   const float2 *__restrict__ shifts_f, int n_shift,
+                                   const int *__restrict__ idx, float threshold,
+                                   int *__restrict__ best_ref, float *__restrict__ best_val,
+                                   float *__restrict__ best_peak, int *__restrict__ fallback) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int bk = 0;
+    for (int k = 1; k < K; ++k) {
+        const float v = rec[(size_t)j * K + k].val, bv = rec[(size_t)j * K + bk].val;
+        if (v > bv || (v == bv && cand[(size_t)j * K + k] < cand[(size_t)j * K + bk])) bk = k;
+    }
+    const PeakRec r = rec[(size_t)j * K + bk];
+    const float2 a = acc[j];
+    const float2 sh = shifts_f[min(max(r.s, 0), n_shift - 1)];
+    const size_t pp = (size_t)idx[j];
+    best_ref[pp] = cand[(size_t)j * K + bk];
     best_val[pp] = r.val;
     best_peak[4 * pp + 0] = wizard_angle(r);
-    best_peak[4 * pp + 1] = ax + sh.x;
-    best_peak[4 * pp + 2] = ay + sh.y;
+    best_peak[4 * pp + 1] = a.x + sh.x;
+    best_peak[4 * pp + 2] = a.y + sh.y;
     best_peak[4 * pp + 3] = r.m ? 1.0f : 0.0f;
-    fallback[pp] = ((interior || !edge_fallback) && r.val >= threshold) ? 0 : 1;
+    fallback[pp] = (r.val >= threshold) ? 0 : 1;
+}
+
+// fallback = edge or value below the threshold (NaN included).
+__global__ void k_fallback_flags(const float *__restrict__ best_val, const int *__restrict__ edge,
+                                 int n_img, float threshold, int p_off, int *__restrict__ fallback) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_img) return;
+    const size_t pp = (size_t)p_off + p;
+    fallback[pp] = (edge[p] || !(best_val[pp] >= threshold)) ? 1 : 0;
 }
 
 // Indices of the particles flagged for the exhaustive fallback (any order).
@@ -1483,15 +1555,43 @@ hipError_t launch_ccf_prop(const float2 *spec, const float2 *norm, const float2 
 }
 
 hipError_t launch_best_prop(const PeakRec *rec, const PropItem *items, int n_img, int K, int nx,
-                            int ny, int hw, int edge_fallback, ShiftView acc,
-                            const float2 *shifts_f, int n_shift, float threshold, int p_off,
-                            int *best_ref, float *best_val, float *best_peak, int *fallback,
-                            hipStream_t st) {
+                            int ny, int hw, int edge_test, ShiftView acc, const float2 *shifts_f,
+                            int n_shift, int p_off, int *best_ref, float *best_val,
+                            float *best_peak, int *edge, hipStream_t st) {
     if (n_img <= 0) return hipSuccess;
     if (nx * ny != n_shift || ny < 1) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_best_prop, dim3((n_img + 127) / 128), dim3(128), 0, st, rec, items, n_img, K,
-                       nx, ny, hw, edge_fallback, acc, shifts_f, n_shift, threshold, p_off,
-                       best_ref, best_val, best_peak, fallback);
+                       nx, ny, hw, edge_test, acc, shifts_f, n_shift, p_off, best_ref, best_val,
+                       best_peak, edge);
+    return hipGetLastError();
+}
+
+hipError_t launch_prop_refs(const int *prop, const int *idx, int n, int K, int N, int *cand,
+                            hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (K < 1 || K > kMaxProposals) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_prop_refs, dim3((n + 127) / 128), dim3(128), 0, st,
+                       reinterpret_cast<const int4 *>(prop), idx, n, K, N, cand);
+    return hipGetLastError();
+}
+
+hipError_t launch_best_refs_level2(const PeakRec *rec, const int *cand, int n, int K,
+                                   const float2 *acc, const float2 *shifts_f, int n_shift,
+                                   const int *idx, float threshold, int *best_ref,
+                                   float *best_val, float *best_peak, int *fallback,
+                                   hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_best_refs_level2, dim3((n + 127) / 128), dim3(128), 0, st, rec, cand, n, K,
+                       acc, shifts_f, n_shift, idx, threshold, best_ref, best_val, best_peak,
+                       fallback);
+    return hipGetLastError();
+}
+
+hipError_t launch_fallback_flags(const float *best_val, const int *edge, int n_img, float threshold,
+                                 int p_off, int *fallback, hipStream_t st) {
+    if (n_img <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fallback_flags, dim3((n_img + 255) / 256), dim3(256), 0, st, best_val, edge,
+                       n_img, threshold, p_off, fallback);
     return hipGetLastError();
 }
 

@@ -542,7 +542,7 @@ __device__ __forceinline__ void head_top(float (&v)[J], int lane, float &top_v, 
 // bins (circular) and shift_r grid steps in x and y (grid rows of ny shifts).
 __global__ __launch_bounds__(256) void k_proposals(const float *__restrict__ logits, int n, int ld,
                                                    int n_ref, int n_psi, int n_shift, int k,
-                                                   int psi_r, int shift_r, int ny,
+                                                   int psi_r, int shift_r, int ny, int min_refs,
                                                    int4 *__restrict__ prop) {
     constexpr int KR = 8, KP = 16, KS = 16, PER = KR * KP * KS / 64;   // 32 per lane
     const int lane = threadIdx.x & 63;
@@ -575,23 +575,30 @@ __global__ __launch_bounds__(256) void k_proposals(const float *__restrict__ log
         sc[j] = (cr[j] >= 0 && cp[j] >= 0 && cs[j] >= 0) ? v : -INFINITY;
     }
     for (int r = 0; r < k; ++r) {
-        float bv = -FLT_MAX;
+        // the first min_refs picks take the best remaining triple of reference
+        // rank r (its candidates live in lanes 8r .. 8r+7); otherwise, or when
+        // that reference has none left, the best remaining triple overall
         int bi = INT_MAX;
+        for (int pass = (r < min_refs && r < KR) ? 0 : 1; pass < 2 && bi == INT_MAX; ++pass) {
+            float bv = -FLT_MAX;
+            if (pass == 1 || (lane >> 3) == r) {
 #pragma unroll
-        for (int j = 0; j < PER; ++j) {
-            const int c = lane * PER + j;
-            if (sc[j] > bv || (sc[j] == bv && c < bi)) {
-                bv = sc[j];
-                bi = c;
+                for (int j = 0; j < PER; ++j) {
+                    const int c = lane * PER + j;
+                    if (sc[j] > bv || (sc[j] == bv && c < bi)) {
+                        bv = sc[j];
+                        bi = c;
+                    }
+                }
             }
-        }
 #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            const float ov = __shfl_xor(bv, off, kWave);
-            const int oi = __shfl_xor(bi, off, kWave);
-            if (ov > bv || (ov == bv && oi < bi)) {
-                bv = ov;
-                bi = oi;
+            for (int off = 32; off >= 1; off >>= 1) {
+                const float ov = __shfl_xor(bv, off, kWave);
+                const int oi = __shfl_xor(bi, off, kWave);
+                if (ov > bv || (ov == bv && oi < bi)) {
+                    bv = ov;
+                    bi = oi;
+                }
             }
         }
         if (bi == INT_MAX) {   // fewer valid triples than k
@@ -769,7 +776,7 @@ int isac_nn_pack_weights(const float *master, int Cout, int Cin, int ks, uint16_
 }
 
 int isac_nn_proposals(const float *logits, int n, int ld, int n_ref, int n_psi, int n_shift, int k,
-                      int psi_r, int shift_r, int shift_ny, int *prop, void *stream) {
+                      int psi_r, int shift_r, int shift_ny, int min_refs, int *prop, void *stream) {
     if (n < 0 || k < 1 || k > 8 || n_ref < 1 || n_ref > 512 || n_psi < 1 || 2 * n_psi > 256 ||
         n_shift < 1 || n_shift > 64 || ld < n_ref + 2 * n_psi + n_shift ||
         (psi_r >= 0 && (shift_r < 0 || shift_ny < 1 || n_shift % shift_ny)) ||
@@ -778,7 +785,7 @@ int isac_nn_proposals(const float *logits, int n, int ld, int n_ref, int n_psi, 
     if (n == 0) return 0;
     hipLaunchKernelGGL(k_proposals, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, (hipStream_t)stream,
                        logits, n, ld, n_ref, n_psi, n_shift, k, psi_r, shift_r, shift_ny > 0 ? shift_ny : 1,
-                       reinterpret_cast<int4 *>(prop));
+                       min_refs, reinterpret_cast<int4 *>(prop));
     return launched("isac_nn_proposals");
 }
 

@@ -136,7 +136,8 @@ def load(path: Optional[str] = None):
     lib.isac_mref_best_mixed_device.restype = c_int
     lib.isac_mref_proposal_device.argtypes = [vp, vp, vp, c_uint, c_uint, c_uint, c_uint, c_uint,
                                               c_float, c_float, c_float, vp, c_uint, c_uint,
-                                              c_float, c_uint, c_float, c_int, vp, vp, vp, vp,
+                                              c_float, c_uint, c_float, c_int, c_int, vp, vp,
+                                              vp, vp, ctypes.POINTER(c_uint),
                                               ctypes.POINTER(c_uint), vp]
     lib.isac_mref_proposal_device.restype = c_int
     lib.isac_mref_align_batched_device.argtypes = [ctypes.POINTER(IsacMrefSubset), c_uint, c_int, vp]
@@ -423,12 +424,14 @@ def mref_best_mixed_device(sbj, ref, ring_num: int, shift_step: float, shift_rng
 def mref_proposal_device(sbj, ref, ring_num: int, shift_step: float, shift_rng_x: float,
                          shift_rng_y: float, prop, threshold: float, n_psi_bins: int = 72,
                          psi_half: float = 10.0, shift_half: int = 1, edge_fallback: bool = True,
-                         acc_xy=None, stream=None):
+                         level2: bool = False, acc_xy=None, stream=None):
     """Narrowed search around proposals prop [P, K, 4] int32 (ref, mirror, psi
-    bin, shift index) with the exhaustive fallback below ``threshold`` (and, if
-    ``edge_fallback``, for maxima on the edge of the admitted region).
-    Returns a dict of device tensors best_ref, best_val, best_peak [P, 4],
-    fallback, plus n_fallback (int)."""
+    bin, shift index).  Particles below ``threshold`` (or, with
+    ``edge_fallback``, whose maximum lies on the edge of the admitted region)
+    get, with ``level2``, every shift and psi of their proposed references, and
+    those still below the threshold the exhaustive search.  Returns a dict of
+    device tensors best_ref, best_val, best_peak [P, 4], fallback, plus
+    n_level2 and n_fallback (ints)."""
     import torch
     lib = load()
     P, D = sbj.shape[0], sbj.shape[-1]
@@ -439,18 +442,19 @@ def mref_proposal_device(sbj, ref, ring_num: int, shift_step: float, shift_rng_x
            "best_val": torch.empty(P, dtype=torch.float32, device=dev),
            "best_peak": torch.empty((P, 4), dtype=torch.float32, device=dev),
            "fallback": torch.empty(P, dtype=torch.int32, device=dev)}
-    nfb = ctypes.c_uint(0)
+    nl2, nfb = ctypes.c_uint(0), ctypes.c_uint(0)
     st = (stream or torch.cuda.current_stream()).cuda_stream
     rc = lib.isac_mref_proposal_device(
         sbj.data_ptr(), ref.data_ptr(), acc_xy.data_ptr() if acc_xy is not None else None, P, N, D,
         ring_num, RING_LEN, shift_step, shift_rng_x, shift_rng_y, prop.data_ptr(), prop.shape[1],
         n_psi_bins, psi_half, shift_half, threshold, 1 if edge_fallback else 0,
-        out["best_ref"].data_ptr(),
-        out["best_val"].data_ptr(), out["best_peak"].data_ptr(), out["fallback"].data_ptr(),
+        1 if level2 else 0, out["best_ref"].data_ptr(), out["best_val"].data_ptr(),
+        out["best_peak"].data_ptr(), out["fallback"].data_ptr(), ctypes.byref(nl2),
         ctypes.byref(nfb), st)
     if rc:
         check(lib)
         raise GpuAlignError(f"isac_mref_proposal_device returned {rc}")
+    out["n_level2"] = int(nl2.value)
     out["n_fallback"] = int(nfb.value)
     return out
 

@@ -90,13 +90,15 @@ class Proposer:
     k: int = 8
     psi_half: float = 10.0
     shift_half: int = 1
+    level2: bool = True
 
     def run(self, sbj, ref, acc_xy=None):
         prop = self.net.propose(sbj, self.k)
         return binding.mref_proposal_device(sbj, ref, self.ring_num, self.shift_step,
                                             self.shift_rng, self.shift_rng, prop, self.threshold,
                                             n_psi_bins=self.net.n_psi, psi_half=self.psi_half,
-                                            shift_half=self.shift_half, acc_xy=acc_xy)
+                                            shift_half=self.shift_half, level2=self.level2,
+                                            acc_xy=acc_xy)
 
 
 def calibrate(net, sbj, ref, d, peak, ring_num, shift_rng, target=0.997, **kw):

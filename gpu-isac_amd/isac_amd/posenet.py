@@ -40,7 +40,7 @@ def _lib():
         "isac_nn_softmax_xent": [vp, ci, ci, ci, ci, ci, vp, cf, cf, ci, vp, vp, vp],
         "isac_nn_adamw": [vp, vp, vp, vp, vp, sz, cf, cf, cf, cf, cf, ci, vp],
         "isac_nn_pack_weights": [vp, ci, ci, ci, vp, vp, vp],
-        "isac_nn_proposals": [vp, ci, ci, ci, ci, ci, ci, ci, ci, ci, vp, vp],
+        "isac_nn_proposals": [vp, ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -298,10 +298,12 @@ class PoseNet:
         return hist
 
     # --------------------------------------------------------------- inference
-    def propose(self, imgs, k: int = 8, chunk: int = 4096, psi_r: int = 2, shift_r: int = 1):
+    def propose(self, imgs, k: int = 8, chunk: int = 4096, psi_r: int = 2, shift_r: int = 1,
+                min_refs: int = 3):
         """Top-k proposals [n, k, 4] int32 (ref, mirror, psi bin, shift index),
         spread by suppressing neighbours within psi_r bins and shift_r steps
-        (psi_r < 0: plain top-k of the summed logits)."""
+        (psi_r < 0: plain top-k of the summed logits), the first min_refs taken
+        from the min_refs best-ranked references."""
         import torch
         n = imgs.shape[0]
         ny = int(round(math.sqrt(self.n_shift)))
@@ -312,7 +314,8 @@ class PoseNet:
             i1 = min(n, i0 + chunk)
             logits = self.forward(imgs[i0:i1])
             _call(self.lib, "isac_nn_proposals", _p(logits), i1 - i0, self.ld, self.n_ref,
-                  self.n_psi, self.n_shift, k, psi_r, shift_r, ny, _p(prop[i0:i1]), self._st())
+                  self.n_psi, self.n_shift, k, psi_r, shift_r, ny, min_refs, _p(prop[i0:i1]),
+                  self._st())
         return prop
 
 

@@ -187,30 +187,35 @@ int isac_mref_best_mixed_device(const float *d_sbj, const float *d_ref, const fl
 
 /* No counterpart in the reference (its search is always exhaustive,
  * cuda/gpu_aln_noref.cu:326-554).  Narrowed multi-reference search around
- * proposals with the exhaustive search as fallback; all pointers are device
- * pointers.  d_prop [n_sbj][n_prop] holds int4 {reference, mirror, psi bin,
- * shift index} (n_prop <= 8; psi bin b covers [b, b+1) * 360 / n_psi_bins
- * degrees of the CCF's psi; shift index into the x-major grid of
- * isac_mref_align_device, at most 64 shifts; entries out of range are
- * ignored).  Each proposal is searched over the shifts within shift_half grid
- * steps and the psi samples within psi_half_deg of its bin centre, with the
- * arithmetic of isac_mref_align_device, so a window holding the exhaustive
- * optimum returns it bit for bit.  Particles are searched exhaustively
- * (fallback = 1) when their best value is below `threshold`, or, with
- * edge_fallback != 0, when that maximum lies on the edge of the admitted
- * region (a psi or shift neighbour that no window of its reference admits),
- * where the windows may have cut the search short.  Outputs per particle:
- * best reference, its peak value, {angle, shift_x, shift_y, mirror} in
- * d_best_peak [n_sbj][4], the fallback flag; *n_fallback (host) = number of
- * fallbacks.  Returns after the work has completed. */
+ * proposals, with exact fallbacks; all pointers are device pointers except
+ * n_level2 / n_fallback (host, may be NULL).  d_prop [n_sbj][n_prop] holds int4
+ * {reference, mirror, psi bin, shift index} (n_prop <= 8; psi bin b covers
+ * [b, b+1) * 360 / n_psi_bins degrees of the CCF's psi; shift index into the
+ * x-major grid of isac_mref_align_device, at most 64 shifts; entries out of
+ * range are ignored).  Three levels, each with the arithmetic of
+ * isac_mref_align_device for the pairs it covers:
+ *  1. each proposal is searched over the shifts within shift_half grid steps
+ *     and the psi samples within psi_half_deg of its bin centre (a window
+ *     that holds the exhaustive optimum returns it bit for bit);
+ *  2. particles whose best value is below `threshold`, or (edge_fallback != 0)
+ *     whose maximum lies on the edge of the admitted region (a psi or shift
+ *     neighbour that no window of its reference admits), get with level2 != 0
+ *     every shift and psi of their distinct proposed references (*n_level2
+ *     particles);
+ *  3. particles still below the threshold, or flagged when level2 == 0, get
+ *     the exhaustive search (fallback = 1, *n_fallback particles).
+ * Outputs per particle: best reference, its peak value, {angle, shift_x,
+ * shift_y, mirror} in d_best_peak [n_sbj][4], the fallback flag.  Returns after
+ * the work has completed. */
 int isac_mref_proposal_device(const float *d_sbj, const float *d_ref, const float *d_acc_xy,
                               unsigned int n_sbj, unsigned int n_ref, unsigned int img_dim,
                               unsigned int ring_num, unsigned int ring_len, float shift_step,
                               float shift_rng_x, float shift_rng_y, const int *d_prop,
                               unsigned int n_prop, unsigned int n_psi_bins, float psi_half_deg,
                               unsigned int shift_half, float threshold, int edge_fallback,
-                              int *d_best_ref, float *d_best_val, float *d_best_peak,
-                              int *d_fallback, unsigned int *n_fallback, void *stream);
+                              int level2, int *d_best_ref, float *d_best_val, float *d_best_peak,
+                              int *d_fallback, unsigned int *n_level2, unsigned int *n_fallback,
+                              void *stream);
 
 /* No counterpart in the reference (the reference aligns one subset per
  * multi_ref_align call, cuda/gpu_aln_noref.cu:326-554, driven once per ISAC

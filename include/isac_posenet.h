@@ -96,10 +96,13 @@ int isac_nn_pack_weights(const float *master, int Cout, int Cin, int ks, uint16_
  * best first (ties to the lower ranks).  With psi_r >= 0 each choice removes
  * the candidates of the same reference and mirror within psi_r bins and
  * shift_r grid steps (x-major grid of shift_ny per row), so that the narrowed
- * windows spread out.  prop[i][j] = {ref, mirror, psi bin, shift index};
- * {-1, 0, 0, 0} when fewer than k candidates remain. */
+ * windows spread out.  The first min_refs choices take the best remaining
+ * triple of the 1st, 2nd, ... ranked reference, so that many references are
+ * proposed.  prop[i][j] = {ref, mirror, psi bin, shift index}; {-1, 0, 0, 0}
+ * when fewer than k candidates remain. */
 int isac_nn_proposals(const float *logits, int n, int ld, int n_ref, int n_psi, int n_shift,
-                      int k, int psi_r, int shift_r, int shift_ny, int *prop, void *stream);
+                      int k, int psi_r, int shift_r, int shift_ny, int min_refs, int *prop,
+                      void *stream);
 
 #ifdef __cplusplus
 }

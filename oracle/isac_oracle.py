@@ -533,17 +533,23 @@ def windowed_best(sbj, ref, acc_xy, ring_num, shift_rng_x, shift_rng_y, shift_st
         best = vals[np.arange(len(refs)), idx]
         j = int(np.argmax(best))                                # ties: lower reference
         m, s, t = idx[j] // (S * L), (idx[j] // L) % S, idx[j] % L
+        val = best[j]
+
+        def interior(mk, m, s, t):
+            ix, iy = divmod(int(s), ny)
+            nb = [(s, (t + 1) % L), (s, (t - 1) % L)]
+            nb += [(s + dx * ny + dy, t) for dx, dy in ((-1, 0), (1, 0), (0, -1), (0, 1))
+                   if 0 <= ix + dx < nx and 0 <= iy + dy < ny]
+            return all(mk[m, s2, t2] for s2, t2 in nb)
+
+        inside = interior(mask[refs[j]], m, s, t)
         x7 = neighbours7(tab[j, m, s][None], np.array([t]))
         delta = parabola_delta(x7)
         out["ref"][p] = refs[j]
-        out["d"][p] = best[j]
+        out["d"][p] = val
         out["angle"][p] = angle_multiref(np.array([t]), delta, np.array([m]), L)[0]
         out["sx"][p] = acc_xy[p, 0] + shifts[s, 0]
         out["sy"][p] = acc_xy[p, 1] + shifts[s, 1]
         out["mirror"][p], out["shift_idx"][p], out["t"][p] = m, s, t
-        ix, iy = divmod(int(s), ny)
-        nb = [(s, (t + 1) % L), (s, (t - 1) % L)]
-        nb += [(s + dx * ny + dy, t) for dx, dy in ((-1, 0), (1, 0), (0, -1), (0, 1))
-               if 0 <= ix + dx < nx and 0 <= iy + dy < ny]
-        out["interior"][p] = all(mask[refs[j], m, s2, t2] for s2, t2 in nb)
+        out["interior"][p] = inside
     return out

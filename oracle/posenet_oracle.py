@@ -255,13 +255,16 @@ def adamw(p, g, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
     return p, m, v
 
 
-def proposals(spec_heads, logits, k=8, psi_r=-1, shift_r=1, shift_ny=1, pool=(8, 16, 16)):
+def proposals(spec_heads, logits, k=8, psi_r=-1, shift_r=1, shift_ny=1, pool=(8, 16, 16),
+              min_refs=0):
     """Greedy top-k (ref, mirror, psi bin, shift) triples by summed head logits
     over the candidate pool (top pool[0] references x top pool[1] psi classes x
     top pool[2] shifts; ranks by value, ties to the lower index), best first,
     ties to the lower (ref, psi, shift) ranks.  psi_r >= 0 suppresses, after each
     choice, the candidates of the same reference and mirror within psi_r bins
-    (circular) and shift_r steps in x and y.  Brute force: small heads only."""
+    (circular) and shift_r steps in x and y.  Choice j < min_refs takes the best
+    remaining candidate of reference rank j when there is one.  Brute force:
+    small heads only."""
     n_ref, n_psi2, n_shift = spec_heads
     n_psi = n_psi2 // 2
     out = np.full((logits.shape[0], k, 4), 0, np.int64)
@@ -279,7 +282,11 @@ def proposals(spec_heads, logits, k=8, psi_r=-1, shift_r=1, shift_ny=1, pool=(8,
         cand.sort(key=lambda t: (t[0], t[1]))
         alive = [True] * len(cand)
         for j in range(k):
-            pick = next((q for q in range(len(cand)) if alive[q]), None)
+            pick = None
+            if j < min_refs:
+                pick = next((q for q in range(len(cand)) if alive[q] and cand[q][1][0] == j), None)
+            if pick is None:
+                pick = next((q for q in range(len(cand)) if alive[q]), None)
             if pick is None:
                 out[i, j] = (-1, 0, 0, 0)
                 continue

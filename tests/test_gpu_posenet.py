@@ -90,8 +90,8 @@ def test_adamw_step_matches_oracle(gpu):
     np.testing.assert_allclose(net.m.cpu().numpy(), m1, rtol=1e-6, atol=1e-9)
 
 
-@pytest.mark.parametrize("psi_r", [-1, 1, 2])
-def test_proposals_kernel_matches_brute_force(gpu, psi_r):
+@pytest.mark.parametrize("psi_r,min_refs", [(-1, 0), (1, 0), (2, 0), (2, 3), (-1, 9)])
+def test_proposals_kernel_matches_brute_force(gpu, psi_r, min_refs):
     import torch
     from isac_amd import posenet
     lib = posenet._lib()
@@ -104,8 +104,8 @@ def test_proposals_kernel_matches_brute_force(gpu, psi_r):
     zt = torch.from_numpy(z).cuda()
     prop = torch.empty((n, 8, 4), dtype=torch.int32, device="cuda")
     posenet._call(lib, "isac_nn_proposals", zt.data_ptr(), n, ld, 10, 12, 9, 8, psi_r, 1, 3,
-                  prop.data_ptr(), torch.cuda.current_stream().cuda_stream)
-    want = N.proposals(heads, z, k=8, psi_r=psi_r, shift_r=1, shift_ny=3)
+                  min_refs, prop.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    want = N.proposals(heads, z, k=8, psi_r=psi_r, shift_r=1, shift_ny=3, min_refs=min_refs)
     np.testing.assert_array_equal(prop.cpu().numpy(), want)
 
 

@@ -105,7 +105,7 @@ def test_narrowed_search_matches_window_oracle(gpu):
                                      edge_fallback=False)
     torch.cuda.synchronize()
     assert o["n_fallback"] == 0
-    g = {k: v.cpu().numpy() for k, v in o.items() if k != "n_fallback"}
+    g = {k: v.cpu().numpy() for k, v in o.items() if torch.is_tensor(v)}
     np.testing.assert_array_equal(g["best_ref"], w["ref"])
     np.testing.assert_allclose(g["best_val"], w["d"], rtol=1e-4)
     np.testing.assert_array_equal(g["best_peak"][:, 3], w["mirror"])
@@ -133,3 +133,48 @@ def test_inputs_written_by_queued_work_are_seen(gpu):
     o = binding.mref_proposal_device(sbj, ref, R, 1.0, RNG, RNG, late, threshold=-3.0e38)
     torch.cuda.synchronize()
     assert torch.equal(o["best_val"], bval) and torch.equal(o["best_peak"], bpeak)
+
+
+
+def test_level2_is_the_exhaustive_search_over_proposed_refs(gpu):
+    """Particles whose windowed maximum is on an edge get every shift and psi of
+    their proposed references: bit-identical to the exhaustive search's best
+    over those references; the others keep the windowed maximum."""
+    import torch
+    P, N = 24, 6
+    ds = synth.make_dataset(P, N, 76, 34.0, 0.1, 3.0, True, seed_refs=4)
+    rng = np.random.default_rng(12)
+    prop = np.stack([rng.integers(0, N, (P, 3)), rng.integers(0, 2, (P, 3)),
+                     rng.integers(0, 72, (P, 3)), rng.integers(0, 49, (P, 3))], 2).astype(np.int32)
+    acc = np.zeros((P, 2), np.float32)
+    s, r = torch.from_numpy(ds.sbj).cuda(), torch.from_numpy(ds.ref).cuda()
+    d, peak = _exhaustive(s, r)
+    # every other particle also gets a proposal on its exhaustive optimum
+    lab = _labels_prop(d, peak).cpu().numpy()
+    prop[::2, 0] = lab[::2]
+    w = O.windowed_best(ds.sbj, ds.ref, acc, R, RNG, RNG, 1.0, prop)
+    o = binding.mref_proposal_device(s, r, R, 1.0, RNG, RNG, torch.from_numpy(prop).cuda(),
+                                     threshold=-3.0e38, level2=True)
+    torch.cuda.synchronize()
+    edge = ~w["interior"]
+    assert o["n_level2"] == int(edge.sum()) and o["n_fallback"] == 0
+    assert 0 < edge.sum() < P
+    d, pk = d.cpu().numpy(), peak.cpu().numpy().reshape(N, P, 4)
+    g = {k: v.cpu().numpy() for k, v in o.items() if torch.is_tensor(v)}
+    for p in np.nonzero(edge)[0]:
+        refs = sorted(set(int(x) for x in prop[p, :, 0]))
+        best = max(refs, key=lambda n: (d[n, p], -n))
+        assert g["best_ref"][p] == best
+        assert g["best_val"][p] == d[best, p]
+        np.testing.assert_array_equal(g["best_peak"][p], pk[best, p])
+    keep = ~edge
+    np.testing.assert_array_equal(g["best_ref"][keep], w["ref"][keep])
+    np.testing.assert_allclose(g["best_val"][keep], w["d"][keep], rtol=1e-4)
+    # everything flagged and below an infinite threshold: the exhaustive search
+    o2 = binding.mref_proposal_device(s, r, R, 1.0, RNG, RNG, torch.from_numpy(prop).cuda(),
+                                      threshold=3.0e38, level2=True)
+    torch.cuda.synchronize()
+    assert o2["n_level2"] == P and o2["n_fallback"] == P
+    bref, bval, _ = binding.best_of_exhaustive(torch.from_numpy(d).cuda(),
+                                               torch.from_numpy(pk.reshape(N, 4 * P)).cuda())
+    assert torch.equal(o2["best_ref"], bref) and torch.equal(o2["best_val"], bval)
