@@ -148,8 +148,9 @@ hipError_t launch_ccf_single(const float2 *spec, const float2 *norm, const float
                              int n_shift, int n_ring, int n_ref, const float2 *twiddle,
                              PeakRec *rec, hipStream_t st);
 
-// Exact FP32 re-scoring of K candidate references per particle (ref cand[p*K+k]),
-// same arithmetic as launch_ccf_mref; rec[p*K + k].
+// Exact FP32 re-scoring of K candidate references per particle (ref cand[p*K+k];
+// a negative entry gives an empty record), same arithmetic as launch_ccf_mref;
+// rec[p*K + k].
 hipError_t launch_ccf_candidates(const float2 *spec, const float2 *norm, const float2 *rspec,
                                  const float *rsum0, const int *cand, int K, int n_img,
                                  int n_shift, int n_ring, int n_ref, const float2 *twiddle,

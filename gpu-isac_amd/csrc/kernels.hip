@@ -371,12 +371,18 @@ __global__ __launch_bounds__(256) void k_ccf_items(
     const int item = blockIdx.x * 4 + wave;
     if (item >= n_items) return;
     const int p = item / K;
-    // the host validates ref ids; the clamp keeps every read in bounds
-    const int n = min(max(ref_of[(size_t)item * ref_stride], 0), N - 1);
     float bv[2];
     int bf[2];
     bv[0] = bv[1] = ZERO_INIT ? 0.0f : -3.40282e+38f;
     bf[0] = bf[1] = ZERO_INIT ? 0 : INT_MAX;
+    const int n_raw = ref_of[(size_t)item * ref_stride];
+    if (!ZERO_INIT && n_raw < 0) {   // no reference for this item: an empty record
+        if (lane < 8) nbuf[wave][0][lane] = 0.0f;
+        write_rec(rec + item, bv, bf, nbuf[wave], lane);
+        return;
+    }
+    // the host validates ref ids; the clamp keeps every read in bounds
+    const int n = min(max(n_raw, 0), N - 1);
     const float rs0 = rsum0[n];
     const float2 *rb0 = rspec + (size_t)n * R * kSlots;
     const size_t img_stride = (size_t)S * R * kSlots;
@@ -1010,7 +1016,7 @@ __global__ void k_best_prop(const PeakRec *__restrict__ rec, const PropItem *__r
 }
 
 // Second level: the distinct valid references among particle idx[j]'s
-// proposals -> cand[j][0..K) (padded with the first one; reference 0 if none).
+// proposals -> cand[j][0..K), padded with -1 (reference 0 if none is valid).
 __global__ void k_prop_refs(const int4 *__restrict__ prop, const int *__restrict__ idx, int n, int K,
                             int N, int *__restrict__ cand) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1026,7 +1032,7 @@ __global__ void k_prop_refs(const int4 *__restrict__ prop, const int *__restrict
         if (!seen) c[cnt++] = r;
     }
     if (cnt == 0) c[cnt++] = 0;
-    for (int k = 0; k < K; ++k) cand[(size_t)j * K + k] = c[k < cnt ? k : 0];
+    for (int k = 0; k < K; ++k) cand[(size_t)j * K + k] = k < cnt ? c[k] : -1;
 }
 
 // Best of the second-level records rec[j][K] (value, then lower reference),
@@ -1039,8 +1045,9 @@ __global__ void k_best_refs_level2(const PeakRec *__restrict__ rec, const int *_
                                    float *__restrict__ best_peak, int *__restrict__ fallback) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
-    int bk = 0;
+    int bk = 0;   // entry 0 always holds a reference
     for (int k = 1; k < K; ++k) {
+        if (cand[(size_t)j * K + k] < 0) continue;
         const float v = rec[(size_t)j * K + k].val, bv = rec[(size_t)j * K + bk].val;
         if (v > bv || (v == bv && cand[(size_t)j * K + k] < cand[(size_t)j * K + bk])) bk = k;
     }

@@ -164,31 +164,50 @@ __global__ __launch_bounds__(256) void k_conv_mfma(const unsigned short *__restr
     for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
         for (int pb = 0; pb < 4; ++pb) acc[cb][pb] = float4_t{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int t = 0; t < KS * KS; ++t) {
-        const int dy = t / KS - KS / 2, dx = t % KS - KS / 2;
-        const unsigned short *src[4];
+    // k-steps (tap, 16 input channels), software-pipelined: the operands of the
+    // next step load while the MFMAs of the current one run.  A tap moves every
+    // pixel by the same offset, so the activation address is the pixel's base
+    // plus a uniform term; the bounds test is per lane.
+    const unsigned short *base[4];
+#pragma unroll
+    for (int pb = 0; pb < 4; ++pb)
+        base[pb] = x + (((size_t)pimg[pb] * H + py[pb]) * W + px[pb]) * Cin + lk;
+    const int NK = KS * KS * (Cin / 16);
+    int t_n = 0, c_n = 0;   // next step to load
+    auto load_step = [&](uint2 (&a)[4], uint2 (&b)[4]) {
+        const int dy = t_n / KS - KS / 2, dx = t_n % KS - KS / 2;
+        const long long off = ((long long)dy * W + dx) * Cin + c_n;
 #pragma unroll
         for (int pb = 0; pb < 4; ++pb) {
             const int sy = py[pb] + dy, sx = px[pb] + dx;
             const bool ok = pval[pb] && sy >= 0 && sy < H && sx >= 0 && sx < W;
-            src[pb] = ok ? x + (((size_t)pimg[pb] * H + sy) * W + sx) * Cin + lk : nullptr;
+            b[pb] = ok ? *reinterpret_cast<const uint2 *>(base[pb] + off) : make_uint2(0u, 0u);
         }
-        for (int c0 = 0; c0 < Cin; c0 += 16) {
-            uint2 a[4], b[4];
 #pragma unroll
-            for (int cb = 0; cb < 4; ++cb)
-                a[cb] = cval[cb] ? *reinterpret_cast<const uint2 *>(wrow[cb] + (size_t)t * Cin + c0)
-                                 : make_uint2(0u, 0u);
-#pragma unroll
-            for (int pb = 0; pb < 4; ++pb)
-                b[pb] = src[pb] ? *reinterpret_cast<const uint2 *>(src[pb] + c0) : make_uint2(0u, 0u);
-#pragma unroll
-            for (int cb = 0; cb < 4; ++cb) {
-                if (!cval[cb]) continue;
-#pragma unroll
-                for (int pb = 0; pb < 4; ++pb) acc[cb][pb] = mfma_bf16(a[cb], b[pb], acc[cb][pb]);
-            }
+        for (int cb = 0; cb < 4; ++cb)
+            a[cb] = cval[cb] ? *reinterpret_cast<const uint2 *>(wrow[cb] + (size_t)t_n * Cin + c_n)
+                             : make_uint2(0u, 0u);
+        c_n += 16;
+        if (c_n == Cin) {
+            c_n = 0;
+            ++t_n;
         }
+    };
+    auto mma = [&](const uint2 (&a)[4], const uint2 (&b)[4]) {
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+            if (!cval[cb]) continue;
+#pragma unroll
+            for (int pb = 0; pb < 4; ++pb) acc[cb][pb] = mfma_bf16(a[cb], b[pb], acc[cb][pb]);
+        }
+    };
+    uint2 a0[4], b0[4], a1[4], b1[4];
+    load_step(a0, b0);
+    for (int ks = 0; ks < NK; ks += 2) {
+        if (ks + 1 < NK) load_step(a1, b1);
+        mma(a0, b0);
+        if (ks + 2 < NK) load_step(a0, b0);
+        if (ks + 1 < NK) mma(a1, b1);
     }
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) {
