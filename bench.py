@@ -236,7 +236,7 @@ def learned_report(n_train=200000, n_cal=10000, n_test=100000, epochs=16, oracle
     _, dc, pc = learned.labels(cal, ref, R, rng)
     pr, raw = learned.calibrate(net, cal, ref, dc, pc, R, rng, target=target)
     out["calibration_set"] = {"narrowed_only_agreement": raw, "threshold": pr.threshold,
-                              "target": target}
+                              "confidence_cut": pr.conf_min, "target": target}
     del cal, dc, pc
     te, _ = make(n_test, 31)
     d, peak = learned.exhaustive(te, ref, R, rng)
@@ -260,6 +260,7 @@ def learned_report(n_train=200000, n_cal=10000, n_test=100000, epochs=16, oracle
     oe = binding.mref_proposal_device(te, ref, R, 1.0, rng, rng, prop, threshold=-3.0e38)
     agree = learned.agreement(o["best_ref"], o["best_peak"], d, peak)
     agree_n = learned.agreement(on["best_ref"], on["best_peak"], d, peak)
+    fb_conf = int((o["fallback"] == 2).sum())
     un = learned.Proposer(posenet.PoseNet(N, 49, D=D, seed=1, **learned.NET), pr.threshold, R, rng)
     t_un, ou = timed(lambda: un.run(te, ref), reps=1)
     agree_un = learned.agreement(ou["best_ref"], ou["best_peak"], d, peak)
@@ -267,6 +268,8 @@ def learned_report(n_train=200000, n_cal=10000, n_test=100000, epochs=16, oracle
         "agreement_with_exhaustive_fp32": float(agree.double().mean()),
         "narrowed_only_agreement": float(agree_n.double().mean()),
         "fallback_fraction": o["n_fallback"] / n_test,
+        "fallback_by_confidence_fraction": fb_conf / n_test,
+        "level2_fraction": o["n_level2"] / n_test,
         "edge_fallback_fraction": oe["n_fallback"] / n_test,
         "plain_fp32_s": t_plain, "mfma_fp16_best_s": t_mfma, "learned_s": t_learn,
         "cnn_proposals_s": t_cnn, "narrowed_without_fallback_s": t_narrow,

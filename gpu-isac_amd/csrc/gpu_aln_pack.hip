@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <climits>
 #include <limits>
 #include <map>
 #include <string>
@@ -129,7 +130,7 @@ struct Workspace {
     DevBuf mix_p, mix_pn, mix_r, mix_rn;   // 16-bit MFMA operands and row norms
     DevBuf cand, rec_c, best, second, tie, val64;   // candidate re-scoring
     DevBuf items, smask, fb_idx, fb_count, fb_img, fb_acc;   // narrowed search + fallback
-    DevBuf edge;                                             // edge-test flags
+    DevBuf edge, smask2;                                     // edge flags, level-2 shifts
     std::vector<float2> shifts_uploaded;
     BatchWs batch;
     void release() {
@@ -138,7 +139,7 @@ struct Workspace {
                          &rspec, &rnorm, &rsum0, &rec, &peak, &dvals, &acc,
                          &mix_p, &mix_pn, &mix_r, &mix_rn, &cand, &rec_c, &best, &second,
                          &tie, &val64, &items, &smask, &fb_idx, &fb_count, &fb_img, &fb_acc,
-                         &edge};
+                         &edge, &smask2};
         for (DevBuf *b : all) b->release();
         tmpl_rings = 0;
         shifts_uploaded.clear();
@@ -459,8 +460,9 @@ int run_mref_mixed(const float *d_sbj, const float *d_ref, ShiftView acc, int P,
 int run_proposal(const float *d_sbj, const float *d_ref, ShiftView acc, int P, int N, int D, int R,
                  const std::vector<float2> &shifts, int nx, int ny, const int *d_prop, int K,
                  int n_bins, float psi_half, int hw, float threshold, int edge_fb, int level2,
-                 int *best_ref, float *best_val, float *best_peak, int *fallback,
-                 unsigned *n_fb_out, unsigned *n_l2_out, hipStream_t st) {
+                 const float *conf, float conf_min, int *best_ref, float *best_val,
+                 float *best_peak, int *fallback, unsigned *n_fb_out, unsigned *n_l2_out,
+                 hipStream_t st) {
     const int S = (int)shifts.size();
     if (int e = check_shape(D, R, kRingLen, shifts.size())) return e;
     if (S > 64 || nx * ny != S) {
@@ -484,8 +486,11 @@ int run_proposal(const float *d_sbj, const float *d_ref, ShiftView acc, int P, i
     HIP_OK(ws.rec.reserve((size_t)pb * K * sizeof(PeakRec)));
     HIP_OK(ws.smask.reserve((size_t)pb * sizeof(uint64_t)));
     HIP_OK(ws.edge.reserve((size_t)pb * sizeof(int)));
+    HIP_OK(ws.smask2.reserve((size_t)pb * sizeof(uint64_t)));
+    HIP_OK(ws.cand.reserve((size_t)pb * K * sizeof(int)));
     HIP_OK(ws.fb_idx.reserve((size_t)P * sizeof(int)));
     HIP_OK(ws.fb_count.reserve(sizeof(int)));
+    unsigned n_l2 = 0;
     // level 1
     for (int p0 = 0; p0 < P; p0 += pb) {
         const int n = std::min(pb, P - p0);
@@ -510,10 +515,38 @@ int run_proposal(const float *d_sbj, const float *d_ref, ShiftView acc, int P, i
         HIP_OK(isac::launch_best_prop(ws.rec.as<PeakRec>(), ws.items.as<isac::PropItem>(), n, K, nx,
                                       ny, hw, edge_fb, a, ws.shifts_f.as<float2>(), S, p0, best_ref,
                                       best_val, best_peak, ws.edge.as<int>(), st));
-        HIP_OK(isac::launch_fallback_flags(best_val, ws.edge.as<int>(), n, threshold, p0, fallback, st));
+        HIP_OK(isac::launch_fallback_flags(best_val, ws.edge.as<int>(), conf, conf_min, n, threshold,
+                                           p0, fallback, st));
+        if (!level2) continue;
+        // level 2 inside the batch: the missing spectra rows, then every shift and
+        // psi of the distinct proposed references of the particles flagged 1
+        int n2 = 0;
+        HIP_OK(isac::launch_compact(fallback + p0, n, 1, 1, ws.fb_idx.as<int>(), ws.fb_count.as<int>(),
+                                    st));
+        HIP_OK(hipMemcpyAsync(&n2, ws.fb_count.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        if (n2 == 0) continue;
+        n_l2 += (unsigned)n2;
+        HIP_OK(isac::launch_prop_refs(d_prop + (size_t)p0 * K * 4, ws.fb_idx.as<int>(), n2, K, N,
+                                      ws.cand.as<int>(), st));
+        HIP_OK(isac::launch_level2_mask(fallback + p0, n, S, ws.smask.as<unsigned long long>(),
+                                        ws.smask2.as<unsigned long long>(), st));
+        HIP_OK(isac::launch_polar_spectra_masked(d_sbj + (size_t)p0 * D * D, n, D, a,
+                                                 ws.shifts_i.as<int2>(), S, ws.tmpl.as<float2>(), R,
+                                                 ws.twiddle.as<float2>(), ws.spec.as<float2>(),
+                                                 ws.norm.as<float2>(),
+                                                 ws.smask2.as<unsigned long long>(), st));
+        HIP_OK(isac::launch_ccf_candidates(ws.spec.as<float2>(), ws.norm.as<float2>(),
+                                           ws.rspec.as<float2>(), ws.rsum0.as<float>(),
+                                           ws.cand.as<int>(), K, n2, S, R, N, ws.twiddle.as<float2>(),
+                                           ws.rec.as<PeakRec>(), st, ws.fb_idx.as<int>()));
+        HIP_OK(isac::launch_best_refs_level2(ws.rec.as<PeakRec>(), ws.cand.as<int>(), n2, K, a,
+                                             ws.shifts_f.as<float2>(), S, ws.fb_idx.as<int>(), p0,
+                                             threshold, best_ref, best_val, best_peak, fallback, st));
     }
-    auto flagged = [&](int &count) -> int {
-        HIP_OK(isac::launch_compact(fallback, P, ws.fb_idx.as<int>(), ws.fb_count.as<int>(), st));
+    if (n_l2_out) *n_l2_out = n_l2;
+    auto flagged = [&](int lo, int hi, int &count) -> int {
+        HIP_OK(isac::launch_compact(fallback, P, lo, hi, ws.fb_idx.as<int>(), ws.fb_count.as<int>(), st));
         HIP_OK(hipMemcpyAsync(&count, ws.fb_count.p, sizeof(int), hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
         return 0;
@@ -521,40 +554,8 @@ int run_proposal(const float *d_sbj, const float *d_ref, ShiftView acc, int P, i
     const size_t per_fb = (size_t)D * D * sizeof(float) + sizeof(float2) + spec_row +
                           (size_t)S * sizeof(float2) + (size_t)std::max(N, K) * sizeof(PeakRec) +
                           (size_t)K * sizeof(int);
-    // level 2: every shift and psi of the proposed references
     int n_fb = 0;
-    if (int e = flagged(n_fb)) return e;
-    if (level2 && n_fb > 0) {
-        if (n_l2_out) *n_l2_out = (unsigned)n_fb;
-        const int fb = batch_for(per_fb, 0, 0.9f, n_fb);
-        HIP_OK(ws.fb_img.reserve((size_t)fb * D * D * sizeof(float)));
-        HIP_OK(ws.fb_acc.reserve((size_t)fb * sizeof(float2)));
-        HIP_OK(ws.spec.reserve((size_t)fb * spec_row));
-        HIP_OK(ws.norm.reserve((size_t)fb * S * sizeof(float2)));
-        HIP_OK(ws.rec.reserve((size_t)fb * K * sizeof(PeakRec)));
-        HIP_OK(ws.cand.reserve((size_t)fb * K * sizeof(int)));
-        for (int j0 = 0; j0 < n_fb; j0 += fb) {
-            const int m = std::min(fb, n_fb - j0);
-            const int *idx = ws.fb_idx.as<int>() + j0;
-            HIP_OK(isac::launch_gather(d_sbj, D, acc, idx, m, ws.fb_img.as<float>(),
-                                       ws.fb_acc.as<float2>(), st));
-            HIP_OK(isac::launch_prop_refs(d_prop, idx, m, K, N, ws.cand.as<int>(), st));
-            HIP_OK(isac::launch_polar_spectra(ws.fb_img.as<float>(), m, D,
-                                              ShiftView{ws.fb_acc.as<float>(), 2},
-                                              ws.shifts_i.as<int2>(), S, ws.tmpl.as<float2>(), R,
-                                              ws.twiddle.as<float2>(), ws.spec.as<float2>(),
-                                              ws.norm.as<float2>(), st));
-            HIP_OK(isac::launch_ccf_candidates(ws.spec.as<float2>(), ws.norm.as<float2>(),
-                                               ws.rspec.as<float2>(), ws.rsum0.as<float>(),
-                                               ws.cand.as<int>(), K, m, S, R, N,
-                                               ws.twiddle.as<float2>(), ws.rec.as<PeakRec>(), st));
-            HIP_OK(isac::launch_best_refs_level2(ws.rec.as<PeakRec>(), ws.cand.as<int>(), m, K,
-                                                 ws.fb_acc.as<float2>(), ws.shifts_f.as<float2>(), S,
-                                                 idx, threshold, best_ref, best_val, best_peak,
-                                                 fallback, st));
-        }
-        if (int e = flagged(n_fb)) return e;
-    }
+    if (int e = flagged(1, INT_MAX, n_fb)) return e;
     // level 3: the exhaustive search
     if (n_fb > 0) {
         const int fb = batch_for(per_fb, 0, 0.9f, n_fb);
@@ -1314,9 +1315,9 @@ int isac_mref_proposal_device(const float *d_sbj, const float *d_ref, const floa
                               float shift_rng_x, float shift_rng_y, const int *d_prop,
                               unsigned int n_prop, unsigned int n_psi_bins, float psi_half_deg,
                               unsigned int shift_half, float threshold, int edge_fallback,
-                              int level2, int *d_best_ref, float *d_best_val, float *d_best_peak,
-                              int *d_fallback, unsigned int *n_level2, unsigned int *n_fallback,
-                              void *stream) {
+                              int level2, const float *d_conf, float conf_min, int *d_best_ref,
+                              float *d_best_val, float *d_best_peak, int *d_fallback,
+                              unsigned int *n_level2, unsigned int *n_fallback, void *stream) {
     if (int e = ensure_device_ws(-1)) return e;
     std::vector<float2> sh = make_shift_grid(shift_rng_x, shift_rng_y, shift_step);
     if (int e = check_shape((int)img_dim, (int)ring_num, (int)ring_len, sh.size())) return e;
@@ -1337,8 +1338,9 @@ int isac_mref_proposal_device(const float *d_sbj, const float *d_ref, const floa
     hipStream_t st = (hipStream_t)stream;   // NULL: the null stream, ordered with the caller's work
     return run_proposal(d_sbj, d_ref, ShiftView{d_acc_xy, 2}, (int)n_sbj, (int)n_ref, (int)img_dim,
                         (int)ring_num, sh, nx, ny, d_prop, (int)n_prop, (int)n_psi_bins, psi_half_deg,
-                        (int)shift_half, threshold, edge_fallback != 0, level2 != 0, d_best_ref,
-                        d_best_val, d_best_peak, d_fallback, n_fallback, n_level2, st);
+                        (int)shift_half, threshold, edge_fallback != 0, level2 != 0, d_conf,
+                        conf_min, d_best_ref, d_best_val, d_best_peak, d_fallback, n_fallback,
+                        n_level2, st);
 }
 
 int isac_mref_align_batched_device(const IsacMrefSubset *subsets, unsigned int n_subsets,

@@ -96,21 +96,26 @@ hipError_t launch_best_prop(const PeakRec *rec, const PropItem *items, int n_img
                             int ny, int hw, int edge_test, ShiftView acc, const float2 *shifts_f,
                             int n_shift, int p_off, int *best_ref, float *best_val,
                             float *best_peak, int *edge, hipStream_t st);
-// Second level: exact search of all shifts and psi over the distinct proposed
-// references of gathered particles idx[0..n) (cand [n][K]), then the best one
-// decoded into the particles' outputs with fallback = value below threshold.
+// Second level, inside a particle batch: exact search of all shifts and psi over
+// the distinct proposed references (cand [n][K]) of batch particles idx[0..n);
+// the missing spectra rows come from a masked polar pass (k_level2_mask); the
+// best reference is decoded into the outputs at p_off + idx[j] with fallback =
+// value below threshold.
 hipError_t launch_prop_refs(const int *prop, const int *idx, int n, int K, int N, int *cand,
                             hipStream_t st);
-hipError_t launch_best_refs_level2(const PeakRec *rec, const int *cand, int n, int K,
-                                   const float2 *acc, const float2 *shifts_f, int n_shift,
-                                   const int *idx, float threshold, int *best_ref,
-                                   float *best_val, float *best_peak, int *fallback,
-                                   hipStream_t st);
-hipError_t launch_fallback_flags(const float *best_val, const int *edge, int n_img, float threshold,
-                                 int p_off, int *fallback, hipStream_t st);
+hipError_t launch_level2_mask(const int *flag, int n, int S, const unsigned long long *done,
+                              unsigned long long *smask, hipStream_t st);
+hipError_t launch_best_refs_level2(const PeakRec *rec, const int *cand, int n, int K, ShiftView acc,
+                                   const float2 *shifts_f, int n_shift, const int *idx, int p_off,
+                                   float threshold, int *best_ref, float *best_val,
+                                   float *best_peak, int *fallback, hipStream_t st);
+hipError_t launch_fallback_flags(const float *best_val, const int *edge, const float *conf,
+                                 float conf_min, int n_img, float threshold, int p_off,
+                                 int *fallback, hipStream_t st);
 // Exhaustive fallback: compact flagged particles, gather them, scatter the best
 // reference of each back to its index.
-hipError_t launch_compact(const int *flag, int n, int *idx, int *count, hipStream_t st);
+hipError_t launch_compact(const int *flag, int n, int lo, int hi, int *idx, int *count,
+                          hipStream_t st);
 hipError_t launch_gather(const float *imgs, int D, ShiftView acc, const int *idx, int n, float *out,
                          float2 *out_acc, hipStream_t st);
 hipError_t launch_best_refs_scatter(const PeakRec *rec, int n, int N, const float2 *acc,
@@ -151,10 +156,11 @@ hipError_t launch_ccf_single(const float2 *spec, const float2 *norm, const float
 // Exact FP32 re-scoring of K candidate references per particle (ref cand[p*K+k];
 // a negative entry gives an empty record), same arithmetic as launch_ccf_mref;
 // rec[p*K + k].
+// pidx (or NULL): spectra row of item i's particle is pidx[i / K] instead of i / K.
 hipError_t launch_ccf_candidates(const float2 *spec, const float2 *norm, const float2 *rspec,
                                  const float *rsum0, const int *cand, int K, int n_img,
                                  int n_shift, int n_ring, int n_ref, const float2 *twiddle,
-                                 PeakRec *rec, hipStream_t st);
+                                 PeakRec *rec, hipStream_t st, const int *pidx = nullptr);
 
 // Top-K (K = 8) references per particle by approximate peak value.
 hipError_t launch_topk(const PeakRec *rec, int n_img, int n_ref, int K, int *cand, hipStream_t st);

@@ -360,7 +360,7 @@ __global__ __launch_bounds__(256) void k_ccf_items(
     const float2 *__restrict__ spec, const float2 *__restrict__ norm,
     const float2 *__restrict__ rspec, const float *__restrict__ rsum0,
     const int *__restrict__ ref_of, int ref_stride, int K, int n_items, int S, int R, int N,
-    const float2 *__restrict__ twg, PeakRec *__restrict__ rec) {
+    const float2 *__restrict__ twg, PeakRec *__restrict__ rec, const int *__restrict__ pidx) {
     __shared__ float2 tw[256];
     __shared__ float2 fbuf[4][512];
     __shared__ float zbuf[4][256];
@@ -370,7 +370,7 @@ __global__ __launch_bounds__(256) void k_ccf_items(
     __syncthreads();
     const int item = blockIdx.x * 4 + wave;
     if (item >= n_items) return;
-    const int p = item / K;
+    const int p = pidx ? pidx[item / K] : item / K;   // spectra row of the item's particle
     float bv[2];
     int bf[2];
     bv[0] = bv[1] = ZERO_INIT ? 0.0f : -3.40282e+38f;
@@ -1035,12 +1035,24 @@ __global__ void k_prop_refs(const int4 *__restrict__ prop, const int *__restrict
     for (int k = 0; k < K; ++k) cand[(size_t)j * K + k] = k < cnt ? c[k] : -1;
 }
 
-// Best of the second-level records rec[j][K] (value, then lower reference),
-// decoded into the outputs of particle idx[j]; fallback = value below threshold.
+// Second-level shifts of a batch: for the particles flagged 1 (flag[p]), every
+// shift the first level did not compute (done[p]); none for the others.
+__global__ void k_level2_mask(const int *__restrict__ flag, int n, int S,
+                              const unsigned long long *__restrict__ done,
+                              unsigned long long *__restrict__ smask) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const unsigned long long full = S >= 64 ? ~0ull : ((1ull << S) - 1ull);
+    smask[p] = flag[p] == 1 ? (full & ~done[p]) : 0ull;
+}
+
+// Best of the second-level records rec[j][K] (value, then lower reference) of
+// batch particle idx[j], decoded into its outputs at p_off + idx[j];
+// fallback = value below threshold.
 __global__ void k_best_refs_level2(const PeakRec *__restrict__ rec, const int *__restrict__ cand,
-                                   int n, int K, const float2 *__restrict__ acc,
+                                   int n, int K, ShiftView acc,
                                    const float2 *__restrict__ shifts_f, int n_shift,
-                                   const int *__restrict__ idx, float threshold,
+                                   const int *__restrict__ idx, int p_off, float threshold,
                                    int *__restrict__ best_ref, float *__restrict__ best_val,
                                    float *__restrict__ best_peak, int *__restrict__ fallback) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1052,9 +1064,11 @@ __global__ void k_best_refs_level2(const PeakRec *__restrict__ rec, const int *_
         if (v > bv || (v == bv && cand[(size_t)j * K + k] < cand[(size_t)j * K + bk])) bk = k;
     }
     const PeakRec r = rec[(size_t)j * K + bk];
-    const float2 a = acc[j];
+    const int p = idx[j];
+    float2 a = f2(0.0f, 0.0f);
+    if (acc.base) a = f2(acc.base[(size_t)p * acc.stride], acc.base[(size_t)p * acc.stride + 1]);
     const float2 sh = shifts_f[min(max(r.s, 0), n_shift - 1)];
-    const size_t pp = (size_t)idx[j];
+    const size_t pp = (size_t)p_off + p;
     best_ref[pp] = cand[(size_t)j * K + bk];
     best_val[pp] = r.val;
     best_peak[4 * pp + 0] = wizard_angle(r);
@@ -1064,20 +1078,25 @@ __global__ void k_best_refs_level2(const PeakRec *__restrict__ rec, const int *_
     fallback[pp] = (r.val >= threshold) ? 0 : 1;
 }
 
-// fallback = edge or value below the threshold (NaN included).
+// fallback = 2 when the proposal confidence is below conf_min (straight to the
+// exhaustive search), else 1 for an edge maximum or a value below the threshold
+// (NaN included), else 0.
 __global__ void k_fallback_flags(const float *__restrict__ best_val, const int *__restrict__ edge,
-                                 int n_img, float threshold, int p_off, int *__restrict__ fallback) {
+                                 const float *__restrict__ conf, float conf_min, int n_img,
+                                 float threshold, int p_off, int *__restrict__ fallback) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_img) return;
     const size_t pp = (size_t)p_off + p;
-    fallback[pp] = (edge[p] || !(best_val[pp] >= threshold)) ? 1 : 0;
+    int f = (edge[p] || !(best_val[pp] >= threshold)) ? 1 : 0;
+    if (conf && !(conf[pp] >= conf_min)) f = 2;
+    fallback[pp] = f;
 }
 
-// Indices of the particles flagged for the exhaustive fallback (any order).
-__global__ void k_compact(const int *__restrict__ flag, int n, int *__restrict__ idx,
+// Indices of the particles whose flag lies in [lo, hi] (any order).
+__global__ void k_compact(const int *__restrict__ flag, int n, int lo, int hi, int *__restrict__ idx,
                           int *__restrict__ count) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n || !flag[p]) return;
+    if (p >= n || flag[p] < lo || flag[p] > hi) return;
     idx[atomicAdd(count, 1)] = p;
 }
 
@@ -1435,18 +1454,18 @@ hipError_t launch_ccf_single(const float2 *spec, const float2 *norm, const float
     if (n_img <= 0) return hipSuccess;
     const int blocks = (n_img + 3) / 4;
     hipLaunchKernelGGL(k_ccf_items<true>, dim3(blocks), dim3(256), 0, st, spec, norm, rspec, rsum0,
-                       ref_of, ref_stride, 1, n_img, n_shift, n_ring, n_ref, twiddle, rec);
+                       ref_of, ref_stride, 1, n_img, n_shift, n_ring, n_ref, twiddle, rec, nullptr);
     return hipGetLastError();
 }
 
 hipError_t launch_ccf_candidates(const float2 *spec, const float2 *norm, const float2 *rspec,
                                  const float *rsum0, const int *cand, int K, int n_img,
                                  int n_shift, int n_ring, int n_ref, const float2 *twiddle,
-                                 PeakRec *rec, hipStream_t st) {
+                                 PeakRec *rec, hipStream_t st, const int *pidx) {
     const int items = n_img * K;
     if (items <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_ccf_items<false>, dim3((items + 3) / 4), dim3(256), 0, st, spec, norm,
-                       rspec, rsum0, cand, 1, K, items, n_shift, n_ring, n_ref, twiddle, rec);
+                       rspec, rsum0, cand, 1, K, items, n_shift, n_ring, n_ref, twiddle, rec, pidx);
     return hipGetLastError();
 }
 
@@ -1582,30 +1601,39 @@ hipError_t launch_prop_refs(const int *prop, const int *idx, int n, int K, int N
     return hipGetLastError();
 }
 
-hipError_t launch_best_refs_level2(const PeakRec *rec, const int *cand, int n, int K,
-                                   const float2 *acc, const float2 *shifts_f, int n_shift,
-                                   const int *idx, float threshold, int *best_ref,
-                                   float *best_val, float *best_peak, int *fallback,
-                                   hipStream_t st) {
+hipError_t launch_level2_mask(const int *flag, int n, int S, const unsigned long long *done,
+                              unsigned long long *smask, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (S > 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_level2_mask, dim3((n + 255) / 256), dim3(256), 0, st, flag, n, S, done, smask);
+    return hipGetLastError();
+}
+
+hipError_t launch_best_refs_level2(const PeakRec *rec, const int *cand, int n, int K, ShiftView acc,
+                                   const float2 *shifts_f, int n_shift, const int *idx, int p_off,
+                                   float threshold, int *best_ref, float *best_val,
+                                   float *best_peak, int *fallback, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_best_refs_level2, dim3((n + 127) / 128), dim3(128), 0, st, rec, cand, n, K,
-                       acc, shifts_f, n_shift, idx, threshold, best_ref, best_val, best_peak,
+                       acc, shifts_f, n_shift, idx, p_off, threshold, best_ref, best_val, best_peak,
                        fallback);
     return hipGetLastError();
 }
 
-hipError_t launch_fallback_flags(const float *best_val, const int *edge, int n_img, float threshold,
-                                 int p_off, int *fallback, hipStream_t st) {
+hipError_t launch_fallback_flags(const float *best_val, const int *edge, const float *conf,
+                                 float conf_min, int n_img, float threshold, int p_off,
+                                 int *fallback, hipStream_t st) {
     if (n_img <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fallback_flags, dim3((n_img + 255) / 256), dim3(256), 0, st, best_val, edge,
-                       n_img, threshold, p_off, fallback);
+                       conf, conf_min, n_img, threshold, p_off, fallback);
     return hipGetLastError();
 }
 
-hipError_t launch_compact(const int *flag, int n, int *idx, int *count, hipStream_t st) {
+hipError_t launch_compact(const int *flag, int n, int lo, int hi, int *idx, int *count,
+                          hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, count, 1);
-    hipLaunchKernelGGL(k_compact, dim3((n + 255) / 256), dim3(256), 0, st, flag, n, idx, count);
+    hipLaunchKernelGGL(k_compact, dim3((n + 255) / 256), dim3(256), 0, st, flag, n, lo, hi, idx, count);
     return hipGetLastError();
 }
 

@@ -126,8 +126,70 @@ __global__ __launch_bounds__(256) void k_conv1(const unsigned short *__restrict_
     }
 }
 
+// conv1 with the 2x2 max pool fused (inference): one thread per pooled pixel
+// computes its four conv1 outputs exactly as k_conv1 does (bf16-rounded after
+// bias and ReLU) and keeps the first maximum, so the result equals
+// k_maxpool2(k_conv1(x)).
+__global__ __launch_bounds__(256) void k_conv1_pool(const unsigned short *__restrict__ x0, int n,
+                                                    int H, int W, const float *__restrict__ master,
+                                                    unsigned short *__restrict__ a) {
+    constexpr int CO = 32;
+    __shared__ float w[9][CO];
+    __shared__ float b[CO];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < CO * 9; e += 256) {
+        const int co = e / 9, t = e % 9;
+        w[t][co] = bf2f(f2bf(master[co * 10 + t]));
+    }
+    if (tid < CO) b[tid] = master[tid * 10 + 9];
+    __syncthreads();
+    const int Ho = H / 2, Wo = W / 2;
+    const long long Q = (long long)blockIdx.x * 256 + tid;
+    if (Q >= (long long)n * Ho * Wo) return;
+    const int img = (int)(Q / ((long long)Ho * Wo));
+    const int rem = (int)(Q - (long long)img * Ho * Wo);
+    const int yo = rem / Wo, xo = rem % Wo;
+    float xin[4][4];   // the 4x4 input patch under the window's four 3x3 stencils
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int sy = 2 * yo + r - 1, sx = 2 * xo + c - 1;
+            xin[r][c] = (sy >= 0 && sy < H && sx >= 0 && sx < W)
+                            ? bf2f(x0[((size_t)img * H + sy) * W + sx])
+                            : 0.0f;
+        }
+    uint4 *dst = reinterpret_cast<uint4 *>(a + (size_t)Q * CO);
+#pragma unroll
+    for (int c8 = 0; c8 < CO / 8; ++c8) {
+        float o[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            float m = 0.0f;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int qy = q / 2, qx = q % 2;
+                float acc = 0.0f;
+#pragma unroll
+                for (int t = 0; t < 9; ++t) acc += w[t][c8 * 8 + e] * xin[qy + t / 3][qx + t % 3];
+                const float v = bf2f(f2bf(fmaxf(acc + b[c8 * 8 + e], 0.0f)));
+                if (q == 0 || v > m) m = v;
+            }
+            o[e] = m;
+        }
+        const uint2 lo = pack4(o[0], o[1], o[2], o[3]), hi = pack4(o[4], o[5], o[6], o[7]);
+        dst[c8] = make_uint4(lo.x, lo.y, hi.x, hi.y);
+    }
+}
+
 // ------------------------------------------------------------ MFMA convolution
-template <int KS, bool OUT_F32>
+// POOL (inference; bias, ReLU, bf16 output): the pixel index runs over 2x2
+// pooling windows x their 4 pixels (the last row / column of an odd map, which
+// no window uses, is skipped), so the four pixels of a window sit in lanes l,
+// l^1, l^2, l^3 of the accumulator layout; the epilogue rounds each value to
+// bf16 as the unpooled kernel does, takes the first maximum across those lanes
+// and stores the pooled map [n][H/2][W/2][Cout].
+template <int KS, bool OUT_F32, bool POOL = false>
 __global__ __launch_bounds__(256) void k_conv_mfma(const unsigned short *__restrict__ x, int n,
                                                    int H, int W, int Cin,
                                                    const unsigned short *__restrict__ w,
@@ -135,7 +197,8 @@ __global__ __launch_bounds__(256) void k_conv_mfma(const unsigned short *__restr
                                                    int Cout, int relu, void *__restrict__ yv) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = (lane >> 4) * 4;
-    const long long NP = (long long)n * H * W;
+    const int Ho = H / 2, Wo = W / 2;
+    const long long NP = POOL ? (long long)n * Ho * Wo * 4 : (long long)n * H * W;
     const long long pbase = (long long)blockIdx.x * 256 + wave * 64;
     const int cb0 = blockIdx.y * 64;
     if (pbase >= NP) return;   // uniform over the wavefront; no barriers below
@@ -146,10 +209,19 @@ __global__ __launch_bounds__(256) void k_conv_mfma(const unsigned short *__restr
         const long long P = pbase + pb * 16 + li;
         pval[pb] = P < NP;
         const long long Pc = pval[pb] ? P : 0;
-        pimg[pb] = (int)(Pc / ((long long)H * W));
-        const int rem = (int)(Pc - (long long)pimg[pb] * H * W);
-        py[pb] = rem / W;
-        px[pb] = rem % W;
+        if (POOL) {
+            const long long win = Pc >> 2;
+            const int sub = (int)(Pc & 3);
+            pimg[pb] = (int)(win / ((long long)Ho * Wo));
+            const int rem = (int)(win - (long long)pimg[pb] * Ho * Wo);
+            py[pb] = 2 * (rem / Wo) + (sub >> 1);
+            px[pb] = 2 * (rem % Wo) + (sub & 1);
+        } else {
+            pimg[pb] = (int)(Pc / ((long long)H * W));
+            const int rem = (int)(Pc - (long long)pimg[pb] * H * W);
+            py[pb] = rem / W;
+            px[pb] = rem % W;
+        }
     }
     bool cval[4];
     const unsigned short *wrow[4];
@@ -220,7 +292,6 @@ __global__ __launch_bounds__(256) void k_conv_mfma(const unsigned short *__restr
         }
 #pragma unroll
         for (int pb = 0; pb < 4; ++pb) {
-            if (!pval[pb]) continue;
             const long long P = pbase + pb * 16 + li;
             float o[4];
 #pragma unroll
@@ -228,6 +299,27 @@ __global__ __launch_bounds__(256) void k_conv_mfma(const unsigned short *__restr
                 o[v] = acc[cb][pb][v] + bb[v];
                 if (relu) o[v] = fmaxf(o[v], 0.0f);
             }
+            if (POOL) {
+                // window = lanes l..l|3; first maximum in window order (sub 0..3)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    float m = bf2f(f2bf(o[v]));
+                    const float m1 = __shfl_xor(m, 1, kWave);   // sub ^ 1
+                    const float m2 = __shfl_xor(m, 2, kWave);   // sub ^ 2
+                    const float m3 = __shfl_xor(m, 3, kWave);   // sub ^ 3
+                    // lane with sub 0 holds subs 0,1,2,3 in m, m1, m2, m3
+                    float r = m;
+                    if (m1 > r) r = m1;
+                    if (m2 > r) r = m2;
+                    if (m3 > r) r = m3;
+                    o[v] = r;
+                }
+                if (!pval[pb] || (li & 3)) continue;
+                unsigned short *y = static_cast<unsigned short *>(yv) + (size_t)(P >> 2) * Cout + co;
+                *reinterpret_cast<uint2 *>(y) = pack4(o[0], o[1], o[2], o[3]);
+                continue;
+            }
+            if (!pval[pb]) continue;
             if (OUT_F32) {
                 float *y = static_cast<float *>(yv) + (size_t)P * Cout + co;
                 *reinterpret_cast<float4 *>(y) = make_float4(o[0], o[1], o[2], o[3]);
@@ -562,7 +654,7 @@ __device__ __forceinline__ void head_top(float (&v)[J], int lane, float &top_v, 
 __global__ __launch_bounds__(256) void k_proposals(const float *__restrict__ logits, int n, int ld,
                                                    int n_ref, int n_psi, int n_shift, int k,
                                                    int psi_r, int shift_r, int ny, int min_refs,
-                                                   int4 *__restrict__ prop) {
+                                                   int4 *__restrict__ prop, float *__restrict__ conf) {
     constexpr int KR = 8, KP = 16, KS = 16, PER = KR * KP * KS / 64;   // 32 per lane
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -574,6 +666,18 @@ __global__ __launch_bounds__(256) void k_proposals(const float *__restrict__ log
 #pragma unroll
     for (int j = 0; j < 4; ++j) pv[j] = lane + 64 * j < 2 * n_psi ? z[n_ref + lane + 64 * j] : -FLT_MAX;
     sv[0] = lane < n_shift ? z[n_ref + 2 * n_psi + lane] : -FLT_MAX;
+    // softmax of the reference head (for the coverage estimate)
+    float mx = -FLT_MAX;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) mx = fmaxf(mx, rv[j]);
+    mx = wave_max(mx);
+    float se = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) se += lane + 64 * j < n_ref ? expf(rv[j] - mx) : 0.0f;
+    se = wave_sum(se);
+    float covered = 0.0f;   // lane 0: softmax mass of the distinct proposed references
+    int seen[8];
+    int n_seen = 0;
     float tr, tp, ts;
     int ir, ip, is;
     head_top<8, KR>(rv, lane, tr, ir);
@@ -626,7 +730,15 @@ __global__ __launch_bounds__(256) void k_proposals(const float *__restrict__ log
         }
         const int sa = bi / (KP * KS), sb = (bi / KS) % KP, sd = bi % KS;
         const int ref = __shfl(ir, sa, kWave), pc = __shfl(ip, sb, kWave), sh = __shfl(is, sd, kWave);
-        if (lane == 0) prop[(size_t)i * k + r] = make_int4(ref, pc / n_psi, pc % n_psi, sh);
+        if (lane == 0) {
+            prop[(size_t)i * k + r] = make_int4(ref, pc / n_psi, pc % n_psi, sh);
+            bool dup = false;
+            for (int q = 0; q < n_seen; ++q) dup = dup || seen[q] == ref;
+            if (!dup && n_seen < 8) {
+                seen[n_seen++] = ref;
+                covered += expf(z[ref] - mx);
+            }
+        }
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
             const int c = lane * PER + j;
@@ -640,6 +752,7 @@ __global__ __launch_bounds__(256) void k_proposals(const float *__restrict__ log
             if (sup) sc[j] = -INFINITY;
         }
     }
+    if (conf && lane == 0) conf[i] = se > 0.0f ? covered / se : 0.0f;
 }
 
 int fail(int code, const char *msg) {
@@ -706,6 +819,30 @@ int isac_nn_conv_fwd(const uint16_t *x, int n, int H, int W, int Cin, const uint
         hipLaunchKernelGGL((k_conv_mfma<1, true>), grid, dim3(256), 0, st, x, n, H, W, Cin, w, bias,
                            bias_stride, Cout, relu, y);
     return launched("isac_nn_conv_fwd");
+}
+
+int isac_nn_conv1_pool_fwd(const uint16_t *x0, int n, int H, int W, const float *master, int Cout,
+                           uint16_t *a, void *stream) {
+    if (Cout != 32 || n < 0 || H < 2 || W < 2 || (n > 0 && (!x0 || !master || !a)))
+        return fail(2, "isac_nn_conv1_pool_fwd: bad arguments (Cout must be 32)");
+    const size_t nq = (size_t)n * (H / 2) * (W / 2);
+    if (nq == 0) return 0;
+    hipLaunchKernelGGL(k_conv1_pool, dim3(blocks_for(nq, 256)), dim3(256), 0, (hipStream_t)stream, x0,
+                       n, H, W, master, a);
+    return launched("isac_nn_conv1_pool_fwd");
+}
+
+int isac_nn_conv_pool_fwd(const uint16_t *x, int n, int H, int W, int Cin, const uint16_t *w,
+                          const float *bias, int bias_stride, int Cout, uint16_t *a, void *stream) {
+    if (Cin < 16 || Cin % 16 || Cout < 16 || Cout % 16 || n < 0 || H < 2 || W < 2 ||
+        (n > 0 && (!x || !w || !a)))
+        return fail(2, "isac_nn_conv_pool_fwd: bad arguments (Cin, Cout multiples of 16)");
+    const size_t np = (size_t)n * (H / 2) * (W / 2) * 4;
+    if (np == 0) return 0;
+    dim3 grid(blocks_for(np, 256), (unsigned)((Cout + 63) / 64));
+    hipLaunchKernelGGL((k_conv_mfma<3, false, true>), grid, dim3(256), 0, (hipStream_t)stream, x, n, H, W,
+                       Cin, w, bias, bias_stride, Cout, 1, a);
+    return launched("isac_nn_conv_pool_fwd");
 }
 
 int isac_nn_maxpool2(const uint16_t *y, int n, int H, int W, int C, uint16_t *a, void *stream) {
@@ -795,7 +932,8 @@ int isac_nn_pack_weights(const float *master, int Cout, int Cin, int ks, uint16_
 }
 
 int isac_nn_proposals(const float *logits, int n, int ld, int n_ref, int n_psi, int n_shift, int k,
-                      int psi_r, int shift_r, int shift_ny, int min_refs, int *prop, void *stream) {
+                      int psi_r, int shift_r, int shift_ny, int min_refs, int *prop, float *conf,
+                      void *stream) {
     if (n < 0 || k < 1 || k > 8 || n_ref < 1 || n_ref > 512 || n_psi < 1 || 2 * n_psi > 256 ||
         n_shift < 1 || n_shift > 64 || ld < n_ref + 2 * n_psi + n_shift ||
         (psi_r >= 0 && (shift_r < 0 || shift_ny < 1 || n_shift % shift_ny)) ||
@@ -804,7 +942,7 @@ int isac_nn_proposals(const float *logits, int n, int ld, int n_ref, int n_psi, 
     if (n == 0) return 0;
     hipLaunchKernelGGL(k_proposals, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, (hipStream_t)stream,
                        logits, n, ld, n_ref, n_psi, n_shift, k, psi_r, shift_r, shift_ny > 0 ? shift_ny : 1,
-                       min_refs, reinterpret_cast<int4 *>(prop));
+                       min_refs, reinterpret_cast<int4 *>(prop), conf);
     return launched("isac_nn_proposals");
 }
 

@@ -136,8 +136,8 @@ def load(path: Optional[str] = None):
     lib.isac_mref_best_mixed_device.restype = c_int
     lib.isac_mref_proposal_device.argtypes = [vp, vp, vp, c_uint, c_uint, c_uint, c_uint, c_uint,
                                               c_float, c_float, c_float, vp, c_uint, c_uint,
-                                              c_float, c_uint, c_float, c_int, c_int, vp, vp,
-                                              vp, vp, ctypes.POINTER(c_uint),
+                                              c_float, c_uint, c_float, c_int, c_int, vp, c_float,
+                                              vp, vp, vp, vp, ctypes.POINTER(c_uint),
                                               ctypes.POINTER(c_uint), vp]
     lib.isac_mref_proposal_device.restype = c_int
     lib.isac_mref_align_batched_device.argtypes = [ctypes.POINTER(IsacMrefSubset), c_uint, c_int, vp]
@@ -424,12 +424,14 @@ def mref_best_mixed_device(sbj, ref, ring_num: int, shift_step: float, shift_rng
 def mref_proposal_device(sbj, ref, ring_num: int, shift_step: float, shift_rng_x: float,
                          shift_rng_y: float, prop, threshold: float, n_psi_bins: int = 72,
                          psi_half: float = 10.0, shift_half: int = 1, edge_fallback: bool = True,
-                         level2: bool = False, acc_xy=None, stream=None):
+                         level2: bool = False, conf=None, conf_min: float = 0.0, acc_xy=None,
+                         stream=None):
     """Narrowed search around proposals prop [P, K, 4] int32 (ref, mirror, psi
     bin, shift index).  Particles below ``threshold`` (or, with
     ``edge_fallback``, whose maximum lies on the edge of the admitted region)
     get, with ``level2``, every shift and psi of their proposed references, and
-    those still below the threshold the exhaustive search.  Returns a dict of
+    those still below the threshold the exhaustive search, as do the particles
+    whose ``conf`` [P] (float32) is below ``conf_min``.  Returns a dict of
     device tensors best_ref, best_val, best_peak [P, 4], fallback, plus
     n_level2 and n_fallback (ints)."""
     import torch
@@ -448,7 +450,8 @@ def mref_proposal_device(sbj, ref, ring_num: int, shift_step: float, shift_rng_x
         sbj.data_ptr(), ref.data_ptr(), acc_xy.data_ptr() if acc_xy is not None else None, P, N, D,
         ring_num, RING_LEN, shift_step, shift_rng_x, shift_rng_y, prop.data_ptr(), prop.shape[1],
         n_psi_bins, psi_half, shift_half, threshold, 1 if edge_fallback else 0,
-        1 if level2 else 0, out["best_ref"].data_ptr(), out["best_val"].data_ptr(),
+        1 if level2 else 0, conf.data_ptr() if conf is not None else None, conf_min,
+        out["best_ref"].data_ptr(), out["best_val"].data_ptr(),
         out["best_peak"].data_ptr(), out["fallback"].data_ptr(), ctypes.byref(nl2),
         ctypes.byref(nfb), st)
     if rc:

@@ -80,6 +80,30 @@ def calibrate_threshold(val, agree, target: float = 0.997):
     return float(np.nextafter(np.float32(bad[allowed]), np.float32(np.inf)))
 
 
+def calibrate_joint(val, agree, conf, target: float = 0.997, grid: int = 41):
+    """Confidence cut and CCF threshold together: for each candidate cut (a
+    quantile of conf, up to the 40 % quantile), the particles below it go to
+    the exhaustive search and the threshold is calibrate_threshold's over the
+    rest (with the allowance of the whole set); the pair with the fewest
+    fallbacks wins.  Returns (conf_min, threshold, expected fallback fraction)."""
+    val = np.asarray(val, np.float64)
+    agree = np.asarray(agree, bool)
+    conf = np.asarray(conf, np.float64)
+    n = val.size
+    allowed = int(math.floor((1.0 - target) * n))
+    best = (0.0, -3.0e38, 2.0)
+    for q in np.linspace(0.0, 0.4, grid):
+        cut = 0.0 if q == 0.0 else float(np.quantile(conf, q))
+        keep = conf >= cut
+        bad = np.sort(val[keep & ~agree])[::-1]
+        thr = -3.0e38 if bad.size <= allowed else \
+            float(np.nextafter(np.float32(bad[allowed]), np.float32(np.inf)))
+        fb = float(np.mean(~keep | (val < thr)))
+        if fb < best[2]:
+            best = (cut, thr, fb)
+    return best
+
+
 @dataclass
 class Proposer:
     net: posenet.PoseNet
@@ -91,20 +115,30 @@ class Proposer:
     psi_half: float = 10.0
     shift_half: int = 1
     level2: bool = True
+    conf_min: float = 0.0
 
     def run(self, sbj, ref, acc_xy=None):
-        prop = self.net.propose(sbj, self.k)
+        prop, conf = self.net.propose(sbj, self.k, with_conf=True)
         return binding.mref_proposal_device(sbj, ref, self.ring_num, self.shift_step,
                                             self.shift_rng, self.shift_rng, prop, self.threshold,
                                             n_psi_bins=self.net.n_psi, psi_half=self.psi_half,
                                             shift_half=self.shift_half, level2=self.level2,
-                                            acc_xy=acc_xy)
+                                            conf=conf, conf_min=self.conf_min, acc_xy=acc_xy)
 
 
-def calibrate(net, sbj, ref, d, peak, ring_num, shift_rng, target=0.997, **kw):
-    """Proposer whose threshold is calibrated on (sbj, exhaustive d / peak)."""
+def calibrate(net, sbj, ref, d, peak, ring_num, shift_rng, target=0.997, use_conf=True, **kw):
+    """Proposer whose threshold (and, with use_conf, confidence cut) is
+    calibrated on (sbj, exhaustive d / peak).  Returns (proposer, agreement of
+    the uncalibrated search)."""
     pr = Proposer(net, -3.0e38, ring_num, shift_rng, **kw)
-    o = pr.run(sbj, ref)
-    agree = agreement(o["best_ref"], o["best_peak"], d, peak)
-    pr.threshold = calibrate_threshold(o["best_val"].cpu().numpy(), agree.cpu().numpy(), target)
-    return pr, float(agree.double().mean())
+    prop, conf = net.propose(sbj, pr.k, with_conf=True)
+    o = binding.mref_proposal_device(sbj, ref, ring_num, pr.shift_step, shift_rng, shift_rng, prop,
+                                     -3.0e38, n_psi_bins=net.n_psi, psi_half=pr.psi_half,
+                                     shift_half=pr.shift_half, level2=pr.level2)
+    agree = agreement(o["best_ref"], o["best_peak"], d, peak).cpu().numpy()
+    val = o["best_val"].cpu().numpy()
+    if use_conf:
+        pr.conf_min, pr.threshold, _ = calibrate_joint(val, agree, conf.cpu().numpy(), target)
+    else:
+        pr.threshold = calibrate_threshold(val, agree, target)
+    return pr, float(agree.mean())

@@ -32,6 +32,8 @@ def _lib():
     sig = {
         "isac_nn_standardize": [vp, ci, ci, vp, vp],
         "isac_nn_conv1_fwd": [vp, ci, ci, ci, vp, ci, vp, vp],
+        "isac_nn_conv1_pool_fwd": [vp, ci, ci, ci, vp, ci, vp, vp],
+        "isac_nn_conv_pool_fwd": [vp, ci, ci, ci, ci, vp, vp, ci, ci, vp, vp],
         "isac_nn_conv_fwd": [vp, ci, ci, ci, ci, vp, vp, ci, ci, ci, ci, ci, vp, vp],
         "isac_nn_maxpool2": [vp, ci, ci, ci, ci, vp, vp],
         "isac_nn_maxpool2_bwd": [vp, vp, ci, ci, ci, ci, vp, vp],
@@ -40,7 +42,7 @@ def _lib():
         "isac_nn_softmax_xent": [vp, ci, ci, ci, ci, ci, vp, cf, cf, ci, vp, vp, vp],
         "isac_nn_adamw": [vp, vp, vp, vp, vp, sz, cf, cf, cf, cf, cf, ci, vp],
         "isac_nn_pack_weights": [vp, ci, ci, ci, vp, vp, vp],
-        "isac_nn_proposals": [vp, ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, vp, vp],
+        "isac_nn_proposals": [vp, ci, ci, ci, ci, ci, ci, ci, ci, ci, ci, vp, vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -174,6 +176,25 @@ class PoseNet:
         x0 = e(n, D, D)
         _call(lib, "isac_nn_standardize", _p(img), n, D, _p(x0), st)
         sv = {"x0": x0}
+        if not train:   # pooled layers with the pool fused into the convolution
+            a = e(n, D // 2, D // 2, self.channels[0])
+            _call(lib, "isac_nn_conv1_pool_fwd", _p(x0), n, D, D, _p(self.master(0)),
+                  self.channels[0], _p(a), st)
+            for li in range(1, 4):
+                H = self.sizes[li]
+                cin, cout, _ = self.layers[li]
+                b, bs = self._bias(li)
+                if li == 3 and not self.pool4:
+                    y = e(n, H, H, cout)
+                    _call(lib, "isac_nn_conv_fwd", _p(a), n, H, H, cin, _p(self.w[li]), b, bs, cout,
+                          3, 1, 0, _p(y), st)
+                    a = y
+                else:
+                    an = e(n, H // 2, H // 2, cout)
+                    _call(lib, "isac_nn_conv_pool_fwd", _p(a), n, H, H, cin, _p(self.w[li]), b, bs,
+                          cout, _p(an), st)
+                    a = an
+            return self._head(a, n, sv, train)
         y = e(n, D, D, self.channels[0])
         _call(lib, "isac_nn_conv1_fwd", _p(x0), n, D, D, _p(self.master(0)), self.channels[0],
               _p(y), st)
@@ -191,6 +212,12 @@ class PoseNet:
                 a = e(n, H // 2, H // 2, cout)
                 _call(lib, "isac_nn_maxpool2", _p(y), n, H, H, cout, _p(a), st)
             sv[f"y{li + 1}"], sv[f"a{li + 1}"] = y, a
+        return self._head(a, n, sv, train)
+
+    def _head(self, a, n, sv, train):
+        import torch
+        st, lib = self._st(), self.lib
+        e = lambda *shape, dt=torch.bfloat16: torch.empty(shape, dtype=dt, device=self.dev)  # noqa: E731
         h = e(n, self.hidden)
         b, bs = self._bias(4)
         _call(lib, "isac_nn_conv_fwd", _p(a), n, 1, 1, self.feat, _p(self.w[4]), b, bs,
@@ -299,24 +326,26 @@ class PoseNet:
 
     # --------------------------------------------------------------- inference
     def propose(self, imgs, k: int = 8, chunk: int = 4096, psi_r: int = 2, shift_r: int = 1,
-                min_refs: int = 3):
+                min_refs: int = 3, with_conf: bool = False):
         """Top-k proposals [n, k, 4] int32 (ref, mirror, psi bin, shift index),
         spread by suppressing neighbours within psi_r bins and shift_r steps
         (psi_r < 0: plain top-k of the summed logits), the first min_refs taken
-        from the min_refs best-ranked references."""
+        from the min_refs best-ranked references.  with_conf: also the softmax
+        mass of the reference head on the proposed references [n]."""
         import torch
         n = imgs.shape[0]
         ny = int(round(math.sqrt(self.n_shift)))
         if ny * ny != self.n_shift:
             psi_r, ny = -1, 1
         prop = torch.empty((n, k, 4), dtype=torch.int32, device=self.dev)
+        conf = torch.empty(n, dtype=torch.float32, device=self.dev) if with_conf else None
         for i0 in range(0, n, chunk):
             i1 = min(n, i0 + chunk)
             logits = self.forward(imgs[i0:i1])
             _call(self.lib, "isac_nn_proposals", _p(logits), i1 - i0, self.ld, self.n_ref,
                   self.n_psi, self.n_shift, k, psi_r, shift_r, ny, min_refs, _p(prop[i0:i1]),
-                  self._st())
-        return prop
+                  None if conf is None else _p(conf[i0:i1]), self._st())
+        return (prop, conf) if with_conf else prop
 
 
 def labels_from_search(d, peak_list, shift_rng: float, shift_step: float = 1.0,

@@ -45,6 +45,14 @@ int isac_nn_conv_fwd(const uint16_t *x, int n, int H, int W, int Cin, const uint
                      const float *bias, int bias_stride, int Cout, int ks, int relu, int out_f32,
                      void *y, void *stream);
 
+/* Inference forms with the 2x2 max pool fused (results equal the unfused
+ * convolution followed by isac_nn_maxpool2): conv1 (Cout 32) and the 3x3 MFMA
+ * convolution with bias and ReLU; a [n][H/2][W/2][Cout] bf16. */
+int isac_nn_conv1_pool_fwd(const uint16_t *x0, int n, int H, int W, const float *master, int Cout,
+                           uint16_t *a, void *stream);
+int isac_nn_conv_pool_fwd(const uint16_t *x, int n, int H, int W, int Cin, const uint16_t *w,
+                          const float *bias, int bias_stride, int Cout, uint16_t *a, void *stream);
+
 /* 2x2 max pooling, stride 2, floor: a [n][H/2][W/2][C]; C a multiple of 8. */
 int isac_nn_maxpool2(const uint16_t *y, int n, int H, int W, int C, uint16_t *a, void *stream);
 
@@ -99,10 +107,12 @@ int isac_nn_pack_weights(const float *master, int Cout, int Cin, int ks, uint16_
  * windows spread out.  The first min_refs choices take the best remaining
  * triple of the 1st, 2nd, ... ranked reference, so that many references are
  * proposed.  prop[i][j] = {ref, mirror, psi bin, shift index}; {-1, 0, 0, 0}
- * when fewer than k candidates remain. */
+ * when fewer than k candidates remain.  conf (or NULL) [n]: the softmax mass of
+ * the reference head on the distinct proposed references, the network's own
+ * estimate that the true reference is among them. */
 int isac_nn_proposals(const float *logits, int n, int ld, int n_ref, int n_psi, int n_shift,
                       int k, int psi_r, int shift_r, int shift_ny, int min_refs, int *prop,
-                      void *stream);
+                      float *conf, void *stream);
 
 #ifdef __cplusplus
 }

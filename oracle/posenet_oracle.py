@@ -304,3 +304,16 @@ def proposals(spec_heads, logits, k=8, psi_r=-1, shift_r=1, shift_ny=1, pool=(8,
                             abs(s2 % shift_ny - si % shift_ny) <= shift_r:
                         alive[q] = False
     return out
+
+
+def proposal_confidence(spec_heads, logits, prop):
+    """Softmax mass of the reference head on each particle's distinct proposed
+    references (entries with ref < 0 ignored)."""
+    n_ref = spec_heads[0]
+    out = np.zeros(len(prop))
+    for i, z in enumerate(np.asarray(logits, np.float64)):
+        zr = z[:n_ref]
+        e = np.exp(zr - zr.max())
+        refs = sorted(set(int(r) for r in np.asarray(prop[i])[:, 0] if r >= 0))
+        out[i] = e[refs].sum() / e.sum()
+    return out

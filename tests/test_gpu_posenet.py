@@ -103,10 +103,12 @@ def test_proposals_kernel_matches_brute_force(gpu, psi_r, min_refs):
     z[5, 10:34] = 0.25
     zt = torch.from_numpy(z).cuda()
     prop = torch.empty((n, 8, 4), dtype=torch.int32, device="cuda")
+    conf = torch.empty(n, dtype=torch.float32, device="cuda")
     posenet._call(lib, "isac_nn_proposals", zt.data_ptr(), n, ld, 10, 12, 9, 8, psi_r, 1, 3,
-                  min_refs, prop.data_ptr(), torch.cuda.current_stream().cuda_stream)
+                  min_refs, prop.data_ptr(), conf.data_ptr(), torch.cuda.current_stream().cuda_stream)
     want = N.proposals(heads, z, k=8, psi_r=psi_r, shift_r=1, shift_ny=3, min_refs=min_refs)
     np.testing.assert_array_equal(prop.cpu().numpy(), want)
+    np.testing.assert_allclose(conf.cpu().numpy(), N.proposal_confidence(heads, z, want), rtol=1e-5)
 
 
 @pytest.mark.parametrize("ks,H,cin,cout", [(3, 11, 32, 48), (1, 1, 64, 304), (3, 5, 16, 16)])
@@ -129,3 +131,19 @@ def test_conv_kernel_shapes(gpu, ks, H, cin, cout):
                   1, cout, ks, 1, 1, y.data_ptr(), torch.cuda.current_stream().cuda_stream)
     want = np.maximum(N.conv(x, w, b.astype(np.float64), ks), 0.0)
     np.testing.assert_allclose(y.cpu().numpy(), want, rtol=1e-5, atol=1e-4)
+
+
+@pytest.mark.parametrize("pool4", [True, False])
+def test_fused_inference_forward_is_bit_identical(gpu, pool4):
+    """Inference fuses the 2x2 pools into the convolutions; the logits must be
+    bit-identical to the training forward's (unfused kernels + pool)."""
+    import torch
+    from isac_amd import posenet
+    net = posenet.PoseNet(100, 49, D=76, seed=2, pool4=pool4,
+                          channels=(32, 64, 128, 128 if pool4 else 32))
+    _perturbed_params(net, 6)
+    img = torch.from_numpy(_images(37, 76, 8)).cuda()
+    a = net.forward(img).clone()
+    b = net.forward(img, train=True).clone()
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)

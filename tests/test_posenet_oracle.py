@@ -143,3 +143,21 @@ def test_proposal_suppression_spreads_windows():
                     db = min(db, 12 - db)
                     near = abs(p[a, 3] // 3 - p[b, 3] // 3) <= 1 and abs(p[a, 3] % 3 - p[b, 3] % 3) <= 1
                     assert not (db <= 2 and near)
+
+
+def test_joint_calibration_prefers_the_confidence_cut_when_it_helps():
+    from isac_amd import learned
+    rng = np.random.default_rng(3)
+    n = 2000
+    val = rng.normal(1.0, 0.1, n)
+    conf = rng.uniform(0.5, 1.0, n)
+    agree = np.ones(n, bool)
+    bad = np.argsort(conf)[:40]            # the disagreeing particles have the lowest confidence
+    agree[bad] = False
+    val[bad] = rng.normal(1.05, 0.05, 40)  # ... and high values: the threshold alone fails
+    cut, thr, fb = learned.calibrate_joint(val, agree, conf, target=0.997)
+    assert cut > 0 and fb < 0.1
+    keep = (conf >= cut) & (val >= thr)
+    assert np.sum(keep & ~agree) <= int(0.003 * n)
+    thr_only = learned.calibrate_threshold(val, agree, 0.997)
+    assert np.mean(val < thr_only) > fb
