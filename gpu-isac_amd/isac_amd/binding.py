@@ -33,7 +33,7 @@ EXTRA_SYMBOLS = (
     "gpu_last_error", "gpu_last_error_string", "gpu_clear_error", "isac_mref_align_device",
     "isac_mref_align_mixed_device", "isac_mref_best_mixed_device", "isac_profile_enable",
     "isac_profile_read", "isac_mref_align_batched_device", "isac_batched_report",
-    "isac_mref_proposal_device",
+    "isac_mref_proposal_device", "isac_assign_groups_device", "isac_assign_groups",
 )
 
 BATCH_PER_SUBSET = 0   # ISAC_BATCH_PER_SUBSET
@@ -140,6 +140,10 @@ def load(path: Optional[str] = None):
                                               vp, vp, vp, vp, ctypes.POINTER(c_uint),
                                               ctypes.POINTER(c_uint), vp]
     lib.isac_mref_proposal_device.restype = c_int
+    lib.isac_assign_groups_device.argtypes = [vp, c_uint, c_uint, c_uint, vp, vp]
+    lib.isac_assign_groups_device.restype = c_int
+    lib.isac_assign_groups.argtypes = [vp, c_uint, c_uint, vp]
+    lib.isac_assign_groups.restype = c_int
     lib.isac_mref_align_batched_device.argtypes = [ctypes.POINTER(IsacMrefSubset), c_uint, c_int, vp]
     lib.isac_mref_align_batched_device.restype = c_int
     ip = ctypes.POINTER(c_int)
@@ -470,6 +474,38 @@ def best_of_exhaustive(d, peak_list):
     best = torch.argmax(d, dim=0)
     cols = torch.arange(P, device=d.device)
     return best.to(torch.int32), d[best, cols], peak_list.view(N, P, 4)[best, cols]
+
+
+def assign_groups_device(d, n_img=None, stream=None):
+    """Equal-size group assignment from d [N, pitch] (device float32): returns the
+    n_img + 1 int32 layout (classes of n_img // N particles, leftovers, their
+    class) as a device tensor."""
+    import torch
+    lib = load()
+    N, pitch = d.shape
+    P = pitch if n_img is None else n_img
+    assert d.dtype == torch.float32 and d.is_cuda and d.is_contiguous()
+    out = torch.empty(P + 1, dtype=torch.int32, device=d.device)
+    st = (stream or torch.cuda.current_stream()).cuda_stream
+    rc = lib.isac_assign_groups_device(d.data_ptr(), N, P, pitch, out.data_ptr(), st)
+    if rc:
+        check(lib)
+        raise GpuAlignError(f"isac_assign_groups_device returned {rc}")
+    return out
+
+
+def assign_groups(d):
+    """Host form (numpy float32 [N, P] -> numpy int32 [P + 1]), the call that
+    replaces Util.assign_groups(address, numref, nima) in the driver."""
+    lib = load()
+    d = np.ascontiguousarray(d, dtype=np.float32)
+    N, P = d.shape
+    out = np.empty(P + 1, np.int32)
+    rc = lib.isac_assign_groups(d.ctypes.data, N, P, out.ctypes.data)
+    if rc:
+        check(lib)
+        raise GpuAlignError(f"isac_assign_groups returned {rc}")
+    return out
 
 
 class BatchedMref:

@@ -258,6 +258,21 @@ int isac_mref_align_batched_device(const IsacMrefSubset *subsets, unsigned int n
 void isac_batched_report(int *n_groups, int *n_nodes, int *replayed, int *nw, int *blocks,
                          float *cand_ms3);
 
+/* Equal-size group assignment from the scores d [n_ref][d_pitch] (columns
+ * 0..n_img-1): the step that consumes multi_ref_align's d
+ * (bin/sp_isac2_gpu.py:1119-1144), there EMAN2's Util.assign_groups on the
+ * host.  Pairs (class, particle) in descending score order (ties: lower class,
+ * then lower particle) place a particle that has no class yet in a class that
+ * holds fewer than cap = n_img / n_ref particles.  Output (n_img + 1 ints):
+ * class k's particles (ascending) at [k*cap, (k+1)*cap), then the particles no
+ * class took (ascending), then one class id for them (the class of their
+ * highest pair; 0 when there are none), the layout the driver reads.
+ * NaN scores count as -infinity; cap <= 2048.  Device form: device pointers,
+ * queued on `stream`, returns after completion; host form: host arrays. */
+int isac_assign_groups_device(const float *d_d, unsigned int n_ref, unsigned int n_img,
+                              unsigned int d_pitch, int *d_out, void *stream);
+int isac_assign_groups(const float *h_d, unsigned int n_ref, unsigned int n_img, int *h_out);
+
 /* No counterpart in the reference.  When enabled, the library brackets every
  * launch of its contraction+psi kernel with HIP events on the stream it
  * launches on, and accumulates launch count and total milliseconds. */

@@ -553,3 +553,47 @@ def windowed_best(sbj, ref, acc_xy, ring_num, shift_rng_x, shift_rng_y, shift_st
         out["mirror"][p], out["shift_idx"][p], out["t"][p] = m, s, t
         out["interior"][p] = inside
     return out
+
+
+# ------------------------------------------------------- group assignment
+# Restates the contract of the step after multi_ref_align (bin/sp_isac2_gpu.py:
+# 1119-1144, where EMAN2's Util.assign_groups -- absent here -- does it): pairs in
+# descending score order fill classes of cap = n_img // n_ref.  The tie order and
+# the class given to the leftover particles are this library's choice (EMAN2's
+# are unknown: parity with it is unpinned); the pins are the stability property
+# and hand-worked cases (tests/test_oracle.py).
+
+def assign_groups(d, n_img=None):
+    """Greedy equal-size assignment; returns the n_img + 1 int layout: class k's
+    particles ascending at [k*cap, (k+1)*cap), the leftover particles ascending,
+    the class of the highest leftover pair (0 if none).  NaN = -inf."""
+    d = np.asarray(d, np.float64)
+    if n_img is not None:
+        d = d[:, :n_img]
+    N, P = d.shape
+    v = np.where(np.isnan(d), -np.inf, d)
+    cap = P // N
+    n_idx, p_idx = np.meshgrid(np.arange(N), np.arange(P), indexing="ij")
+    order = np.lexsort((p_idx.ravel(), n_idx.ravel(), -v.ravel()))
+    cls = np.full(P, -1)
+    cnt = np.zeros(N, np.int64)
+    placed = 0
+    for k in order:
+        n, p = divmod(int(k), P)
+        if cls[p] < 0 and cnt[n] < cap:
+            cls[p] = n
+            cnt[n] += 1
+            placed += 1
+            if placed == N * cap:
+                break
+    out = []
+    for n in range(N):
+        out += sorted(np.nonzero(cls == n)[0].tolist())
+    rest = sorted(np.nonzero(cls < 0)[0].tolist())
+    out += rest
+    if rest:
+        cand = [(-v[n, p], n, p) for p in rest for n in range(N)]
+        out.append(min(cand)[1])
+    else:
+        out.append(0)
+    return np.array(out, np.int64)

@@ -360,3 +360,48 @@ def test_proposal_mask_window_shape():
     # bin 0 centre 2.5 deg, +-10 deg -> [-7.5, 12.5] deg -> t in {-5..8} mod 256
     assert set(ts) == set(list(range(0, 9)) + list(range(251, 256)))
     assert not mk[0].any() and not mk[2].any()                  # ref 9 is out of range: ignored
+
+
+def _blocking_pairs(d, out):
+    """Pairs (class b, particle p) that both prefer each other to what they have."""
+    N, P = d.shape
+    cap = P // N
+    cls = np.full(P, -1)
+    for n in range(N):
+        cls[out[n * cap:(n + 1) * cap]] = n
+    key = lambda n, p: (d[n, p], -n, -p)   # larger is better
+    bad = []
+    for p in range(P):
+        for b in range(N):
+            if b == cls[p]:
+                continue
+            if cls[p] >= 0 and key(b, p) <= key(cls[p], p):
+                continue
+            members = np.nonzero(cls == b)[0]
+            if len(members) < cap or any(key(b, q) < key(b, p) for q in members):
+                bad.append((b, p))
+    return bad
+
+
+def test_assign_groups_hand_case():
+    d = np.array([[4.0, 3.0, 2.0, 1.0], [3.5, 2.5, 1.5, 0.5]])
+    np.testing.assert_array_equal(O.assign_groups(d), [0, 1, 2, 3, 0])
+    # a leftover particle: 3 classes, 7 particles, cap 2
+    rng = np.random.default_rng(0)
+    d = rng.standard_normal((3, 7))
+    out = O.assign_groups(d)
+    assert len(out) == 8 and sorted(out[:7].tolist()) == list(range(7))
+    left = out[6]
+    assert out[7] == int(np.argmax(d[:, left]))
+
+
+def test_assign_groups_is_stable_and_equal_size():
+    rng = np.random.default_rng(1)
+    for N, P, ties in ((4, 13, False), (5, 20, True), (3, 3, False)):
+        d = rng.integers(0, 4, (N, P)).astype(float) if ties else rng.standard_normal((N, P))
+        out = O.assign_groups(d)
+        cap = P // N
+        assert sorted(out[:P].tolist()) == list(range(P))
+        for n in range(N):
+            assert list(out[n * cap:(n + 1) * cap]) == sorted(out[n * cap:(n + 1) * cap])
+        assert _blocking_pairs(d, out) == []
