@@ -125,3 +125,11 @@ def test_driver_post_rotation_matches_oracle():
                                     pk[:, 2::4].astype(np.float64))
     assert np.allclose(out[:, 1::4], sx, atol=1e-5) and np.allclose(out[:, 2::4], sy, atol=1e-5)
     assert np.array_equal(out[:, 0::4], pk[:, 0::4]) and np.array_equal(out[:, 3::4], pk[:, 3::4])
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and "one GPU" in r.stderr
