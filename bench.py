@@ -15,6 +15,8 @@ Also reported:
   cpu_baseline  the same algorithm's C restatement (FP32 build, OpenMP) on the
                 host cores, on a time-bounded sample of the same particles; its
                 FP64 build checks the GPU parameters on that sample
+  host_abi      the same search behind the reference's host-pointer ABI
+                (multi_ref_align: host images in, PCIe both ways), median of 5
 """
 from __future__ import annotations
 
@@ -79,6 +81,41 @@ def cpu_baseline(sbj_host, ref_host, acc, R, rng, gpu_peak, gpu_d, budget_s, cor
         "psi_max_err_deg_non_tie": float(dpsi[~tie].max()) if (~tie).any() else 0.0,
         "ccf_max_rel_err": float(drel.max())}
     return out
+
+
+def host_abi_rate(sbj_h, ref_h, R, rng, runs=5):
+    """The reference's ABI end to end: multi_ref_align with host image pointers
+    (gather into pinned memory, H2D, polar spectra, search, D2H of peak_list and
+    d).  Median of `runs` warm calls after one untimed call, one init for all."""
+    import ctypes
+    import numpy as np
+    from isac_amd import binding
+    lib = binding.load()
+    P, D, N = sbj_h.shape[0], sbj_h.shape[-1], ref_h.shape[0]
+    peak = np.zeros((N, 4 * P), np.float32)
+    d = np.zeros((N, P), np.float32)
+    cfg = binding.make_config(P, N, D, R, 1.0, rng, rng, "mref_aln")   # retained until gpu_clear
+    prm = lib.multi_ref_align_init(ctypes.byref(cfg), ctypes.c_uint(0))
+    binding.check(lib)
+    try:
+        for i in range(P):
+            prm[i].shift_x = 0.0
+            prm[i].shift_y = 0.0
+        ps = binding.get_c_ptr_array([sbj_h[i] for i in range(P)])
+        pr = binding.get_c_ptr_array([ref_h[i] for i in range(N)])
+        times = []
+        for _ in range(runs + 1):
+            t0 = time.perf_counter()
+            lib.multi_ref_align(ps, pr, peak.ctypes.data_as(binding.float_ptr),
+                                d.ctypes.data_as(binding.float_ptr), ctypes.c_uint(P),
+                                ctypes.c_float(0.9))
+            times.append(time.perf_counter() - t0)
+        binding.check(lib)
+    finally:
+        lib.gpu_clear()
+    med = float(np.median(times[1:]))
+    return {"value": P * N / med, "unit": "pairs/s", "median_s": med, "runs": runs,
+            "host_bytes_in": int(sbj_h.nbytes + ref_h.nbytes), "peak_list_d_bytes_out": int(peak.nbytes + d.nbytes)}
 
 
 def precision_report(n_particles=1000):
@@ -307,6 +344,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the mixed-precision forms")
+    ap.add_argument("--no-host-abi", action="store_true",
+                    help="skip the host-pointer ABI timing (multi_ref_align, PCIe included)")
     ap.add_argument("--probe", action="store_true",
                     help="one step of the workload plus a 1 GiB device copy, nothing printed: the "
                          "target of rocprofv3 --pmc runs (the copy calibrates the byte counters)")
@@ -422,6 +461,10 @@ def main():
             dt = (time.perf_counter() - t0) / k
             forms[name] = {"value": P * N / dt, "ms_per_step": dt * 1e3, "speedup_vs_fp32": ms_step / (dt * 1e3)}
         res["mfma_forms"] = forms
+    if not args.no_host_abi:
+        # not `value`: the same search behind the reference's host-pointer ABI,
+        # PCIe transfers and the host-side gather included
+        res["host_abi"] = host_abi_rate(sbj.cpu().numpy(), ref.cpu().numpy(), R, rng)
     if not args.no_cpu_baseline:
         cores = min(16, len(os.sched_getaffinity(0)))
         m = min(P, 4096)
