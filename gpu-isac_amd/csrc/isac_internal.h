@@ -204,6 +204,37 @@ hipError_t launch_class_average(const float *tr, int img_dim, const int *cid_idx
 hipError_t launch_tanl_filter(float *refs, int n, int img_dim, float cutoff, float falloff,
                               float2 *work, const float2 *tw_d, hipStream_t st);
 
+// Tanh low-pass with absolute frequency (cycles per pixel) and the phase ramp that
+// moves image n's content by shift[n]; output scaled by 1 / (D*D) (a proper inverse).
+hipError_t launch_filter_shift(float *refs, int n, int img_dim, float cutoff, float falloff,
+                               const float2 *shift, float2 *work, const float2 *tw_d,
+                               hipStream_t st);
+
+// Device scratch of the reference update (P particles, N classes, D pixels).
+struct UpdateScratch {
+    float4 *tp;       // [P] transform parameters (cos, sin, sx, sy)
+    int *mir;         // [P]
+    float2 *post;     // [P] shifts after the driver's rotation
+    float4 *raw;      // [P] (angle, sx, sy, mirror) as read from the peak list
+    int *cnt;         // [N]
+    int *off;         // [N + 1]
+    int *members;     // [P] particles grouped by class, ascending within a class
+    double2 *avg;     // [N] average shift per class
+    float2 *neg;      // [N] centring shift per class (-avg)
+    float2 *work;     // [N][D][D/2 + 1] spectra
+    const float2 *tw; // [D] twiddles e^{-2 pi i m / D}
+    int *bad;         // one flag: a class id outside [0, N)
+    float2 *avg_out;  // [N] average shift per class as floats, or NULL
+    int *members_out; // [N] member count per class, or NULL
+};
+
+// Reference update: class averages of the transformed particles, tanh filter and
+// centring shift into refs [N][D][D]; per-particle parameters (angle, sx, sy,
+// mirror) about the new centres into params [P].
+hipError_t launch_update_references(const float *imgs, int P, int D, const float *peak,
+                                   const int *cls, int N, float cutoff, float falloff,
+                                   UpdateScratch w, float *refs, float4 *params, hipStream_t st);
+
 // Mixed-precision (MFMA) multi-reference search: FP32 spectra -> 16-bit MFMA
 // operands [row][ceil(R/4)][kSlots][re,im][4 rings] (+ Frobenius norm per row).
 hipError_t launch_spec_to_mixed(const float2 *spec, const float2 *norm, int n_rows, int n_ring,

@@ -1261,6 +1261,168 @@ __global__ __launch_bounds__(256) void k_class_average(const float *__restrict__
     }
 }
 
+// ------------------------------------------------------- reference update
+// Device form of the multi-reference driver's reference update
+// (bin/sp_isac2_gpu.py:1146-1232): every particle is transformed with the
+// parameters of its class's peak and added to that class's average, the average
+// shift of each class centres the new reference, and each particle's shifts are
+// re-expressed about that centre.
+
+// Per particle i of class k = cls[i]: the peak (angle, sx, sy, mirror) of
+// peak_list[k][4i..4i+3] as the library writes it.  tp[i] = (cos, sin, sx, sy) for
+// the image transform of k_transform; post[i] = the shifts after the driver's
+// rotation (bin/sp_isac2_gpu.py:1016-1022, in FP32 like the driver's numpy code).
+// A class id outside [0, N) sets *bad and gives the particle neutral values.
+__global__ void k_upd_params(const float *__restrict__ peak, int P, int N,
+                             const int *__restrict__ cls, float4 *__restrict__ tp,
+                             int *__restrict__ mir, float2 *__restrict__ post,
+                             float4 *__restrict__ raw, int *__restrict__ bad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const int k = cls[i];
+    if (k < 0 || k >= N) {
+        atomicOr(bad, 1);
+        tp[i] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+        mir[i] = 0;
+        post[i] = f2(0.0f, 0.0f);
+        raw[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const float *q = peak + (size_t)k * 4 * P + 4 * (size_t)i;
+    const float a = q[0], sx = q[1], sy = q[2], m = q[3];
+    const float ang = (3.141592653589793f / 180.0f) * a;   // as k_transform
+    tp[i] = make_float4(cosf(ang), sinf(ang), sx, sy);
+    mir[i] = (m != 0.0f) ? 1 : 0;
+    raw[i] = make_float4(a, sx, sy, m);
+    const float rad = a * (float)(3.141592653589793 / 180.0);
+    const float c = cosf(rad), sn = -sinf(rad);
+    const float sxn = -sx, syn = -sy;
+    post[i] = f2(sxn * c - syn * sn, sxn * sn + syn * c);
+}
+
+// Block k: the particles of class k in ascending order.  pass 0 counts them into
+// cnt[k]; pass 1 writes them to members[off[k] ...].
+template <int PASS>
+__global__ __launch_bounds__(256) void k_class_members(const int *__restrict__ cls, int P,
+                                                       int *__restrict__ cnt,
+                                                       const int *__restrict__ off,
+                                                       int *__restrict__ members) {
+    __shared__ int wsum[4];
+    const int k = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    int base = PASS ? off[k] : 0;
+    for (int p0 = 0; p0 < P; p0 += 256) {
+        const int i = p0 + tid;
+        const bool hit = i < P && cls[i] == k;
+        const unsigned long long m = __ballot(hit);
+        if (lane == 0) wsum[w] = __popcll(m);
+        __syncthreads();
+        int before = 0, total = 0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            before += (v < w) ? wsum[v] : 0;
+            total += wsum[v];
+        }
+        if (PASS && hit) {
+            const unsigned long long lt = (lane == 0) ? 0ull : (m & (~0ull >> (64 - lane)));
+            members[base + before + __popcll(lt)] = i;
+        }
+        base += total;
+        __syncthreads();
+    }
+    if (!PASS && tid == 0) cnt[k] = base;
+}
+
+// Exclusive scan of cnt[0..N) into off[0..N] (one block).
+__global__ void k_scan_counts(const int *__restrict__ cnt, int N, int *__restrict__ off) {
+    if (threadIdx.x != 0) return;
+    int s = 0;
+    for (int k = 0; k < N; ++k) {
+        off[k] = s;
+        s += cnt[k];
+    }
+    off[N] = s;
+}
+
+// One thread per class: the shift sums of the driver (mirrored particles enter
+// with -sx), summed in double in ascending particle order, rounded to float as the
+// driver's MPI_FLOAT reduction does, divided by the member count in double.
+// neg[k] = -(avg_x, avg_y) (the centring shift of the new reference); avg_out and
+// members_out (either may be NULL) receive (avg_x, avg_y) as floats and the counts.
+__global__ void k_class_shift(const int *__restrict__ off, const int *__restrict__ members, int N,
+                              const float2 *__restrict__ post, const int *__restrict__ mir,
+                              double2 *__restrict__ avg, float2 *__restrict__ neg,
+                              float2 *__restrict__ avg_out, int *__restrict__ members_out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const int a = off[k], b = off[k + 1];
+    double sx = 0.0, sy = 0.0;
+    for (int j = a; j < b; ++j) {
+        const int i = members[j];
+        const float2 q = post[i];
+        sx += mir[i] ? -(double)q.x : (double)q.x;
+        sy += (double)q.y;
+    }
+    double ax = 0.0, ay = 0.0;
+    if (b > a) {
+        ax = (double)(float)sx / (double)(b - a);
+        ay = (double)(float)sy / (double)(b - a);
+    }
+    avg[k] = make_double2(ax, ay);
+    neg[k] = f2((float)-ax, (float)-ay);
+    if (avg_out) avg_out[k] = f2((float)ax, (float)ay);
+    if (members_out) members_out[k] = b - a;
+}
+
+// Per particle: (angle, sx -/+ avg_x, sy - avg_y, mirror) in the driver's
+// convention (bin/sp_isac2_gpu.py:1200-1208).
+__global__ void k_upd_params_out(const int *__restrict__ cls, int P, int N,
+                                 const float4 *__restrict__ raw, const float2 *__restrict__ post,
+                                 const int *__restrict__ mir, const double2 *__restrict__ avg,
+                                 float4 *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const int k = cls[i];
+    if (k < 0 || k >= N) {
+        out[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const double2 g = avg[k];
+    const float2 q = post[i];
+    const double x = mir[i] ? (double)q.x + g.x : (double)q.x - g.x;
+    out[i] = make_float4(raw[i].x, (float)x, (float)((double)q.y - g.y), raw[i].w);
+}
+
+// Block (tile, k): pixels of class k's average.  Each member image is transformed
+// as k_transform does and added in ascending particle order; the sum is scaled
+// by 1 / members (a class without members gives zeros).
+__global__ __launch_bounds__(256) void k_class_transform_average(
+    const float *__restrict__ imgs, int D, const int *__restrict__ off,
+    const int *__restrict__ members, const float4 *__restrict__ tp, const int *__restrict__ mir,
+    float *__restrict__ refs) {
+    const int k = blockIdx.y;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= D * D) return;
+    const int a = off[k], b = off[k + 1];
+    const float c = (float)(D / 2);
+    const int x = idx % D, y = idx / D;
+    float s = 0.0f;
+    for (int j = a; j < b; ++j) {
+        const int i = members[j];
+        const float4 q = tp[i];
+        float sx = mir[i] ? (float)(D - x) : (float)x;
+        float sy = (float)y;
+        sx -= c;
+        sy -= c;
+        const float tx = sx;
+        sx = tx * q.x - sy * q.y;
+        sy = tx * q.y + sy * q.x;
+        sx += c;
+        sy += c;
+        s += bilinear(imgs + (size_t)i * D * D, D, sx + q.z, sy + q.w);
+    }
+    refs[(size_t)k * D * D + idx] = (b > a) ? s * (1.0f / (float)(b - a)) : 0.0f;
+}
+
 // Tangent filter, three separable passes over a half-plane spectrum
 // F[n][ky][kx], kx = 0..D/2 (cuda/gpu_aln_noref.cu:1007-1037, :2203-2266).
 __global__ void k_row_dft(const float *__restrict__ refs, int D, float2 *__restrict__ F,
@@ -1278,14 +1440,22 @@ __global__ void k_row_dft(const float *__restrict__ refs, int D, float2 *__restr
     }
 }
 
+// Column pass: forward DFT over y, the tanh low-pass, and (shift != NULL) the phase
+// ramp exp(-2 pi i (kx dx + ky dy) / D) that moves image n's content by
+// shift[n] = (dx, dy) (signed ky; ky = D/2 taken as +D/2), then the inverse DFT.
+// abs_freq = 0: the reference's frequency coordinates (x = kx / (D/2+1) / 2,
+// y = min(ky, D-ky) / D); 1: absolute frequency in cycles per pixel
+// (x = kx / D, y = min(ky, D-ky) / D).
 __global__ void k_col_filter(float2 *__restrict__ F, int D, const float2 *__restrict__ tw,
-                             float cutoff, float falloff) {
+                             float cutoff, float falloff, const float2 *__restrict__ shift,
+                             int abs_freq) {
     extern __shared__ float2 col[];
     const int n = blockIdx.y, kx = blockIdx.x, Dh = D / 2 + 1;
     float2 *base = F + (size_t)n * D * Dh + kx;
     const float cc = 3.141592653589793f / (2.0f * falloff * cutoff);
-    const float xx = (float)kx / (float)Dh * 0.5f;
+    const float xx = abs_freq ? (float)kx / (float)D : (float)kx / (float)Dh * 0.5f;
     const float half = (float)D / 2.0f;
+    const float2 sh = shift ? shift[n] : f2(0.0f, 0.0f);
     for (int ky = threadIdx.x; ky < D; ky += blockDim.x) {
         float re = 0.0f, im = 0.0f;
         for (int y = 0; y < D; ++y) {
@@ -1298,7 +1468,19 @@ __global__ void k_col_filter(float2 *__restrict__ F, int D, const float2 *__rest
         yy = yy / half * 0.5f;
         const float dd = sqrtf(xx * xx + yy * yy);
         const float h = 0.5f * (tanhf(cc * (dd + cutoff)) - tanhf(cc * (dd - cutoff)));
-        col[ky] = f2(re * h, im * h);
+        re *= h;
+        im *= h;
+        if (shift) {
+            const int kys = (2 * ky > D) ? ky - D : ky;
+            float t = ((float)kx * sh.x + (float)kys * sh.y) / (float)D;   // turns
+            t -= rintf(t);
+            float sn, cs;
+            sincospif(-2.0f * t, &sn, &cs);
+            const float r2 = re * cs - im * sn;
+            im = re * sn + im * cs;
+            re = r2;
+        }
+        col[ky] = f2(re, im);
     }
     __syncthreads();
     for (int y = threadIdx.x; y < D; y += blockDim.x) {
@@ -1313,12 +1495,14 @@ __global__ void k_col_filter(float2 *__restrict__ F, int D, const float2 *__rest
     }
 }
 
+// Row C2R.  gain > 0 scales every image by gain; otherwise the reference's
+// scaling: 1 / (D*D)^2 for image 0 only (quirk of cuda/gpu_aln_noref.cu:2235-2262).
 __global__ void k_row_c2r(float *__restrict__ refs, int D, const float2 *__restrict__ F,
-                          const float2 *__restrict__ tw) {
+                          const float2 *__restrict__ tw, float gain_all) {
     const int n = blockIdx.y, y = blockIdx.x, Dh = D / 2 + 1;
     const float2 *row = F + ((size_t)n * D + y) * Dh;
     const float dd = (float)D * (float)D;
-    const float gain = (n == 0) ? 1.0f / (dd * dd) : 1.0f;
+    const float gain = gain_all > 0.0f ? gain_all : ((n == 0) ? 1.0f / (dd * dd) : 1.0f);
     for (int x = threadIdx.x; x < D; x += blockDim.x) {
         float s = 0.0f;
         for (int kx = 0; kx < Dh; ++kx) {
@@ -1543,19 +1727,55 @@ hipError_t launch_class_average(const float *tr, int img_dim, const int *cid_idx
     return hipGetLastError();
 }
 
-hipError_t launch_tanl_filter(float *refs, int n, int img_dim, float cutoff, float falloff,
-                              float2 *work, const float2 *tw_d, hipStream_t st) {
+static hipError_t filter_passes(float *refs, int n, int D, float cutoff, float falloff,
+                                const float2 *shift, int abs_freq, float gain, float2 *work,
+                                const float2 *tw_d, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    const int D = img_dim, Dh = D / 2 + 1;
+    const int Dh = D / 2 + 1;
     hipLaunchKernelGGL(k_row_dft, dim3(D, n), dim3(128), 0, st, refs, D, work, tw_d);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_col_filter, dim3(Dh, n), dim3(128), D * sizeof(float2), st, work, D,
-                       tw_d, cutoff, falloff);
+                       tw_d, cutoff, falloff, shift, abs_freq);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_row_c2r, dim3(D, n), dim3(128), 0, st, refs, D, work, tw_d);
+    hipLaunchKernelGGL(k_row_c2r, dim3(D, n), dim3(128), 0, st, refs, D, work, tw_d, gain);
     return hipGetLastError();
+}
+
+hipError_t launch_tanl_filter(float *refs, int n, int img_dim, float cutoff, float falloff,
+                              float2 *work, const float2 *tw_d, hipStream_t st) {
+    return filter_passes(refs, n, img_dim, cutoff, falloff, nullptr, 0, -1.0f, work, tw_d, st);
+}
+
+hipError_t launch_filter_shift(float *refs, int n, int img_dim, float cutoff, float falloff,
+                               const float2 *shift, float2 *work, const float2 *tw_d,
+                               hipStream_t st) {
+    const float dd = (float)img_dim * (float)img_dim;
+    return filter_passes(refs, n, img_dim, cutoff, falloff, shift, 1, 1.0f / dd, work, tw_d, st);
+}
+
+hipError_t launch_update_references(const float *imgs, int P, int D, const float *peak,
+                                   const int *cls, int N, float cutoff, float falloff,
+                                   UpdateScratch w, float *refs, float4 *params, hipStream_t st) {
+    if (P <= 0 || N <= 0) return hipSuccess;
+    const int gp = (P + 255) / 256;
+    hipLaunchKernelGGL(k_upd_params, dim3(gp), dim3(256), 0, st, peak, P, N, cls, w.tp, w.mir,
+                       w.post, w.raw, w.bad);
+    hipLaunchKernelGGL(k_class_members<0>, dim3(N), dim3(256), 0, st, cls, P, w.cnt, w.off,
+                       w.members);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(64), 0, st, w.cnt, N, w.off);
+    hipLaunchKernelGGL(k_class_members<1>, dim3(N), dim3(256), 0, st, cls, P, w.cnt, w.off,
+                       w.members);
+    hipLaunchKernelGGL(k_class_shift, dim3((N + 63) / 64), dim3(64), 0, st, w.off, w.members, N,
+                       w.post, w.mir, w.avg, w.neg, w.avg_out, w.members_out);
+    hipLaunchKernelGGL(k_upd_params_out, dim3(gp), dim3(256), 0, st, cls, P, N, w.raw, w.post,
+                       w.mir, w.avg, params);
+    hipLaunchKernelGGL(k_class_transform_average, dim3((D * D + 255) / 256, N), dim3(256), 0, st,
+                       imgs, D, w.off, w.members, w.tp, w.mir, refs);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_filter_shift(refs, N, D, cutoff, falloff, w.neg, w.work, w.tw, st);
 }
 
 hipError_t launch_prop_items(const int *prop, int n_img, int K, int N, int nx, int ny, int hw,

@@ -34,6 +34,7 @@ EXTRA_SYMBOLS = (
     "isac_mref_align_mixed_device", "isac_mref_best_mixed_device", "isac_profile_enable",
     "isac_profile_read", "isac_mref_align_batched_device", "isac_batched_report",
     "isac_mref_proposal_device", "isac_assign_groups_device", "isac_assign_groups",
+    "isac_update_references_device",
 )
 
 BATCH_PER_SUBSET = 0   # ISAC_BATCH_PER_SUBSET
@@ -144,6 +145,9 @@ def load(path: Optional[str] = None):
     lib.isac_assign_groups_device.restype = c_int
     lib.isac_assign_groups.argtypes = [vp, c_uint, c_uint, vp]
     lib.isac_assign_groups.restype = c_int
+    lib.isac_update_references_device.argtypes = [vp, c_uint, c_uint, vp, vp, c_uint, c_float,
+                                                  c_float, vp, vp, vp, vp, vp]
+    lib.isac_update_references_device.restype = c_int
     lib.isac_mref_align_batched_device.argtypes = [ctypes.POINTER(IsacMrefSubset), c_uint, c_int, vp]
     lib.isac_mref_align_batched_device.restype = c_int
     ip = ctypes.POINTER(c_int)
@@ -506,6 +510,48 @@ def assign_groups(d):
         check(lib)
         raise GpuAlignError(f"isac_assign_groups returned {rc}")
     return out
+
+
+def classes_from_assignment(groups, n_ref):
+    """Class of each particle from assign_groups' n_img + 1 layout, as the driver
+    builds ``belongsto`` (bin/sp_isac2_gpu.py:1122-1135)."""
+    g = np.asarray(groups)
+    P = g.size - 1
+    cap = P // n_ref
+    cls = np.empty(P, np.int32)
+    for i in range(cap * n_ref):
+        cls[g[i]] = i // cap
+    for i in range(P - cap * n_ref):
+        cls[g[cap * n_ref + i]] = g[-1]
+    return cls
+
+
+def update_references_device(imgs, peak_list, cls, n_ref, cutoff, falloff, stream=None):
+    """Reference update after the assignment (isac_update_references_device):
+    imgs [P, D, D], peak_list [N, 4P] (the library's layout, before the driver's
+    shift rotation), cls [P] int32 class per particle.  Returns (refs [N, D, D],
+    params [P, 4] = (angle, sx, sy, mirror) about the new centres, members [N],
+    shift_avg [N, 2]) as device tensors."""
+    import torch
+    lib = load()
+    P, D = imgs.shape[0], imgs.shape[-1]
+    assert imgs.dtype == torch.float32 and imgs.is_cuda and imgs.is_contiguous()
+    assert peak_list.shape == (n_ref, 4 * P) and peak_list.is_contiguous()
+    cls = cls.to(device=imgs.device, dtype=torch.int32).contiguous()
+    assert cls.numel() == P
+    refs = torch.empty((n_ref, D, D), dtype=torch.float32, device=imgs.device)
+    params = torch.empty((P, 4), dtype=torch.float32, device=imgs.device)
+    members = torch.empty(n_ref, dtype=torch.int32, device=imgs.device)
+    shift_avg = torch.empty((n_ref, 2), dtype=torch.float32, device=imgs.device)
+    st = (stream or torch.cuda.current_stream()).cuda_stream
+    rc = lib.isac_update_references_device(imgs.data_ptr(), P, D, peak_list.data_ptr(), cls.data_ptr(),
+                                           n_ref, float(cutoff), float(falloff), refs.data_ptr(),
+                                           params.data_ptr(), members.data_ptr(),
+                                           shift_avg.data_ptr(), st)
+    if rc:
+        check(lib)
+        raise GpuAlignError(f"isac_update_references_device returned {rc}")
+    return refs, params, members, shift_avg
 
 
 class BatchedMref:

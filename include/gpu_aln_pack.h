@@ -273,6 +273,29 @@ int isac_assign_groups_device(const float *d_d, unsigned int n_ref, unsigned int
                               unsigned int d_pitch, int *d_out, void *stream);
 int isac_assign_groups(const float *h_d, unsigned int n_ref, unsigned int n_img, int *h_out);
 
+/* Reference update after the assignment: the step the driver runs on the host
+ * with EMAN2 (bin/sp_isac2_gpu.py:1146-1232), on the device.  Inputs: d_imgs
+ * [n_img][img_dim][img_dim], d_peak_list [n_ref][4*n_img] as multi_ref_align /
+ * isac_mref_align_device write it (before the driver's shift rotation), d_class
+ * [n_img] the class of each particle (0..n_ref-1).  For class k: every member is
+ * transformed with its peak of class k (the reference-free path's image
+ * transform, cuda/gpu_aln_noref.cu:1539-1590) and averaged in ascending particle
+ * order; the average shift of the members (their rotated shifts,
+ * bin/sp_isac2_gpu.py:1016-1022, mirrored members with -sx) is
+ * (avg_x, avg_y); d_refs[k] = the average low-passed with the tanh filter
+ * (cutoff, falloff in absolute frequency, cycles per pixel) and moved by
+ * (-avg_x, -avg_y) with a Fourier phase ramp.  d_params [n_img][4] = (angle,
+ * sx -/+ avg_x (+ for mirrored), sy - avg_y, mirror) in the driver's
+ * convention (bin/sp_isac2_gpu.py:1200-1208).  d_members [n_ref] and
+ * d_shift_avg [n_ref][2] (either may be NULL) receive the member counts and
+ * (avg_x, avg_y).  A class without members gets a zero image.  img_dim <= 128.
+ * Queued on `stream`; returns after completion (2 if a class id is outside
+ * [0, n_ref)). */
+int isac_update_references_device(const float *d_imgs, unsigned int n_img, unsigned int img_dim,
+                                  const float *d_peak_list, const int *d_class, unsigned int n_ref,
+                                  float cutoff, float falloff, float *d_refs, float *d_params,
+                                  int *d_members, float *d_shift_avg, void *stream);
+
 /* No counterpart in the reference.  When enabled, the library brackets every
  * launch of its contraction+psi kernel with HIP events on the stream it
  * launches on, and accumulates launch count and total milliseconds. */

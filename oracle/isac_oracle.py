@@ -24,6 +24,9 @@ source it restates (paths relative to the ucdavis/GPU-ISAC tree):
   class_averages          cuda/gpu_aln_noref.cu:1592-1622
   tangent_filter          cuda/gpu_aln_noref.cu:1007-1037, :2203-2266
   driver_post_rotation    bin/sp_isac2_gpu.py:1016-1022
+  filter_shift            the tanh low-pass of :1020-1035 in absolute frequency, with
+                          the Fourier shift EMAN2's fshift applies (bin/sp_isac2_gpu.py:1221)
+  update_references       bin/sp_isac2_gpu.py:1146-1232 with transform_images
 
 Deliberate, documented divergences from the (unrunnable) CUDA path:
   * bilinear weights are exact (CUDA's texture unit uses 8-bit fractions);
@@ -470,6 +473,78 @@ def ref_free_iteration(sbj, refs, cid, acc_xy, ring_num, shifts, img_dim,
     cid = np.asarray(cid)
     cid_idx = list(np.flatnonzero(np.r_[True, cid[1:] != cid[:-1]])) + [len(cid)]
     return prm, class_averages(tr, cid_idx)
+
+
+# ------------------------------------------------------------ reference update
+# Device reference update (isac_update_references_device): the host step of
+# bin/sp_isac2_gpu.py:1146-1232, with the reference-free path's image transform
+# (transform_images) in place of EMAN2's rot_shift2D, an absolute-frequency tanh
+# low-pass for filt_tanl and a Fourier phase ramp for fshift.  EMAN2 is absent,
+# so agreement with its own routines is unpinned; the pins are closed forms
+# (integer shifts are circular shifts, a cosine keeps its shape, the filter gain
+# on a cosine) and the integration check that noise-free particles averaged with
+# their exhaustive peaks give back their reference (tests/test_oracle.py).
+
+def filter_shift(imgs, cutoff, falloff, dx, dy):
+    """Per image n: multiply the 2-D spectrum by H(r) exp(-2 pi i (kx dx_n + ky
+    dy_n) / D) and invert (numpy irfft2: the real part of the kx = 0 and kx = D/2
+    columns), so the content moves by (dx_n, dy_n).
+    H(r) = 0.5 (tanh(c (r + f)) - tanh(c (r - f))), c = pi / (2 falloff f),
+    r = sqrt(kx^2 + min(ky, D - ky)^2) / D cycles per pixel; ky is signed
+    (ky = D/2 taken as +D/2) in the phase ramp."""
+    imgs = np.asarray(imgs, dtype=np.float64)
+    N, D = imgs.shape[0], imgs.shape[-1]
+    F = np.fft.rfft2(imgs)                                   # (N, D, D//2+1)
+    kx = np.arange(D // 2 + 1, dtype=np.float64)
+    ky = np.arange(D, dtype=np.float64)
+    r = np.sqrt((kx[None, :] / D) ** 2 + (np.minimum(ky, D - ky)[:, None] / D) ** 2)
+    cc = np.pi / (2.0 * falloff * cutoff)
+    H = 0.5 * (np.tanh(cc * (r + cutoff)) - np.tanh(cc * (r - cutoff)))
+    kys = np.where(2 * ky > D, ky - D, ky)
+    dx = np.asarray(dx, np.float64).reshape(-1, 1, 1)
+    dy = np.asarray(dy, np.float64).reshape(-1, 1, 1)
+    ramp = np.exp(-2j * np.pi * (kx[None, None, :] * dx + kys[None, :, None] * dy) / D)
+    return np.fft.irfft2(F * H[None] * ramp, s=(D, D))
+
+
+def update_references(imgs, peak_list, cls, n_ref, cutoff, falloff):
+    """(refs [N, D, D], params [P, 4], members [N], shift_avg [N, 2]).
+
+    Particle i of class k = cls[i] has the peak (alpha, sx, sy, m) =
+    peak_list[k, 4i:4i+4]; (sx', sy') = driver_post_rotation(alpha, sx, sy);
+    class sums of (m ? -sx' : sx', sy') in ascending particle order, rounded to
+    float32 (the driver's MPI_FLOAT reduction) and divided by the member count;
+    params = (alpha, m ? sx' + avg_x : sx' - avg_x, sy' - avg_y, m)
+    (bin/sp_isac2_gpu.py:1200-1208); refs = filter_shift(class mean of
+    transform_images(imgs, alpha, sx, sy, m), cutoff, falloff, -avg).  A class
+    without members gives zeros."""
+    imgs = np.asarray(imgs, dtype=np.float64)
+    P = imgs.shape[0]
+    cls = np.asarray(cls, dtype=np.int64)
+    pk = np.asarray(peak_list, dtype=np.float64).reshape(n_ref, P, 4)
+    q = pk[cls, np.arange(P)]
+    alpha, sx, sy, m = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    mir = m != 0
+    sxp, syp = driver_post_rotation(alpha, sx, sy)
+    members = np.bincount(cls, minlength=n_ref)
+    sum_x = np.zeros(n_ref)
+    sum_y = np.zeros(n_ref)
+    for i in range(P):
+        sum_x[cls[i]] += -sxp[i] if mir[i] else sxp[i]
+        sum_y[cls[i]] += syp[i]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        avg_x = np.where(members > 0, sum_x.astype(np.float32).astype(np.float64) / members, 0.0)
+        avg_y = np.where(members > 0, sum_y.astype(np.float32).astype(np.float64) / members, 0.0)
+    params = np.stack([alpha, np.where(mir, sxp + avg_x[cls], sxp - avg_x[cls]),
+                       syp - avg_y[cls], m], axis=1)
+    tr = transform_images(imgs, alpha, sx, sy, mir.astype(int))
+    D = imgs.shape[-1]
+    avg = np.zeros((n_ref, D, D))
+    for k in range(n_ref):
+        if members[k]:
+            avg[k] = tr[cls == k].mean(0)
+    refs = filter_shift(avg, cutoff, falloff, -avg_x, -avg_y)
+    return refs, params, members, np.stack([avg_x, avg_y], axis=1)
 
 
 # ------------------------------------------------------------ narrowed search

@@ -405,3 +405,77 @@ def test_assign_groups_is_stable_and_equal_size():
         for n in range(N):
             assert list(out[n * cap:(n + 1) * cap]) == sorted(out[n * cap:(n + 1) * cap])
         assert _blocking_pairs(d, out) == []
+
+
+# ------------------------------------------------------ reference update
+
+def _pass_all():
+    # cutoff 100, falloff 0.01: c (r +- f) = +-157 for every r <= 0.71, so H = 1
+    return 100.0, 0.01
+
+
+def test_filter_shift_integer_shift_is_circular_shift():
+    rng = np.random.default_rng(5)
+    img = rng.standard_normal((2, 12, 12))
+    out = O.filter_shift(img, *_pass_all(), [3.0, -1.0], [-2.0, 4.0])
+    np.testing.assert_allclose(out[0], np.roll(img[0], shift=(-2, 3), axis=(0, 1)), atol=1e-12)
+    np.testing.assert_allclose(out[1], np.roll(img[1], shift=(4, -1), axis=(0, 1)), atol=1e-12)
+
+
+def test_filter_shift_moves_a_cosine_by_a_fraction():
+    D = 16
+    y, x = np.mgrid[0:D, 0:D].astype(float)
+    wave = lambda X, Y: np.cos(2 * np.pi * (3 * X + 2 * Y) / D + 0.3)
+    out = O.filter_shift(wave(x, y)[None], *_pass_all(), [0.5], [-0.25])
+    np.testing.assert_allclose(out[0], wave(x - 0.5, y + 0.25), atol=1e-12)
+
+
+def test_filter_shift_gain_on_a_cosine_and_a_constant():
+    D, f, ff = 32, 0.2, 0.3
+    y, x = np.mgrid[0:D, 0:D].astype(float)
+    c = np.pi / (2 * ff * f)
+    H = lambda r: 0.5 * (np.tanh(c * (r + f)) - np.tanh(c * (r - f)))
+    r = np.hypot(4, 3) / D                      # frequency (4, 3) cycles over D pixels
+    img = np.cos(2 * np.pi * (4 * x + 3 * y) / D)
+    out = O.filter_shift(img[None], f, ff, [0.0], [0.0])[0]
+    np.testing.assert_allclose(out, H(r) * img, atol=1e-12)
+    out = O.filter_shift(np.full((1, D, D), 2.0), f, ff, [0.0], [0.0])[0]
+    np.testing.assert_allclose(out, 2.0 * np.tanh(c * f), atol=1e-12)
+
+
+def test_update_references_hand_case():
+    # 3 particles, 2 classes; zero angles so the driver rotation only negates shifts
+    D = 8
+    imgs = np.zeros((3, D, D))
+    imgs[:, 4, 4] = [1.0, 2.0, 3.0]
+    peak = np.zeros((2, 12))
+    peak[0, 0:4] = [0.0, 1.0, 0.0, 0.0]      # particle 0 in class 0: sx = 1
+    peak[0, 8:12] = [0.0, 0.0, 2.0, 1.0]     # particle 2 in class 0: sy = 2, mirrored
+    peak[1, 4:8] = [0.0, -1.0, 1.0, 0.0]     # particle 1 in class 1
+    refs, params, members, avg = O.update_references(imgs, peak, [0, 1, 0], 2, *_pass_all())
+    np.testing.assert_array_equal(members, [2, 1])
+    # rotated shifts: p0 (-1, 0); p2 (0, -2), mirrored -> -sx' = 0; p1 (1, -1)
+    np.testing.assert_allclose(avg, [[-0.5, -1.0], [1.0, -1.0]])
+    np.testing.assert_allclose(params, [[0.0, -1.0 + 0.5, 0.0 + 1.0, 0.0],
+                                        [0.0, 1.0 - 1.0, -1.0 + 1.0, 0.0],
+                                        [0.0, 0.0 - 0.5, -2.0 + 1.0, 1.0]])
+
+
+def test_update_references_recovers_references_from_exact_peaks():
+    # noise-free particles, classes = generating references, peaks from the
+    # exhaustive search: the transformed members line up with their reference,
+    # so each class average is that reference (bilinear blur aside), moved by
+    # minus the class's average shift
+    ds = synth.make_dataset(12, 3, 64, 28.0, None, 2.0, True)
+    res = O.multi_ref_align(ds.sbj, ds.ref, np.zeros((12, 2)), 29, 2, 2, 1)
+    peak = np.zeros((3, 48))
+    for k in range(3):
+        peak[k, 0::4], peak[k, 1::4] = res["angle"][k], res["sx"][k]
+        peak[k, 2::4], peak[k, 3::4] = res["sy"][k], res["mirror"][k]
+    refs, params, members, avg = O.update_references(ds.sbj, peak, ds.ref_idx, 3, *_pass_all())
+    np.testing.assert_array_equal(members, np.bincount(ds.ref_idx, minlength=3))
+    yy, xx = np.mgrid[0:64, 0:64] - 32
+    inside = np.hypot(xx, yy) < 24
+    for k in range(3):
+        back = O.filter_shift(refs[k:k + 1], *_pass_all(), [avg[k, 0]], [avg[k, 1]])[0]
+        assert np.corrcoef(back[inside], ds.ref[k][inside])[0, 1] > 0.999
