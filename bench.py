@@ -232,6 +232,42 @@ def batched_report(steps=3, warmup=1, n_subsets=64):
     return out
 
 
+def iteration_report(steps=5, warmup=1, fl=0.25, ff=0.2):
+    """One multi-reference generation on the device at the headline size (20 000
+    particles x 100 references, 76 px): the search (isac_mref_align_device), the
+    equal-size assignment from d (isac_assign_groups_device) and the reference
+    update (isac_update_references_device), each timed over `steps` calls."""
+    import numpy as np
+    import torch
+    from isac_amd import binding, synth
+    P, N, D, radius, snr, max_shift, int_shifts, R, rng = WORKLOADS["cfg2"]
+    sbj, ref, _ = synth.make_dataset_torch(P, N, D, radius, snr, max_shift, int_shifts, "cuda")
+    torch.cuda.synchronize()
+
+    def timed(fn):
+        for _ in range(warmup):
+            out = fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            out = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps, out
+
+    t_search, (peak, d) = timed(lambda: binding.mref_align_device(sbj, ref, R, 1.0, rng, rng))
+    t_assign, groups = timed(lambda: binding.assign_groups_device(d))
+    cls = torch.from_numpy(binding.classes_from_assignment(groups.cpu().numpy(), N)).cuda()
+    t_update, (refs, params, members, avg) = timed(
+        lambda: binding.update_references_device(sbj, peak, cls, N, fl, ff))
+    m = members.cpu().numpy()
+    return {"report": "one multi-reference generation on the device (cfg2 sizes)",
+            "particles": P, "refs": N, "img_dim": D, "steps": steps, "warmup": warmup,
+            "search_s": t_search, "assign_s": t_assign, "update_s": t_update,
+            "generation_s": t_search + t_assign + t_update,
+            "class_size_min": int(m.min()), "class_size_max": int(m.max()),
+            "filter": {"cutoff": fl, "falloff": ff}}
+
+
 def learned_report(n_train=200000, n_cal=10000, n_test=100000, epochs=16, oracle_sample=1000,
                    target=0.997, snr=0.1, integer_shifts=True, log=None):
     """Config 5: the pose-proposal network trained on exhaustive-search labels of
@@ -349,9 +385,10 @@ def main():
     ap.add_argument("--probe", action="store_true",
                     help="one step of the workload plus a 1 GiB device copy, nothing printed: the "
                          "target of rocprofv3 --pmc runs (the copy calibrates the byte counters)")
-    ap.add_argument("--report", choices=["precision", "batched", "learned"], default=None,
+    ap.add_argument("--report", choices=["precision", "batched", "learned", "iteration"], default=None,
                     help="precision: config-3 accuracy report; batched: config-4 uneven-subset "
-                         "throughput, graph vs one launch per subset")
+                         "throughput, graph vs one launch per subset; learned: config-5 proposal "
+                         "search; iteration: search + assignment + reference update at cfg2 sizes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -368,6 +405,9 @@ def main():
         return
     if args.report == "learned":
         print(json.dumps(learned_report()))
+        return
+    if args.report == "iteration":
+        print(json.dumps(iteration_report()))
         return
     if args.report == "batched":
         print(json.dumps(batched_report(steps=args.steps if args.steps != 10 else 3,

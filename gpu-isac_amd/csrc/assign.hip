@@ -87,11 +87,19 @@ __global__ __launch_bounds__(256) void k_propose(const float *__restrict__ d, in
     atomicAdd(n_props, 1);
 }
 
-// Bitonic sort of kSortN (value, index) pairs in LDS, order `before`.
-__device__ void lds_sort(float *v, int *a, int tid, int nthreads) {
-    for (int k = 2; k <= kSortN; k <<= 1) {
+// Smallest power of two >= x (x >= 1).
+__device__ __forceinline__ int pow2_at_least(int x) {
+    int n = 1;
+    while (n < x) n <<= 1;
+    return n;
+}
+
+// Bitonic sort of the first n (a power of two <= kSortN) (value, index) pairs in
+// LDS, order `before`.
+__device__ void lds_sort(float *v, int *a, int n, int tid, int nthreads) {
+    for (int k = 2; k <= n; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = tid; i < kSortN; i += nthreads) {
+            for (int i = tid; i < n; i += nthreads) {
                 const int l = i ^ j;
                 if (l > i) {
                     const bool up = (i & k) == 0;
@@ -132,16 +140,17 @@ __global__ __launch_bounds__(256) void k_accept(const float *__restrict__ d, int
     const int chunk = kSortN - cap;
     for (int q0 = 0; q0 < m; q0 += chunk) {
         const int q1 = min(m, q0 + chunk);
-        for (int i = cap + tid; i < kSortN; i += 256) {
+        const int n = pow2_at_least(cap + (q1 - q0));   // <= kSortN
+        for (int i = cap + tid; i < n; i += 256) {
             const int q = q0 + (i - cap);
             const int p = q < q1 ? props[(size_t)c * P + q] : -1;
             sv[i] = p >= 0 ? score(d, pitch, c, p) : -INFINITY;
             sa[i] = p >= 0 ? p : INT_MAX;
         }
         __syncthreads();
-        lds_sort(sv, sa, tid, 256);
+        lds_sort(sv, sa, n, tid, 256);
         // valid entries now lead; those past cap are released
-        for (int i = cap + tid; i < kSortN; i += 256)
+        for (int i = cap + tid; i < n; i += 256)
             if (sa[i] != INT_MAX) status[sa[i]] = 0;
         h = min(cap, h + (q1 - q0));
         __syncthreads();
@@ -153,32 +162,36 @@ __global__ __launch_bounds__(256) void k_accept(const float *__restrict__ d, int
     }
 }
 
-// Output: each class's holders ascending; then the unplaced particles
-// ascending and the class of the highest remaining pair.  One workgroup.
-__global__ __launch_bounds__(256) void k_assign_output(const float *__restrict__ d, int pitch, int N,
-                                                       int P, int cap, const int *__restrict__ holders,
-                                                       const int *__restrict__ hold_cnt,
-                                                       const int *__restrict__ status,
-                                                       int *__restrict__ out) {
+// Output, part 1: block c writes class c's holders ascending at out[c*cap ...].
+__global__ __launch_bounds__(256) void k_class_out(int cap, const int *__restrict__ holders,
+                                                   const int *__restrict__ hold_cnt,
+                                                   int *__restrict__ out) {
     __shared__ float sv[kSortN];
     __shared__ int sa[kSortN];
+    const int c = blockIdx.x, tid = threadIdx.x;
+    const int h = hold_cnt[c];
+    const int n = pow2_at_least(cap);   // cap <= kMaxCap <= kSortN
+    for (int i = tid; i < n; i += 256) {
+        // sort by particle ascending: value = -particle, index = particle
+        const int p = i < h ? holders[(size_t)c * cap + i] : INT_MAX;
+        sv[i] = p == INT_MAX ? -INFINITY : -(float)p;
+        sa[i] = p;
+    }
+    __syncthreads();
+    lds_sort(sv, sa, n, tid, 256);
+    for (int i = tid; i < cap; i += 256) out[(size_t)c * cap + i] = i < h ? sa[i] : -1;
+}
+
+// Output, part 2 (one workgroup): the unplaced particles ascending after the
+// classes, then the class of the highest remaining pair.
+__global__ __launch_bounds__(256) void k_assign_output(const float *__restrict__ d, int pitch, int N,
+                                                       int P, int cap, const int *__restrict__ status,
+                                                       int *__restrict__ out) {
+    __shared__ int sa[256];
     __shared__ int base;
     __shared__ float rv[256];
     __shared__ int rn[256], rp[256];
     const int tid = threadIdx.x;
-    for (int c = 0; c < N; ++c) {
-        const int h = hold_cnt[c];
-        for (int i = tid; i < kSortN; i += 256) {
-            // sort by particle ascending: value = -particle, index = particle
-            const int p = i < h ? holders[(size_t)c * cap + i] : INT_MAX;
-            sv[i] = p == INT_MAX ? -INFINITY : -(float)p;
-            sa[i] = p;
-        }
-        __syncthreads();
-        lds_sort(sv, sa, tid, 256);
-        for (int i = tid; i < cap; i += 256) out[(size_t)c * cap + i] = i < h ? sa[i] : -1;
-        __syncthreads();
-    }
     // unplaced particles, ascending (ordered block scan over chunks of 256)
     if (tid == 0) base = N * cap;
     __syncthreads();
@@ -307,8 +320,10 @@ int assign_groups(const float *d_d, int N, int P, int pitch, int *d_out, hipStre
         hipLaunchKernelGGL(k_accept, dim3(N), dim3(256), 0, st, d_d, pitch, P, cap, cnt.as<int>(),
                            props.as<int>(), holders.as<int>(), hold_cnt.as<int>(), status.as<int>());
     }
+    hipLaunchKernelGGL(k_class_out, dim3(N), dim3(256), 0, st, cap, holders.as<int>(),
+                       hold_cnt.as<int>(), d_out);
     hipLaunchKernelGGL(k_assign_output, dim3(1), dim3(256), 0, st, d_d, pitch, N, P, cap,
-                       holders.as<int>(), hold_cnt.as<int>(), status.as<int>(), d_out);
+                       status.as<int>(), d_out);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
         return err(1, "assign_groups: launch failed");
     return 0;
