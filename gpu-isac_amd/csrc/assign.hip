@@ -305,20 +305,26 @@ int assign_groups(const float *d_d, int N, int P, int pitch, int *d_out, hipStre
     hipLaunchKernelGGL(k_assign_init, dim3((mx + 255) / 256), dim3(256), 0, st, status.as<int>(),
                        last_n.as<int>(), last_v.as<float>(), P, hold_cnt.as<int>(), cnt.as<int>(), N);
     // every round makes at least one proposal until none is left; a particle
-    // proposes to each class at most once, so N * P + 1 rounds bound the loop
+    // proposes to each class at most once, so N * P + 1 rounds bound the loop.
+    // A round without proposals changes nothing (no particle is free), so rounds
+    // are queued kRoundsPerCheck at a time and the last one's count is read back.
+    constexpr int kRoundsPerCheck = 8;
     const long long max_rounds = (long long)N * P + 1;
-    for (long long round = 0; round < max_rounds; ++round) {
-        hipLaunchKernelGGL(k_zero1, dim3(1), dim3(64), 0, st, n_props.as<int>());
-        hipLaunchKernelGGL(k_propose, dim3((P + 3) / 4), dim3(256), 0, st, d_d, pitch, N, P,
-                           status.as<int>(), last_n.as<int>(), last_v.as<float>(), cnt.as<int>(),
-                           props.as<int>(), n_props.as<int>());
+    for (long long round = 0; round < max_rounds; round += kRoundsPerCheck) {
+        for (int r = 0; r < kRoundsPerCheck; ++r) {
+            hipLaunchKernelGGL(k_zero1, dim3(1), dim3(64), 0, st, n_props.as<int>());
+            hipLaunchKernelGGL(k_propose, dim3((P + 3) / 4), dim3(256), 0, st, d_d, pitch, N, P,
+                               status.as<int>(), last_n.as<int>(), last_v.as<float>(),
+                               cnt.as<int>(), props.as<int>(), n_props.as<int>());
+            hipLaunchKernelGGL(k_accept, dim3(N), dim3(256), 0, st, d_d, pitch, P, cap,
+                               cnt.as<int>(), props.as<int>(), holders.as<int>(),
+                               hold_cnt.as<int>(), status.as<int>());
+        }
         int made = 0;
         if (hipMemcpyAsync(&made, n_props.p, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess)
             return err(1, "assign_groups: proposal round failed");
         if (made == 0) break;
-        hipLaunchKernelGGL(k_accept, dim3(N), dim3(256), 0, st, d_d, pitch, P, cap, cnt.as<int>(),
-                           props.as<int>(), holders.as<int>(), hold_cnt.as<int>(), status.as<int>());
     }
     hipLaunchKernelGGL(k_class_out, dim3(N), dim3(256), 0, st, cap, holders.as<int>(),
                        hold_cnt.as<int>(), d_out);
