@@ -1,6 +1,6 @@
 """Benchmark of the multi-reference polar alignment hot path on one MI355X.
 
-Workload (default "cfg2"): 20,000 float32 particles of 76x76 px rendered from
+Workload (default "p20k_n100_d76"): 20,000 float32 particles of 76x76 px rendered from
 100 seeded blob references (radius <= 34 px) with random psi, mirror and
 sub-pixel shifts in +-3 px at SNR 0.05; search over rings 1..35, shifts +-3
 step 1 (49 shifts), 256 psi samples and both mirrors.  A step is one
@@ -34,8 +34,8 @@ for _p in (os.path.join(ROOT, "gpu-isac_amd"), ROOT):
 
 WORKLOADS = {
     # name: (P, N, D, radius, snr, max_shift, integer_shifts, R, shift_rng)
-    "cfg2": (20000, 100, 76, 34.0, 0.05, 3.0, False, 35, 3.0),
-    "cfg1": (1000, 8, 64, 28.0, 0.1, 2.0, True, 29, 2.0),
+    "p20k_n100_d76": (20000, 100, 76, 34.0, 0.05, 3.0, False, 35, 3.0),
+    "p1k_n8_d64": (1000, 8, 64, 28.0, 0.1, 2.0, True, 29, 2.0),
 }
 PEAK_FP32_TFLOPS = 157.3     # MI355X published FP32 peak
 RING_LEN = 256
@@ -119,14 +119,14 @@ def host_abi_rate(sbj_h, ref_h, R, rng, runs=5):
 
 
 def precision_report(n_particles=1000):
-    """Config 3: config-1 inputs (1000 x 64^2, 8 references, rings 1..29, +-2 shifts)
+    """Precision report on the small workload (1000 x 64^2, 8 references, rings 1..29, +-2 shifts)
     with the ring contraction in FP16 and in BF16 on MFMA, the top-8 candidates
     re-scored in FP32 and near-ties in FP64, against the FP64 oracle."""
     import numpy as np
     import torch
     from isac_amd import binding, synth
     from oracle import coracle
-    ds = synth.config1(n_particles=n_particles)
+    ds = synth.dataset_p1k_n8_d64(n_particles=n_particles)
     P, N = ds.sbj.shape[0], ds.ref.shape[0]
     s = torch.from_numpy(ds.sbj).cuda()
     r = torch.from_numpy(ds.ref).cuda()
@@ -139,7 +139,7 @@ def precision_report(n_particles=1000):
     ob = order[0]
     vb, v2 = ref["d"][ob, cols], ref["d"][order[1], cols]
     otie = (vb - v2) <= 1e-5 * np.abs(vb)
-    out = {"report": "precision (config 3)", "particles": P, "refs": N,
+    out = {"report": "precision (FP16/BF16 contraction, 1000 particles x 8 refs, 64 px)", "particles": P, "refs": N,
            "oracle": f"FP64 C restatement, {t_oracle:.1f} s", "oracle_best_ref_ties": int(otie.sum())}
     for prec in ("fp16", "bf16"):
         o = binding.mref_best_mixed_device(s, r, 29, 1.0, 2.0, 2.0, prec)
@@ -178,7 +178,7 @@ def traffic_per_launch(pairs_per_launch):
     """HBM bytes per k_ccf_mref launch from the committed rocprofv3 --pmc summary
     (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the factor a 1 GiB copy
     gives), as bytes per pair times the pairs of this launch; None without it."""
-    path = os.path.join(ROOT, "profiles", "k_ccf_mref_traffic_cfg2.json")
+    path = os.path.join(ROOT, "profiles", "k_ccf_mref_traffic_p20k_n100_d76.json")
     try:
         with open(path) as f:
             t = json.load(f)
@@ -194,12 +194,12 @@ def batched_report(steps=3, warmup=1, n_subsets=64):
     import numpy as np
     import torch
     from isac_amd import binding, synth
-    shapes = synth.config4_shapes(n_subsets)
+    shapes = synth.uneven_subset_shapes(n_subsets)
     subs = synth.make_subsets_torch(shapes)
     torch.cuda.synchronize()
     pairs = sum(sh["P"] * sh["N"] for sh in shapes)
     bm = binding.BatchedMref(subs)
-    out = {"report": "batched uneven subsets (config 4)", "subsets": n_subsets,
+    out = {"report": "batched uneven subsets (64 subsets)", "subsets": n_subsets,
            "particles": sum(sh["P"] for sh in shapes), "pairs": pairs,
            "boxes": {str(b): sum(1 for sh in shapes if sh["D"] == b) for b in synth.BATCHED_BOXES}}
     t0 = time.perf_counter()
@@ -240,7 +240,7 @@ def iteration_report(steps=5, warmup=1, fl=0.25, ff=0.2):
     import numpy as np
     import torch
     from isac_amd import binding, synth
-    P, N, D, radius, snr, max_shift, int_shifts, R, rng = WORKLOADS["cfg2"]
+    P, N, D, radius, snr, max_shift, int_shifts, R, rng = WORKLOADS["p20k_n100_d76"]
     sbj, ref, _ = synth.make_dataset_torch(P, N, D, radius, snr, max_shift, int_shifts, "cuda")
     torch.cuda.synchronize()
 
@@ -260,7 +260,7 @@ def iteration_report(steps=5, warmup=1, fl=0.25, ff=0.2):
     t_update, (refs, params, members, avg) = timed(
         lambda: binding.update_references_device(sbj, peak, cls, N, fl, ff))
     m = members.cpu().numpy()
-    return {"report": "one multi-reference generation on the device (cfg2 sizes)",
+    return {"report": "one multi-reference generation on the device (20 000 particles x 100 refs, 76 px)",
             "particles": P, "refs": N, "img_dim": D, "steps": steps, "warmup": warmup,
             "search_s": t_search, "assign_s": t_assign, "update_s": t_update,
             "generation_s": t_search + t_assign + t_update,
@@ -271,7 +271,7 @@ def iteration_report(steps=5, warmup=1, fl=0.25, ff=0.2):
 def learned_report(n_train=200000, n_cal=10000, n_test=100000, epochs=16, oracle_sample=1000,
                    target=0.997, snr=0.1, integer_shifts=True, log=None):
     """Config 5: the pose-proposal network trained on exhaustive-search labels of
-    200,000 config-1-style particles (blob references, uniform psi and mirror,
+    200,000 particles drawn like the small workload's (blob references, uniform psi and mirror,
     integer shifts, SNR 0.1) at 76 px, 100 references, +-3 px (49 shifts);
     threshold calibrated on 10,000 more; evaluated on 100,000 held-out
     particles: agreement with the exhaustive FP32 search and with the FP64
@@ -289,7 +289,7 @@ def learned_report(n_train=200000, n_cal=10000, n_test=100000, epochs=16, oracle
                                            seed_particles=seed, seed_noise=seed + 1)
         return s, r
 
-    out = {"report": "learned proposal (config 5)", "refs": N, "box": D, "shifts": 49,
+    out = {"report": "learned proposal (100 000 held-out particles, 76 px)", "refs": N, "box": D, "shifts": 49,
            "snr": snr, "integer_shifts": integer_shifts, "train": n_train,
            "calibration": n_cal, "held_out": n_test, "epochs": epochs}
     tr, ref = make(n_train, 11)
@@ -375,7 +375,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="p20k_n100_d76", choices=sorted(WORKLOADS))
     ap.add_argument("--particles", type=int, default=None, help="override particle count")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -386,9 +386,9 @@ def main():
                     help="one step of the workload plus a 1 GiB device copy, nothing printed: the "
                          "target of rocprofv3 --pmc runs (the copy calibrates the byte counters)")
     ap.add_argument("--report", choices=["precision", "batched", "learned", "iteration"], default=None,
-                    help="precision: config-3 accuracy report; batched: config-4 uneven-subset "
-                         "throughput, graph vs one launch per subset; learned: config-5 proposal "
-                         "search; iteration: search + assignment + reference update at cfg2 sizes")
+                    help="precision: FP16/BF16 accuracy report; batched: uneven-subset "
+                         "throughput, graph vs one launch per subset; learned: proposal "
+                         "search; iteration: search + assignment + reference update, 20 000 x 100")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "

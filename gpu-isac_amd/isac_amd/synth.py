@@ -143,12 +143,12 @@ def make_dataset(n_particles: int, n_refs: int, D: int, radius: float, snr,
     return Dataset(sbj, ref, ref_idx, phi, shift, mirror)
 
 
-def config1(n_particles: int = 1000, snr=0.1) -> Dataset:
+def dataset_p1k_n8_d64(n_particles: int = 1000, snr=0.1) -> Dataset:
     """1,000 x 64^2 particles, 8 refs (r <= 28), int shifts +-2, SNR 0.1."""
     return make_dataset(n_particles, 8, 64, 28.0, snr, 2, True)
 
 
-def config2(n_particles: int = 20000, snr=0.05, n_refs: int = 100) -> Dataset:
+def dataset_p20k_n100_d76(n_particles: int = 20000, snr=0.05, n_refs: int = 100) -> Dataset:
     """20,000 x 76^2 particles, 100 refs (r <= 34), sub-pixel shifts +-3, SNR 0.05."""
     return make_dataset(n_particles, n_refs, 76, 34.0, snr, 3.0, False)
 
@@ -220,7 +220,7 @@ def make_dataset_torch(n_particles: int, n_refs: int, D: int, radius: float, snr
 BATCHED_BOXES = (64, 76, 96, 128)
 
 
-def config4_shapes(n_subsets: int = 64, seed: int = 4):
+def uneven_subset_shapes(n_subsets: int = 64, seed: int = 4):
     """Shapes of the batched uneven workload: per subset a particle count
     uniform in 500..5000, a box from {64, 76, 96, 128}, a reference count
     uniform in 20..200, a shift range of +-2..+-4 (step 1), rings 1..box/2-4 and

@@ -22,9 +22,9 @@ def _stats(peak, d, ref):
 
 
 @pytest.mark.parametrize("precision,min_agree,max_rel", [("fp16", 0.97, 2e-3), ("bf16", 0.85, 2e-2)])
-def test_mixed_vs_oracle_cfg1(gpu, precision, min_agree, max_rel):
+def test_mixed_vs_oracle_p1k_n8_d64(gpu, precision, min_agree, max_rel):
     import torch
-    ds = synth.config1(n_particles=128)
+    ds = synth.dataset_p1k_n8_d64(n_particles=128)
     acc = np.random.default_rng(1).uniform(-1, 1, (128, 2)).astype(np.float32)
     s = torch.from_numpy(ds.sbj).cuda()
     r = torch.from_numpy(ds.ref).cuda()
@@ -33,19 +33,19 @@ def test_mixed_vs_oracle_cfg1(gpu, precision, min_agree, max_rel):
     torch.cuda.synchronize()
     ref = coracle.mref(ds.sbj, ds.ref, acc, 29, 2.0, 2.0, 1.0)
     st = _stats(peak.cpu().numpy(), d.cpu().numpy(), ref)
-    print(precision, "cfg1", st)
+    print(precision, "1000 x 8, 64 px", st)
     assert st["agree"] >= min_agree and st["d_rel_max"] <= max_rel
 
 
 @pytest.mark.parametrize("precision", ["fp16", "bf16"])
-def test_mixed_vs_oracle_cfg2(gpu, precision):
+def test_mixed_vs_oracle_p20k_n100_d76(gpu, precision):
     import torch
-    ds = synth.config2(n_particles=48, n_refs=100)
+    ds = synth.dataset_p20k_n100_d76(n_particles=48, n_refs=100)
     s = torch.from_numpy(ds.sbj).cuda()
     r = torch.from_numpy(ds.ref).cuda()
     peak, d = binding.mref_align_mixed_device(s, r, 35, 1.0, 3.0, 3.0, precision)
     torch.cuda.synchronize()
     ref = coracle.mref(ds.sbj, ds.ref, np.zeros((48, 2), np.float32), 35, 3.0, 3.0, 1.0)
     st = _stats(peak.cpu().numpy(), d.cpu().numpy(), ref)
-    print(precision, "cfg2", st)
+    print(precision, "100 refs, 76 px", st)
     assert st["d_rel_max"] <= (5e-3 if precision == "fp16" else 5e-2)

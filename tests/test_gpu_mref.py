@@ -17,16 +17,16 @@ def _run(ds, R, rng, step, acc, label, **kw):
     return peak, d, ref
 
 
-def test_config1_subset(gpu):
-    ds = synth.config1(n_particles=96)
+def test_small_workload_subset(gpu):
+    ds = synth.dataset_p1k_n8_d64(n_particles=96)
     acc = np.zeros((96, 2), np.float32)
-    _run(ds, 29, 2.0, 1.0, acc, "cfg1-96")
+    _run(ds, 29, 2.0, 1.0, acc, "64 px, 8 refs, 96 particles")
 
 
-def test_config2_subset(gpu):
-    ds = synth.config2(n_particles=64, n_refs=100)
+def test_headline_workload_subset(gpu):
+    ds = synth.dataset_p20k_n100_d76(n_particles=64, n_refs=100)
     acc = np.random.default_rng(11).uniform(-1.5, 1.5, (64, 2)).astype(np.float32)
-    _run(ds, 35, 3.0, 1.0, acc, "cfg2-64x100")
+    _run(ds, 35, 3.0, 1.0, acc, "76 px, 100 refs, 64 particles")
 
 
 @pytest.mark.parametrize("D,R,nref,rng", [(32, 13, 5, 1.0), (48, 20, 7, 4.0), (96, 44, 3, 2.0),
@@ -48,7 +48,7 @@ def test_single_particle_and_pitch(gpu):
 
 def test_device_entry_matches_abi(gpu):
     import torch
-    ds = synth.config1(n_particles=40)
+    ds = synth.dataset_p1k_n8_d64(n_particles=40)
     acc = np.random.default_rng(3).uniform(-1, 1, (40, 2)).astype(np.float32)
     peak_h, d_h = binding.multi_ref_align(ds.sbj, ds.ref, 29, 1.0, 2.0, 2.0, acc_xy=acc)
     s = torch.from_numpy(ds.sbj).cuda()

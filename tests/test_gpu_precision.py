@@ -21,10 +21,10 @@ def _oracle_best(ref):
 
 
 @pytest.mark.parametrize("precision", ["fp16", "bf16"])
-def test_best_reference_cfg2(gpu, precision):
+def test_best_reference_headline_workload(gpu, precision):
     import torch
     P, N = 160, 100
-    ds = synth.config2(n_particles=P, n_refs=N)
+    ds = synth.dataset_p20k_n100_d76(n_particles=P, n_refs=N)
     s = torch.from_numpy(ds.sbj).cuda()
     r = torch.from_numpy(ds.ref).cuda()
     out = binding.mref_best_mixed_device(s, r, 35, 1.0, 3.0, 3.0, precision)

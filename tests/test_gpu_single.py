@@ -105,9 +105,9 @@ def test_size_checks(gpu):
     cfg = binding.make_config(1000, 1, 76, 35, 1.0, 4.0, 4.0, "pre_aln")
     assert lib.pre_align_size_check(1000, cfg, 0, 0.9, False) == 1
     assert lib.pre_align_size_check(1000, cfg, 0, 1e-9, False) == 0
-    cfg2 = binding.make_config(5000, 100, 76, 35, 1.0, 1.0, 1.0, "multigroup_aln")
-    assert lib.ref_free_alignment_2D_size_check(cfg2, 0, 0.9, False) == 1
-    assert lib.ref_free_alignment_2D_size_check(cfg2, 0, 1e-9, False) == 0
+    big = binding.make_config(5000, 100, 76, 35, 1.0, 1.0, 1.0, "multigroup_aln")
+    assert lib.ref_free_alignment_2D_size_check(big, 0, 0.9, False) == 1
+    assert lib.ref_free_alignment_2D_size_check(big, 0, 1e-9, False) == 0
 
 
 def test_reset_shifts_keeps_count(gpu):
