@@ -331,6 +331,167 @@ __global__ __launch_bounds__(256) void k_conv_mfma(const unsigned short *__restr
     }
 }
 
+// ------------------------------------------------------------ banded pooled convolution
+// Inference form of k_conv_mfma<3, false, true> for Cout in {64, 128, 256}:
+// workgroup = (image, band of P pooled rows).  The band's input rows
+// 2 r0 - 1 .. 2 r1 and columns -1 .. 2 Wo are staged in LDS once (zeros outside
+// the image; pixel stride Cin + 8 and row stride RS elements, RS = 64 mod 128,
+// which keeps the B reads of a window group nearly free of bank conflicts), so
+// the nine taps read it from LDS instead of global memory.  Wavefront w takes
+// co block w / (4 / ncb) and every (4 / ncb)-th group of 4 windows, 4 groups at a
+// time (64 output channels x 64 pixels, 16 accumulator tiles).  The k-steps,
+// their order and each tile's MFMA chain are those of k_conv_mfma, and so is
+// the epilogue: the pooled map is bit-identical to it.
+constexpr int kBandLdsMax = 40 * 1024;
+
+// G groups of 4 windows (16 pixels each) of one wavefront x 64 output channels.
+// G is a template parameter so every MFMA of the chain is unconditional and the
+// accumulators stay in their registers across the k-loop.
+template <int G>
+__device__ __forceinline__ void band_chunk(const unsigned short *__restrict__ band,
+                                           const unsigned short *const (&wrow)[4],
+                                           const float *__restrict__ bias, int bias_stride, int Cin,
+                                           int Cout, int RS, int Cs, int Wo, int nwin, int cb0,
+                                           int gfirst, int gstride, size_t win0,
+                                           unsigned short *__restrict__ y) {
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = (lane >> 4) * 4, sub = li & 3;
+    bool pval[G];
+    int pix[G];
+#pragma unroll
+    for (int gi = 0; gi < G; ++gi) {
+        const int win = 4 * (gfirst + gi * gstride) + (li >> 2);
+        pval[gi] = win < nwin;
+        const int wc = pval[gi] ? win : 0;
+        const int pr = wc / Wo, pcl = wc - pr * Wo;
+        pix[gi] = (2 * pr + (sub >> 1) + 1) * RS + (2 * pcl + (sub & 1) + 1) * Cs + lk;
+    }
+    float4_t acc[4][G];
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+        for (int gi = 0; gi < G; ++gi) acc[ct][gi] = float4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    // k-step s = (tap t, channels c .. c + 15) reads weights at k = 16 s; the
+    // weights of step s + 2 load while step s runs (indices clamped at the end,
+    // so the loop body has no branch and the accumulators keep their registers)
+    const int NK = 9 * (Cin / 16);
+    uint2 a0[4], a1[4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+        a0[ct] = *reinterpret_cast<const uint2 *>(wrow[ct]);
+        a1[ct] = *reinterpret_cast<const uint2 *>(wrow[ct] + 16 * min(1, NK - 1));
+    }
+    int t = 0, c = 0;
+    for (int ks = 0; ks < NK; ++ks) {
+        uint2 a2[4];
+        const int sn = min(ks + 2, NK - 1);
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) a2[ct] = *reinterpret_cast<const uint2 *>(wrow[ct] + 16 * sn);
+        const int toff = (t / 3 - 1) * RS + (t % 3 - 1) * Cs + c;
+        uint2 b[G];
+#pragma unroll
+        for (int gi = 0; gi < G; ++gi) b[gi] = *reinterpret_cast<const uint2 *>(band + pix[gi] + toff);
+#pragma unroll
+        for (int gi = 0; gi < G; ++gi)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) acc[ct][gi] = mfma_bf16(a0[ct], b[gi], acc[ct][gi]);
+        c += 16;
+        if (c == Cin) {
+            c = 0;
+            ++t;
+        }
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) {
+            a0[ct] = a1[ct];
+            a1[ct] = a2[ct];
+        }
+    }
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+        const int co = cb0 + ct * 16 + lk;
+        float bb[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (bias) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) bb[v] = bias[(size_t)(co + v) * bias_stride];
+        }
+#pragma unroll
+        for (int gi = 0; gi < G; ++gi) {
+            float o[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                // window = lanes l..l|3; first maximum in window order (sub 0..3)
+                const float m = bf2f(f2bf(fmaxf(acc[ct][gi][v] + bb[v], 0.0f)));
+                const float m1 = __shfl_xor(m, 1, kWave);
+                const float m2 = __shfl_xor(m, 2, kWave);
+                const float m3 = __shfl_xor(m, 3, kWave);
+                float r = m;
+                if (m1 > r) r = m1;
+                if (m2 > r) r = m2;
+                if (m3 > r) r = m3;
+                o[v] = r;
+            }
+            if (!pval[gi] || sub) continue;
+            const int win = 4 * (gfirst + gi * gstride) + (li >> 2);
+            *reinterpret_cast<uint2 *>(y + (win0 + win) * Cout + co) = pack4(o[0], o[1], o[2], o[3]);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_conv_pool_band(const unsigned short *__restrict__ x, int H,
+                                                        int W, int Cin,
+                                                        const unsigned short *__restrict__ w,
+                                                        const float *__restrict__ bias,
+                                                        int bias_stride, int Cout, int P, int nband,
+                                                        int RS, unsigned short *__restrict__ y) {
+    extern __shared__ __align__(16) unsigned short band[];
+    const int Ho = H / 2, Wo = W / 2, Wl = 2 * Wo + 2, Cs = Cin + 8, C8 = Cin / 8;
+    const int img = blockIdx.x / nband, bnd = blockIdx.x - img * nband;
+    const int r0 = bnd * P, r1 = min(Ho, r0 + P), nrow = 2 * (r1 - r0) + 2;
+    const int tid = threadIdx.x;
+    const unsigned short *src = x + (size_t)img * H * W * Cin;
+    for (int i = tid; i < nrow * Wl * C8; i += 256) {
+        const int c8 = i % C8, pc = i / C8;
+        const int row = pc / Wl, col = pc - row * Wl;
+        const int sy = 2 * r0 - 1 + row, sx = col - 1;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (sy >= 0 && sy < H && sx >= 0 && sx < W)
+            v = *reinterpret_cast<const uint4 *>(src + ((size_t)sy * W + sx) * Cin + c8 * 8);
+        *reinterpret_cast<uint4 *>(band + row * RS + col * Cs + c8 * 8) = v;
+    }
+    __syncthreads();
+
+    const int wave = tid >> 6, li = tid & 15;
+    const int nwin = (r1 - r0) * Wo, ngrp = (nwin + 3) / 4;
+    const int wpc = 4 / (Cout / 64);   // wavefronts per co block
+    const int cb0 = (wave / wpc) * 64, h = wave % wpc;
+    const int K = 9 * Cin;
+    const unsigned short *wrow[4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) wrow[ct] = w + (size_t)(cb0 + ct * 16 + li) * K + (tid & 63) / 16 * 4;
+    const size_t win0 = ((size_t)img * Ho + r0) * Wo;
+    // this wavefront's groups: h, h + wpc, h + 2 wpc, ... taken 4 at a time
+    const int mine = h < ngrp ? (ngrp - h + wpc - 1) / wpc : 0;
+    for (int c = 0; c < mine; c += 4) {
+        const int gfirst = h + c * wpc;
+        switch (min(4, mine - c)) {
+        case 4:
+            band_chunk<4>(band, wrow, bias, bias_stride, Cin, Cout, RS, Cs, Wo, nwin, cb0, gfirst, wpc,
+                          win0, y);
+            break;
+        case 3:
+            band_chunk<3>(band, wrow, bias, bias_stride, Cin, Cout, RS, Cs, Wo, nwin, cb0, gfirst, wpc,
+                          win0, y);
+            break;
+        case 2:
+            band_chunk<2>(band, wrow, bias, bias_stride, Cin, Cout, RS, Cs, Wo, nwin, cb0, gfirst, wpc,
+                          win0, y);
+            break;
+        default:
+            band_chunk<1>(band, wrow, bias, bias_stride, Cin, Cout, RS, Cs, Wo, nwin, cb0, gfirst, wpc,
+                          win0, y);
+        }
+    }
+}
+
 // ------------------------------------------------------------ pooling
 // One thread per (output pixel, 8 channels).
 __global__ void k_maxpool2(const unsigned short *__restrict__ y, int n, int H, int W, int C,
@@ -839,6 +1000,19 @@ int isac_nn_conv_pool_fwd(const uint16_t *x, int n, int H, int W, int Cin, const
         return fail(2, "isac_nn_conv_pool_fwd: bad arguments (Cin, Cout multiples of 16)");
     const size_t np = (size_t)n * (H / 2) * (W / 2) * 4;
     if (np == 0) return 0;
+    // banded form: largest band of pooled rows whose input rows fit kBandLdsMax,
+    // then the bands evened out over the pooled rows
+    const int Ho = H / 2, Cs = Cin + 8;
+    const int RS = (2 * (W / 2) + 2) * Cs + ((64 - (2 * (W / 2) + 2) * Cs % 128) + 128) % 128;
+    const int pmax = (kBandLdsMax / (RS * 2) - 2) / 2;
+    if ((Cout == 64 || Cout == 128 || Cout == 256) && pmax >= 1) {
+        const int nb0 = (Ho + pmax - 1) / pmax, P = (Ho + nb0 - 1) / nb0, nband = (Ho + P - 1) / P;
+        const size_t lds = (size_t)(2 * P + 2) * RS * 2;
+        hipLaunchKernelGGL(k_conv_pool_band, dim3((unsigned)((size_t)n * nband)), dim3(256), lds,
+                           (hipStream_t)stream, x, H, W, Cin, w, bias, bias_stride, Cout, P, nband, RS,
+                           a);
+        return launched("isac_nn_conv_pool_fwd");
+    }
     dim3 grid(blocks_for(np, 256), (unsigned)((Cout + 63) / 64));
     hipLaunchKernelGGL((k_conv_mfma<3, false, true>), grid, dim3(256), 0, (hipStream_t)stream, x, n, H, W,
                        Cin, w, bias, bias_stride, Cout, 1, a);

@@ -147,3 +147,33 @@ def test_fused_inference_forward_is_bit_identical(gpu, pool4):
     b = net.forward(img, train=True).clone()
     torch.cuda.synchronize()
     assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("H,cin,cout", [(38, 32, 64), (19, 64, 128), (76, 16, 64), (7, 32, 128),
+                                        (10, 16, 256), (9, 128, 128), (6, 16, 48)])
+def test_pooled_conv_is_bit_identical_to_conv_then_pool(gpu, H, cin, cout):
+    """The pooled convolution (banded LDS form for Cout 64 / 128 / 256, several
+    bands per image at 38 and 76 px, odd maps) equals the convolution followed
+    by the 2x2 max pool, bit for bit."""
+    import torch
+    from isac_amd import posenet
+    lib = posenet._lib()
+    rng = np.random.default_rng(H * 1000 + cout)
+    n = 5
+    x = N.bf16(rng.standard_normal((n, H, H, cin)))
+    w = N.bf16(rng.standard_normal((cout, 9 * cin)) * 0.1)
+    b = rng.standard_normal(cout).astype(np.float32)
+    xt = torch.from_numpy(x.astype(np.float32)).cuda().to(torch.bfloat16)
+    wt = torch.from_numpy(w.astype(np.float32)).cuda().to(torch.bfloat16)
+    bt = torch.from_numpy(b).cuda()
+    st = torch.cuda.current_stream().cuda_stream
+    y = torch.empty((n, H, H, cout), dtype=torch.bfloat16, device="cuda")
+    posenet._call(lib, "isac_nn_conv_fwd", xt.data_ptr(), n, H, H, cin, wt.data_ptr(), bt.data_ptr(),
+                  1, cout, 3, 1, 0, y.data_ptr(), st)
+    want = torch.empty((n, H // 2, H // 2, cout), dtype=torch.bfloat16, device="cuda")
+    posenet._call(lib, "isac_nn_maxpool2", y.data_ptr(), n, H, H, cout, want.data_ptr(), st)
+    got = torch.full((n, H // 2, H // 2, cout), float("nan"), dtype=torch.bfloat16, device="cuda")
+    posenet._call(lib, "isac_nn_conv_pool_fwd", xt.data_ptr(), n, H, H, cin, wt.data_ptr(),
+                  bt.data_ptr(), 1, cout, got.data_ptr(), st)
+    torch.cuda.synchronize()
+    assert torch.equal(got.view(torch.int16), want.view(torch.int16))
