@@ -492,6 +492,115 @@ __global__ __launch_bounds__(256) void k_conv_pool_band(const unsigned short *__
     }
 }
 
+// ------------------------------------------------------------ pooled convolution, resident weights
+// Inference form for (Cin 32, Cout 64/128/256) and (Cin 64, Cout 32/64/128):
+// persistent workgroups; each wavefront loads the weights of its CB blocks of 32
+// output channels for all NK = 9 Cin / 16 k-steps into registers once (CB NK 4
+// VGPRs = 144) and keeps them for every (image, band) item the workgroup takes.
+// Per item the input band is staged in LDS as in k_conv_pool_band, and each
+// wavefront runs 32 pixels (8 windows) at a time through the fully unrolled
+// k-steps on v_mfma_f32_32x32x16_bf16 (B operand = one 16-byte LDS read):
+//   A[i = co][k]   lane l: co = l%32, k = 16 s + 8(l/32) + e      -> w[co][16 s + ..]
+//   B[k][j = px]   lane l: px = l%32, k = 16 s + 8(l/32) + e      -> band[px + tap][c + ..]
+//   D[i = co][j]   lane l: co = 8(v/4) + 4(l/32) + v%4, px = l%32
+// One 32x32x16 step gives the bits of the 16x16x16 step over the same 16 k
+// (checked on the device for all four 16x16 quarters), and the k-steps run in
+// the order of k_conv_mfma, so the pooled map is bit-identical to it.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float float16_t __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ float16_t mfma32_bf16(uint4 a, uint4 b, float16_t c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, a),
+                                                   __builtin_bit_cast(bf16x8_t, b), c, 0, 0, 0);
+}
+
+constexpr int kResidentLdsMax = 64 * 1024;
+
+template <int CIN, int CB>
+__global__ __launch_bounds__(256, 2) void k_conv_pool_resident(
+    const unsigned short *__restrict__ x, int H, int W, const unsigned short *__restrict__ w,
+    const float *__restrict__ bias, int bias_stride, int Cout, int P, int nband, int nitems, int RS,
+    unsigned short *__restrict__ y) {
+    constexpr int NK = 9 * CIN / 16, CPT = CIN / 16, Cs = CIN + 8, C8 = CIN / 8, K = 9 * CIN;
+    extern __shared__ __align__(16) unsigned short band[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ncg = Cout / (32 * CB), wpg = 4 / ncg;   // co groups, wavefronts per co group
+    const int co0 = (wave / wpg) * 32 * CB, wi = wave % wpg;
+    uint4 wa[CB][NK];
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+        for (int s = 0; s < NK; ++s)
+            wa[cb][s] = *reinterpret_cast<const uint4 *>(
+                w + (size_t)(co0 + cb * 32 + (lane & 31)) * K + 16 * s + 8 * (lane >> 5));
+    const int Ho = H / 2, Wo = W / 2, Wl = 2 * Wo + 2;
+    const int j = lane & 31, sub = j & 3;
+    for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+        const int img = item / nband, bnd = item - img * nband;
+        const int r0 = bnd * P, r1 = min(Ho, r0 + P), nrow = 2 * (r1 - r0) + 2;
+        const unsigned short *src = x + (size_t)img * H * W * CIN;
+        __syncthreads();   // the previous item's band is no longer read
+        for (int i = tid; i < nrow * Wl * C8; i += 256) {
+            const int c8 = i % C8, pc = i / C8;
+            const int row = pc / Wl, col = pc - row * Wl;
+            const int sy = 2 * r0 - 1 + row, sx = col - 1;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (sy >= 0 && sy < H && sx >= 0 && sx < W)
+                v = *reinterpret_cast<const uint4 *>(src + ((size_t)sy * W + sx) * CIN + c8 * 8);
+            *reinterpret_cast<uint4 *>(band + row * RS + col * Cs + c8 * 8) = v;
+        }
+        __syncthreads();
+        const int nwin = (r1 - r0) * Wo, nblk = (nwin + 7) / 8;
+        const size_t win0 = ((size_t)img * Ho + r0) * Wo;
+        for (int b = wi; b < nblk; b += wpg) {
+            const int win = 8 * b + (j >> 2);
+            const bool pv = win < nwin;
+            const int wc = pv ? win : 0;
+            const int pr = wc / Wo, pcl = wc - pr * Wo;
+            // top-left tap of the lane's pixel (band row / column of input pixel - 1)
+            const unsigned short *pb =
+                band + (2 * pr + (sub >> 1)) * RS + (2 * pcl + (sub & 1)) * Cs + 8 * (lane >> 5);
+            float16_t acc[CB];
+#pragma unroll
+            for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) acc[cb][v] = 0.0f;
+#pragma unroll
+            for (int s = 0; s < NK; ++s) {
+                const int t = s / CPT, c = (s % CPT) * 16;
+                const uint4 bv = *reinterpret_cast<const uint4 *>(pb + (t / 3) * RS + (t % 3) * Cs + c);
+#pragma unroll
+                for (int cb = 0; cb < CB; ++cb) acc[cb] = mfma32_bf16(wa[cb][s], bv, acc[cb]);
+            }
+#pragma unroll
+            for (int cb = 0; cb < CB; ++cb) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int co = co0 + cb * 32 + 8 * q + 4 * (lane >> 5);
+                    float o[4];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const float bb = bias ? bias[(size_t)(co + e) * bias_stride] : 0.0f;
+                        // window = lanes l..l|3; first maximum in window order (sub 0..3)
+                        const float m = bf2f(f2bf(fmaxf(acc[cb][4 * q + e] + bb, 0.0f)));
+                        const float m1 = __shfl_xor(m, 1, kWave);
+                        const float m2 = __shfl_xor(m, 2, kWave);
+                        const float m3 = __shfl_xor(m, 3, kWave);
+                        float r = m;
+                        if (m1 > r) r = m1;
+                        if (m2 > r) r = m2;
+                        if (m3 > r) r = m3;
+                        o[e] = r;
+                    }
+                    if (pv && sub == 0)
+                        *reinterpret_cast<uint2 *>(y + (win0 + win) * Cout + co) =
+                            pack4(o[0], o[1], o[2], o[3]);
+                }
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------ pooling
 // One thread per (output pixel, 8 channels).
 __global__ void k_maxpool2(const unsigned short *__restrict__ y, int n, int H, int W, int C,
@@ -1004,6 +1113,49 @@ int isac_nn_conv_pool_fwd(const uint16_t *x, int n, int H, int W, int Cin, const
     // then the bands evened out over the pooled rows
     const int Ho = H / 2, Cs = Cin + 8;
     const int RS = (2 * (W / 2) + 2) * Cs + ((64 - (2 * (W / 2) + 2) * Cs % 128) + 128) % 128;
+    const int cb = 1;   // co blocks of 32 per wavefront in the resident form
+    const int ncg = Cout % (32 * cb) ? 0 : Cout / (32 * cb);
+    const int pres = (kResidentLdsMax / (RS * 2) - 2) / 2;
+    if ((Cin == 32 || Cin == 64) && (ncg == 1 || ncg == 2 || ncg == 4) && pres >= 1) {
+        // band height: fewest wavefront rounds of 8 windows over the image, plus
+        // half a round per band for its staging
+        const int wpg = 4 / ncg, Wo = W / 2;
+        int P = 1;
+        double best = 1e30;
+        for (int p = 1; p <= std::min(pres, Ho); ++p) {
+            double cost = 0.0;
+            for (int r0 = 0; r0 < Ho; r0 += p) {
+                const int nblk = (std::min(p, Ho - r0) * Wo + 7) / 8;
+                cost += (nblk + wpg - 1) / wpg + 0.5;
+            }
+            if (cost <= best) {
+                best = cost;
+                P = p;
+            }
+        }
+        const int nband = (Ho + P - 1) / P;
+        const size_t lds = (size_t)(2 * P + 2) * RS * 2;
+        const long long items = (long long)n * nband;
+        if (items > INT_MAX) return fail(2, "isac_nn_conv_pool_fwd: too many images");
+        const void *fn = Cin == 32 ? reinterpret_cast<const void *>(&k_conv_pool_resident<32, 1>)
+                                   : reinterpret_cast<const void *>(&k_conv_pool_resident<64, 1>);
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, lds) != hipSuccess ||
+            per_cu < 1)
+            return fail(3, "isac_nn_conv_pool_fwd: occupancy query failed");
+        const unsigned grid = (unsigned)std::min<long long>(items, (long long)cus * per_cu);
+        if (Cin == 32)
+            hipLaunchKernelGGL((k_conv_pool_resident<32, 1>), dim3(grid), dim3(256), lds,
+                               (hipStream_t)stream, x, H, W, w, bias, bias_stride, Cout, P, nband,
+                               (int)items, RS, a);
+        else
+            hipLaunchKernelGGL((k_conv_pool_resident<64, 1>), dim3(grid), dim3(256), lds,
+                               (hipStream_t)stream, x, H, W, w, bias, bias_stride, Cout, P, nband,
+                               (int)items, RS, a);
+        return launched("isac_nn_conv_pool_fwd");
+    }
     const int pmax = (kBandLdsMax / (RS * 2) - 2) / 2;
     if ((Cout == 64 || Cout == 128 || Cout == 256) && pmax >= 1) {
         const int nb0 = (Ho + pmax - 1) / pmax, P = (Ho + nb0 - 1) / nb0, nband = (Ho + P - 1) / P;

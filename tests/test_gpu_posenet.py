@@ -149,12 +149,14 @@ def test_fused_inference_forward_is_bit_identical(gpu, pool4):
     assert torch.equal(a, b)
 
 
-@pytest.mark.parametrize("H,cin,cout", [(38, 32, 64), (19, 64, 128), (76, 16, 64), (7, 32, 128),
+@pytest.mark.parametrize("H,cin,cout", [(38, 32, 64), (19, 64, 128), (7, 32, 128), (11, 64, 32),
+                                        (8, 32, 32), (13, 64, 64), (3, 64, 128), (76, 16, 64),
                                         (10, 16, 256), (9, 128, 128), (6, 16, 48)])
 def test_pooled_conv_is_bit_identical_to_conv_then_pool(gpu, H, cin, cout):
-    """The pooled convolution (banded LDS form for Cout 64 / 128 / 256, several
-    bands per image at 38 and 76 px, odd maps) equals the convolution followed
-    by the 2x2 max pool, bit for bit."""
+    """The pooled convolution equals the convolution followed by the 2x2 max
+    pool, bit for bit: the resident-weight form (Cin 32 / 64, Cout 32 / 64 /
+    128, one or several bands per image, one to four wavefronts per output
+    block), the banded form (Cin 16 / 128) and the unbanded kernel (Cout 48)."""
     import torch
     from isac_amd import posenet
     lib = posenet._lib()
