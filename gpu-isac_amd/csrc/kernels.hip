@@ -135,7 +135,7 @@ __device__ __forceinline__ void wave_argmax(float &v, int &t) {
 // sums (radius-weighted sum and sum of squares) accumulate in double.
 // MASKED: only the shifts whose bit is set in smask[p] (S <= 64) are computed;
 // the other rows of spec / norm are left as they were.
-template <bool MASKED>
+template <bool MASKED, int G = 4>
 __global__ __launch_bounds__(256) void k_polar_spectra(
     const float *__restrict__ imgs, int D, ShiftView acc, const int2 *__restrict__ shift_i,
     int S, const float2 *__restrict__ tmpl, int R, const float2 *__restrict__ twg,
@@ -167,68 +167,86 @@ __global__ __launch_bounds__(256) void k_polar_spectra(
         ax = acc.base[(size_t)p * acc.stride];
         ay = acc.base[(size_t)p * acc.stride + 1];
     }
-    for (int j = wave; j < n_rows; j += 4) {
-    const int s = MASKED ? slist[j] : j;
-    const int2 si = shift_i[s];
-    const float cx = (float)(D / 2 + si.x) + ax;
-    const float cy = (float)(D / 2 + si.y) + ay;
     float2 *b0 = fb + wave * 512;
     float2 *b1 = b0 + 256;
-    double sA = 0.0, sQ = 0.0;
-    float2 *out = spec + ((size_t)p * S + s) * (size_t)R * kSlots;
     const int pl = (64 - lane) & 63;
-    for (int i = 0; i < R; i += 2) {
-        const bool hb = (i + 1) < R;
-        const double wa = (double)(i + 1), wb = (double)(i + 2);
-        float2 v[4];
+    // rows j0, j0 + 4, ..., j0 + 4 (G - 1) of this wavefront share each ring
+    // pair's template samples (loaded once per group instead of once per row)
+    for (int j0 = wave; j0 < n_rows; j0 += 4 * G) {
+        int sg[G];
+        float cxg[G], cyg[G];
+        double sA[G], sQ[G];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int j = lane + 64 * q;
-            const float2 oa = tmpl[i * kRingLen + j];
-            const float va = bilinear(img, D, cx + oa.x, cy + oa.y);
-            float vb = 0.0f;
-            if (hb) {
-                const float2 ob = tmpl[(i + 1) * kRingLen + j];
-                vb = bilinear(img, D, cx + ob.x, cy + ob.y);
+        for (int g = 0; g < G; ++g) {
+            const int j = min(j0 + 4 * g, n_rows - 1);
+            sg[g] = MASKED ? slist[j] : j;
+            const int2 si = shift_i[sg[g]];
+            cxg[g] = (float)(D / 2 + si.x) + ax;
+            cyg[g] = (float)(D / 2 + si.y) + ay;
+            sA[g] = 0.0;
+            sQ[g] = 0.0;
+        }
+        for (int i = 0; i < R; i += 2) {
+            const bool hb = (i + 1) < R;
+            const double wa = (double)(i + 1), wb = (double)(i + 2);
+            float2 ta[4], tb[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                ta[q] = tmpl[i * kRingLen + lane + 64 * q];
+                tb[q] = hb ? tmpl[(i + 1) * kRingLen + lane + 64 * q] : f2(0.0f, 0.0f);
             }
-            v[q] = f2(va, vb);
-            sA += wa * (double)va + wb * (double)vb;
-            sQ += wa * (double)va * (double)va + wb * (double)vb * (double)vb;
-        }
-        fft256_wave<false>(v, b0, b1, tw, lane);
-        // v[q] = Z[lane + 64 q].  Partners Z[256-k]: k=lane -> lane pl, q=3;
-        // k=lane+64 -> lane pl, q=2 (lane 0: Z[0] itself and Z[192] = own q=3).
-        const float2 u2 = shfl2(v[2], pl), u3 = shfl2(v[3], pl);
-        const float2 zl = v[0], zlp = lane ? u3 : v[0];
-        const float2 zh = v[1], zhp = lane ? u2 : v[3];
-        float2 al = f2(0.5f * (zl.x + zlp.x), 0.5f * (zl.y - zlp.y));
-        float2 bl = f2(0.5f * (zl.y + zlp.y), 0.5f * (zlp.x - zl.x));
-        const float2 ah = f2(0.5f * (zh.x + zhp.x), 0.5f * (zh.y - zhp.y));
-        const float2 bh = f2(0.5f * (zh.y + zhp.y), 0.5f * (zhp.x - zh.x));
-        if (lane == 0) {  // pack Nyquist: A[128] = Re Z[128], B[128] = Im Z[128]
-            al = f2(zl.x, v[2].x);
-            bl = f2(zl.y, v[2].y);
-        }
-        float2 *oa = out + (size_t)i * kSlots;
-        oa[lane] = al;
-        oa[lane + 64] = ah;
-        if (hb) {
-            oa[kSlots + lane] = bl;
-            oa[kSlots + lane + 64] = bh;
-        }
-    }
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        sA += __shfl_xor(sA, off, kWave);
-        sQ += __shfl_xor(sQ, off, kWave);
-    }
-    if (lane == 0) {
-        const double nn = (double)kRingLen * 0.5 * (double)R * (double)(R + 1);
-        const double mu = sA / nn;
-        const double sd = sqrt(fabs(sQ / nn - mu * mu));
-        norm[(size_t)p * S + s] = f2((float)(mu * kRingLen), (float)(1.0 / (sd + 1e-7)));
-    }
-    }  // shifts
+            for (int g = 0; g < G; ++g) {
+                if (j0 + 4 * g >= n_rows) break;   // uniform over the wavefront
+                float2 v[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float va = bilinear(img, D, cxg[g] + ta[q].x, cyg[g] + ta[q].y);
+                    const float vb = hb ? bilinear(img, D, cxg[g] + tb[q].x, cyg[g] + tb[q].y) : 0.0f;
+                    v[q] = f2(va, vb);
+                    sA[g] += wa * (double)va + wb * (double)vb;
+                    sQ[g] += wa * (double)va * (double)va + wb * (double)vb * (double)vb;
+                }
+                fft256_wave<false>(v, b0, b1, tw, lane);
+                // v[q] = Z[lane + 64 q].  Partners Z[256-k]: k=lane -> lane pl, q=3;
+                // k=lane+64 -> lane pl, q=2 (lane 0: Z[0] itself and Z[192] = own q=3).
+                const float2 u2 = shfl2(v[2], pl), u3 = shfl2(v[3], pl);
+                const float2 zl = v[0], zlp = lane ? u3 : v[0];
+                const float2 zh = v[1], zhp = lane ? u2 : v[3];
+                float2 al = f2(0.5f * (zl.x + zlp.x), 0.5f * (zl.y - zlp.y));
+                float2 bl = f2(0.5f * (zl.y + zlp.y), 0.5f * (zlp.x - zl.x));
+                const float2 ah = f2(0.5f * (zh.x + zhp.x), 0.5f * (zh.y - zhp.y));
+                const float2 bh = f2(0.5f * (zh.y + zhp.y), 0.5f * (zhp.x - zh.x));
+                if (lane == 0) {  // pack Nyquist: A[128] = Re Z[128], B[128] = Im Z[128]
+                    al = f2(zl.x, v[2].x);
+                    bl = f2(zl.y, v[2].y);
+                }
+                float2 *oa = spec + (((size_t)p * S + sg[g]) * (size_t)R + i) * kSlots;
+                oa[lane] = al;
+                oa[lane + 64] = ah;
+                if (hb) {
+                    oa[kSlots + lane] = bl;
+                    oa[kSlots + lane + 64] = bh;
+                }
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            if (j0 + 4 * g >= n_rows) break;
+            double a2 = sA[g], q2 = sQ[g];
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                a2 += __shfl_xor(a2, off, kWave);
+                q2 += __shfl_xor(q2, off, kWave);
+            }
+            if (lane == 0) {
+                const double nn = (double)kRingLen * 0.5 * (double)R * (double)(R + 1);
+                const double mu = a2 / nn;
+                const double sd = sqrt(fabs(q2 / nn - mu * mu));
+                norm[(size_t)p * S + sg[g]] = f2((float)(mu * kRingLen), (float)(1.0 / (sd + 1e-7)));
+            }
+        }
+    }  // row groups
 }
 
 __global__ __launch_bounds__(128) void k_ref_finalize(float2 *__restrict__ rspec,
