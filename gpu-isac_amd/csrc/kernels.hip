@@ -442,6 +442,104 @@ __global__ __launch_bounds__(256) void k_ccf_items(
     write_rec(rec + item, bv, bf, nbuf[wave], lane);
 }
 
+// Candidate items four at a time: wavefront = (particle, 4 consecutive candidate
+// slots), walking the shifts four at a time (4 rows x 4 references, the register
+// tile of k_ccf_mref), so each spectrum row load serves four references.  Per
+// (row, reference) pair the contraction order and the psi stage are those of
+// k_ccf_items<false>, and each reference keeps its own running maximum, so every
+// record is bit-identical to it.  Slots with a negative reference get the empty
+// record.  K is a multiple of 4.
+__global__ __launch_bounds__(256) void k_ccf_cand4(
+    const float2 *__restrict__ spec, const float2 *__restrict__ norm,
+    const float2 *__restrict__ rspec, const float *__restrict__ rsum0, const int *__restrict__ cand,
+    int K, int n_img, int S, int R, int N, const float2 *__restrict__ twg, PeakRec *__restrict__ rec,
+    const int *__restrict__ pidx) {
+    __shared__ float2 tw[256];
+    __shared__ float2 fbuf[4][512];
+    __shared__ float zbuf[4][256];
+    __shared__ float nbuf[4][4][2][8];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    tw[tid] = twg[tid];
+    __syncthreads();
+    const int G = K / 4;
+    const int grp = blockIdx.x * 4 + wave;
+    if (grp >= n_img * G) return;
+    const int pi = grp / G, g = grp - pi * G;
+    const int p = pidx ? pidx[pi] : pi;   // spectra row of the particle
+    const int *cs = cand + (size_t)pi * K + 4 * g;
+    bool live[4];
+    const float2 *rb[4];
+    float rs0[4], bv[4][2];
+    int bf[4][2];
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int raw = cs[j];
+        live[j] = raw >= 0;
+        any = any || live[j];
+        // the host validates candidate ids; the clamp keeps every read in bounds
+        const int n = min(max(raw, 0), N - 1);
+        rb[j] = rspec + (size_t)n * R * kSlots;
+        rs0[j] = rsum0[n];
+        bv[j][0] = bv[j][1] = -3.40282e+38f;
+        bf[j][0] = bf[j][1] = INT_MAX;
+        if (!live[j] && lane < 8) nbuf[wave][j][0][lane] = 0.0f;
+    }
+    const size_t img_stride = (size_t)S * R * kSlots;
+    for (int s0 = 0; any && s0 < S; s0 += 4) {
+        float acc[4][4][2][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) acc[r][j][h][e] = 0.0f;
+        const float2 *arow[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            arow[r] = spec + (size_t)p * img_stride + (size_t)min(s0 + r, S - 1) * R * kSlots;
+        for (int i = 0; i < R; ++i) {
+            float2 a0[4], a1[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                a0[r] = arow[r][i * kSlots + lane];
+                a1[r] = arow[r][i * kSlots + lane + 64];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float2 b0v = rb[j][i * kSlots + lane], b1v = rb[j][i * kSlots + lane + 64];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    acc[r][j][0][0] += b0v.x * a0[r].x;
+                    acc[r][j][0][1] += b0v.y * a0[r].y;
+                    acc[r][j][0][2] += b0v.x * a0[r].y;
+                    acc[r][j][0][3] += b0v.y * a0[r].x;
+                    acc[r][j][1][0] += b1v.x * a1[r].x;
+                    acc[r][j][1][1] += b1v.y * a1[r].y;
+                    acc[r][j][1][2] += b1v.x * a1[r].y;
+                    acc[r][j][1][3] += b1v.y * a1[r].x;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int s = s0 + r;
+            if (s >= S) break;
+            const float2 nm = norm[(size_t)p * S + s];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (!live[j]) continue;   // uniform over the wavefront
+                psi_update<false>(acc[r][j][0], acc[r][j][1], nm, rs0[j], s, lane, fbuf[wave],
+                                  fbuf[wave] + 256, tw, zbuf[wave], nbuf[wave][j], bv[j], bf[j]);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) write_rec(rec + (size_t)pi * K + 4 * g + j, bv[j], bf[j], nbuf[wave][j], lane);
+}
+
 // Top-K references of every particle by approximate peak value (descending; ties
 // to the lower reference index).  One thread per particle.
 template <int K>
@@ -1666,6 +1764,12 @@ hipError_t launch_ccf_candidates(const float2 *spec, const float2 *norm, const f
                                  PeakRec *rec, hipStream_t st, const int *pidx) {
     const int items = n_img * K;
     if (items <= 0) return hipSuccess;
+    if (K % 4 == 0) {
+        const int groups = n_img * (K / 4);
+        hipLaunchKernelGGL(k_ccf_cand4, dim3((groups + 3) / 4), dim3(256), 0, st, spec, norm, rspec,
+                           rsum0, cand, K, n_img, n_shift, n_ring, n_ref, twiddle, rec, pidx);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_ccf_items<false>, dim3((items + 3) / 4), dim3(256), 0, st, spec, norm,
                        rspec, rsum0, cand, 1, K, items, n_shift, n_ring, n_ref, twiddle, rec, pidx);
     return hipGetLastError();
