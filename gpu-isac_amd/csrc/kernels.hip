@@ -958,20 +958,22 @@ __global__ void k_prop_items(const int4 *__restrict__ prop, int n_img, int K, in
 // One wavefront per item: the shifts of the item's (2 hw + 1)^2 window (in
 // increasing grid order) three at a time against its reference, with exactly
 // the arithmetic of the exhaustive kernels (ring contraction, psi FFT), the
-// maximum taken over the item's (mirror, psi) mask only.
-__global__ __launch_bounds__(256) void k_ccf_prop(
+// maximum taken over the item's (mirror, psi) mask only.  Workgroup = one
+// particle's K <= 8 items (K wavefronts), so the spectrum rows its windows share
+// are read through one CU and one L2.
+__global__ __launch_bounds__(512) void k_ccf_prop(
     const float2 *__restrict__ spec, const float2 *__restrict__ norm,
     const float2 *__restrict__ rspec, const float *__restrict__ rsum0,
     const PropItem *__restrict__ items, int K, int n_items, int S, int R, int N, int nx, int ny,
     int hw, const float2 *__restrict__ twg, PeakRec *__restrict__ rec) {
     __shared__ float2 tw[256];
-    __shared__ float2 fbuf[4][512];
-    __shared__ float zbuf[4][256];
-    __shared__ float nbuf[4][2][8];
+    __shared__ float2 fbuf[kMaxProposals][512];
+    __shared__ float zbuf[kMaxProposals][256];
+    __shared__ float nbuf[kMaxProposals][2][8];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    tw[tid] = twg[tid];
+    for (int i = tid; i < 256; i += blockDim.x) tw[i] = twg[i];
     __syncthreads();
-    const int item = blockIdx.x * 4 + wave;
+    const int item = blockIdx.x * K + wave;
     if (item >= n_items) return;
     const int p = item / K;
     const PropItem &it = items[item];
@@ -1917,7 +1919,8 @@ hipError_t launch_ccf_prop(const float2 *spec, const float2 *norm, const float2 
                            PeakRec *rec, hipStream_t st) {
     const int n_items = n_img * K;
     if (n_items <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_ccf_prop, dim3((n_items + 3) / 4), dim3(256), 0, st, spec, norm, rspec, rsum0,
+    if (K < 1 || K > kMaxProposals) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_ccf_prop, dim3(n_img), dim3(64 * K), 0, st, spec, norm, rspec, rsum0,
                        items, K, n_items, n_shift, n_ring, n_ref, nx, ny, hw, twiddle, rec);
     return hipGetLastError();
 }
