@@ -15,9 +15,11 @@
 #include <cstdlib>
 #include <cstring>
 #include <climits>
+#include <functional>
 #include <limits>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 using isac::DevParam;
@@ -123,6 +125,7 @@ struct BatchWs {
 struct Workspace {
     int dev = -1;
     hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;   // host-to-device copies overlapped with the search
     int tmpl_rings = 0;
     DevBuf tmpl, twiddle, zero_shift, shifts_i, shifts_f;
     DevBuf spec, norm, rspec, rnorm, rsum0, rec;
@@ -145,6 +148,8 @@ struct Workspace {
         shifts_uploaded.clear();
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
+        if (copy_stream) (void)hipStreamDestroy(copy_stream);
+        copy_stream = nullptr;
     }
 };
 
@@ -317,9 +322,12 @@ int ref_spectra(const float *d_ref, int N, int D, int R, hipStream_t st) {
 }
 
 // Device-resident multi-reference pipeline.
+// before_batch(p0, n), when given, runs on the host before a particle batch's
+// launches are queued (the host-pointer entry copies that batch's images there).
 int run_mref(const float *d_sbj, const float *d_ref, ShiftView acc, int P, int N, int D, int R,
              const std::vector<float2> &shifts, float *d_peak, int peak_row, float *d_d,
-             int d_pitch, float mem_frac, hipStream_t st) {
+             int d_pitch, float mem_frac, hipStream_t st,
+             const std::function<int(int, int)> &before_batch = {}) {
     const int S = (int)shifts.size();
     if (int e = check_shape(D, R, kRingLen, shifts.size())) return e;
     if (P <= 0 || N <= 0) return 0;
@@ -334,6 +342,8 @@ int run_mref(const float *d_sbj, const float *d_ref, ShiftView acc, int P, int N
     HIP_OK(ws.rec.reserve((size_t)pb * N * sizeof(PeakRec)));
     for (int p0 = 0; p0 < P; p0 += pb) {
         const int n = std::min(pb, P - p0);
+        if (before_batch)
+            if (int e = before_batch(p0, n)) return e;
         ShiftView a = acc;
         if (a.base) a.base += (size_t)p0 * a.stride;
         {
@@ -1065,14 +1075,22 @@ int alloc_params(size_t n, int ref_id) {
     return 0;
 }
 
-int upload_images(DevBuf &dst, const float **imgs, unsigned n, int D, hipStream_t st) {
+// Host image list checks and the pinned staging buffer of the host-pointer entries.
+int check_image_list(const float **imgs, unsigned n) {
     if (n == 0) return 0;
     if (!imgs) {
         set_error(kErrArg, "NULL image pointer array");
         return kErrArg;
     }
-    const size_t img_bytes = (size_t)D * D * sizeof(float);
-    const size_t need = (size_t)n * img_bytes;
+    for (unsigned i = 0; i < n; ++i)
+        if (!imgs[i]) {
+            set_error(kErrArg, "NULL image pointer at index %u", i);
+            return kErrArg;
+        }
+    return 0;
+}
+
+int reserve_stage(size_t need) {
     if (g.h_stage_cap < need) {
         if (g.h_stage) (void)hipHostFree(g.h_stage);
         g.h_stage = nullptr;
@@ -1080,13 +1098,35 @@ int upload_images(DevBuf &dst, const float **imgs, unsigned n, int D, hipStream_
         HIP_OK(hipHostMalloc((void **)&g.h_stage, need, hipHostMallocDefault));
         g.h_stage_cap = need;
     }
-    for (unsigned i = 0; i < n; ++i) {
-        if (!imgs[i]) {
-            set_error(kErrArg, "NULL image pointer at index %u", i);
-            return kErrArg;
-        }
-        memcpy(reinterpret_cast<char *>(g.h_stage) + (size_t)i * img_bytes, imgs[i], img_bytes);
+    return 0;
+}
+
+// Gathers images i0 .. i0+n-1 into the staging buffer (same offsets), split over
+// up to 8 host threads for large ranges.
+void gather_images(const float **imgs, unsigned i0, unsigned n, size_t img_bytes) {
+    char *dst = reinterpret_cast<char *>(g.h_stage);
+    auto part = [&](unsigned a, unsigned b) {
+        for (unsigned i = a; i < b; ++i) memcpy(dst + (size_t)i * img_bytes, imgs[i], img_bytes);
+    };
+    const unsigned hw = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    const unsigned nt = (size_t)n * img_bytes < ((size_t)8 << 20) ? 1u : std::min(hw, n);
+    if (nt <= 1) {
+        part(i0, i0 + n);
+        return;
     }
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t)
+        th.emplace_back(part, i0 + (unsigned)((size_t)n * t / nt), i0 + (unsigned)((size_t)n * (t + 1) / nt));
+    for (auto &x : th) x.join();
+}
+
+int upload_images(DevBuf &dst, const float **imgs, unsigned n, int D, hipStream_t st) {
+    if (n == 0) return 0;
+    if (int e = check_image_list(imgs, n)) return e;
+    const size_t img_bytes = (size_t)D * D * sizeof(float);
+    const size_t need = (size_t)n * img_bytes;
+    if (int e = reserve_stage(need)) return e;
+    gather_images(imgs, 0, n, img_bytes);
     HIP_OK(dst.reserve(need));
     HIP_OK(hipMemcpyAsync(dst.p, g.h_stage, need, hipMemcpyHostToDevice, st));
     HIP_OK(hipStreamSynchronize(st));
@@ -1221,9 +1261,41 @@ void multi_ref_align(const float **sbj_data, const float **ref_data, float *isac
         return;
     }
     hipStream_t st = ws.stream;
-    if (upload_images(g.sbj_img, sbj_data, (unsigned)P, D, st)) return;
+    if (check_image_list(sbj_data, (unsigned)P)) return;
     if (upload_images(g.ref_img, ref_data, (unsigned)N, D, st)) return;
     if (upload_params(0, (size_t)P, st)) return;
+    // particle images: each batch is gathered into pinned memory and copied on a
+    // second stream just before its launches are queued, and the search stream
+    // waits for that copy only; so the host gathers batch b + 1, and the copy
+    // engine moves it, while the device searches batch b
+    const size_t img_bytes = (size_t)D * D * sizeof(float);
+    if (reserve_stage((size_t)P * img_bytes)) return;
+    if (g.sbj_img.reserve((size_t)P * img_bytes) != hipSuccess ||
+        (!ws.copy_stream && hipStreamCreateWithFlags(&ws.copy_stream, hipStreamNonBlocking) != hipSuccess)) {
+        set_error(kErrHip, "particle image buffer or copy stream: allocation failed");
+        return;
+    }
+    std::vector<hipEvent_t> copied;
+    const std::function<int(int, int)> stage_batch = [&](int p0, int n) -> int {
+        gather_images(sbj_data, (unsigned)p0, (unsigned)n, img_bytes);
+        hipEvent_t ev = nullptr;
+        HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        copied.push_back(ev);
+        HIP_OK(hipMemcpyAsync(g.sbj_img.as<char>() + (size_t)p0 * img_bytes,
+                              reinterpret_cast<char *>(g.h_stage) + (size_t)p0 * img_bytes,
+                              (size_t)n * img_bytes, hipMemcpyHostToDevice, ws.copy_stream));
+        HIP_OK(hipEventRecord(ev, ws.copy_stream));
+        HIP_OK(hipStreamWaitEvent(st, ev, 0));
+        return 0;
+    };
+    struct EventsGone {   // after the search (or a failure) the copies are waited for
+        std::vector<hipEvent_t> &e;
+        hipStream_t cs;
+        ~EventsGone() {
+            (void)hipStreamSynchronize(cs);
+            for (hipEvent_t x : e) (void)hipEventDestroy(x);
+        }
+    } events_gone{copied, ws.copy_stream};
     if (ws.peak.reserve((size_t)N * 4 * P * sizeof(float)) != hipSuccess ||
         ws.dvals.reserve((size_t)N * P * sizeof(float)) != hipSuccess) {
         set_error(kErrHip, "output buffer allocation failed");
@@ -1231,7 +1303,8 @@ void multi_ref_align(const float **sbj_data, const float **ref_data, float *isac
     }
     ShiftView acc{&g.prm.as<DevParam>()[0].shift_x, (int)(sizeof(DevParam) / sizeof(float))};
     if (run_mref(g.sbj_img.as<float>(), g.ref_img.as<float>(), acc, P, N, D, R, g.shifts,
-                 ws.peak.as<float>(), 4 * P, ws.dvals.as<float>(), P, cuda_mem_request, st))
+                 ws.peak.as<float>(), 4 * P, ws.dvals.as<float>(), P, cuda_mem_request, st,
+                 stage_batch))
         return;
     if (hipMemcpy(isac_peak_list, ws.peak.p, (size_t)N * 4 * P * sizeof(float),
                   hipMemcpyDeviceToHost) != hipSuccess ||
