@@ -515,15 +515,29 @@ __device__ __forceinline__ float16_t mfma32_bf16(uint4 a, uint4 b, float16_t c) 
 }
 
 constexpr int kResidentLdsMax = 64 * 1024;
+constexpr int kResidentCoutMax = 128;
+
+// First maximum (in lane order) of the four lanes of a quad, valid in lane 4k:
+// the pairs (4k, 4k+1) and (4k+2, 4k+3) keep their first maximum, then the pair
+// of those, through two DPP quad permutations (lane ^ 1, lane ^ 2).  Same pick
+// as comparing the four values in order with '>'.
+__device__ __forceinline__ float quad_first_max(float m) {
+    const float m1 = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(m), 0xB1, 0xF, 0xF, false));
+    const float p = m1 > m ? m1 : m;
+    const float p2 = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(p), 0x4E, 0xF, 0xF, false));
+    return p2 > p ? p2 : p;
+}
 
 template <int CIN, int CB>
-__global__ __launch_bounds__(256, 2) void k_conv_pool_resident(
+__global__ __launch_bounds__(256, CIN == 32 ? 4 : 2) void k_conv_pool_resident(
     const unsigned short *__restrict__ x, int H, int W, const unsigned short *__restrict__ w,
     const float *__restrict__ bias, int bias_stride, int Cout, int P, int nband, int nitems, int RS,
     unsigned short *__restrict__ y) {
     constexpr int NK = 9 * CIN / 16, CPT = CIN / 16, Cs = CIN + 8, C8 = CIN / 8, K = 9 * CIN;
     extern __shared__ __align__(16) unsigned short band[];
+    __shared__ __align__(16) float sbias[kResidentCoutMax];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < Cout; i += 256) sbias[i] = bias ? bias[(size_t)i * bias_stride] : 0.0f;
     const int ncg = Cout / (32 * CB), wpg = 4 / ncg;   // co groups, wavefronts per co group
     const int co0 = (wave / wpg) * 32 * CB, wi = wave % wpg;
     uint4 wa[CB][NK];
@@ -577,21 +591,12 @@ __global__ __launch_bounds__(256, 2) void k_conv_pool_resident(
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int co = co0 + cb * 32 + 8 * q + 4 * (lane >> 5);
+                    const float4 bb = *reinterpret_cast<const float4 *>(sbias + co);
+                    const float bv[4] = {bb.x, bb.y, bb.z, bb.w};
                     float o[4];
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const float bb = bias ? bias[(size_t)(co + e) * bias_stride] : 0.0f;
-                        // window = lanes l..l|3; first maximum in window order (sub 0..3)
-                        const float m = bf2f(f2bf(fmaxf(acc[cb][4 * q + e] + bb, 0.0f)));
-                        const float m1 = __shfl_xor(m, 1, kWave);
-                        const float m2 = __shfl_xor(m, 2, kWave);
-                        const float m3 = __shfl_xor(m, 3, kWave);
-                        float r = m;
-                        if (m1 > r) r = m1;
-                        if (m2 > r) r = m2;
-                        if (m3 > r) r = m3;
-                        o[e] = r;
-                    }
+                    for (int e = 0; e < 4; ++e)   // window = the lanes of a quad (sub 0..3)
+                        o[e] = quad_first_max(bf2f(f2bf(fmaxf(acc[cb][4 * q + e] + bv[e], 0.0f))));
                     if (pv && sub == 0)
                         *reinterpret_cast<uint2 *>(y + (win0 + win) * Cout + co) =
                             pack4(o[0], o[1], o[2], o[3]);
@@ -1116,7 +1121,8 @@ int isac_nn_conv_pool_fwd(const uint16_t *x, int n, int H, int W, int Cin, const
     const int cb = 1;   // co blocks of 32 per wavefront in the resident form
     const int ncg = Cout % (32 * cb) ? 0 : Cout / (32 * cb);
     const int pres = (kResidentLdsMax / (RS * 2) - 2) / 2;
-    if ((Cin == 32 || Cin == 64) && (ncg == 1 || ncg == 2 || ncg == 4) && pres >= 1) {
+    if ((Cin == 32 || Cin == 64) && (ncg == 1 || ncg == 2 || ncg == 4) && Cout <= kResidentCoutMax &&
+        pres >= 1) {
         // band height: fewest wavefront rounds of 8 windows over the image, plus
         // half a round per band for its staging
         const int wpg = 4 / ncg, Wo = W / 2;
