@@ -514,7 +514,7 @@ __device__ __forceinline__ float16_t mfma32_bf16(uint4 a, uint4 b, float16_t c) 
                                                    __builtin_bit_cast(bf16x8_t, b), c, 0, 0, 0);
 }
 
-constexpr int kResidentLdsMax = 64 * 1024;
+constexpr int kResidentLdsMax = 63 * 1024;   // dynamic band; with the static bias copy <= 64 KB
 constexpr int kResidentCoutMax = 128;
 
 // First maximum (in lane order) of the four lanes of a quad, valid in lane 4k:
@@ -602,6 +602,75 @@ __global__ __launch_bounds__(256, CIN == 32 ? 4 : 2) void k_conv_pool_resident(
                             pack4(o[0], o[1], o[2], o[3]);
                 }
             }
+        }
+    }
+}
+
+// ------------------------------------------------------------ 3x3 convolution, resident weights
+// Unpooled 3x3 convolution (pad 1, bias, optional ReLU, bf16 output) for Cin 32 /
+// 64 / 128 and Cout 16 / 32 / 64: persistent workgroups; each wavefront keeps
+// the A operands of its 16 output channels for all NK = 9 Cin / 16 k-steps in
+// registers (NK x 2 VGPRs) and streams 16-pixel blocks (row-major over a band
+// of rows, input band with halo staged in LDS as in the pooled forms) through
+// the fully unrolled k-steps on v_mfma_f32_16x16x16_bf16, in the order of
+// k_conv_mfma: the output is bit-identical to it.
+template <int CIN>
+__global__ __launch_bounds__(256, 2) void k_conv_resident(
+    const unsigned short *__restrict__ x, int H, int W, const unsigned short *__restrict__ w,
+    const float *__restrict__ bias, int bias_stride, int Cout, int relu, int TH, int nband,
+    int nitems, int RS, unsigned short *__restrict__ y) {
+    constexpr int NK = 9 * CIN / 16, CPT = CIN / 16, Cs = CIN + 8, C8 = CIN / 8, K = 9 * CIN;
+    extern __shared__ __align__(16) unsigned short band[];
+    __shared__ __align__(16) float sbias[kResidentCoutMax];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < Cout; i += 256) sbias[i] = bias ? bias[(size_t)i * bias_stride] : 0.0f;
+    const int ncb = Cout / 16, wpc = 4 / ncb;   // co blocks of 16, wavefronts per co block
+    const int co0 = (wave / wpc) * 16, wi = wave % wpc;
+    const int li = lane & 15, lk = (lane >> 4) * 4;
+    uint2 wa[NK];
+#pragma unroll
+    for (int s = 0; s < NK; ++s)
+        wa[s] = *reinterpret_cast<const uint2 *>(w + (size_t)(co0 + li) * K + 16 * s + lk);
+    const int Wl = W + 2;
+    for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+        const int img = item / nband, bnd = item - img * nband;
+        const int r0 = bnd * TH, r1 = min(H, r0 + TH), nrow = (r1 - r0) + 2;
+        const unsigned short *src = x + (size_t)img * H * W * CIN;
+        __syncthreads();   // the previous item's band is no longer read
+        for (int i = tid; i < nrow * Wl * C8; i += 256) {
+            const int c8 = i % C8, pc = i / C8;
+            const int row = pc / Wl, col = pc - row * Wl;
+            const int sy = r0 - 1 + row, sx = col - 1;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (sy >= 0 && sy < H && sx >= 0 && sx < W)
+                v = *reinterpret_cast<const uint4 *>(src + ((size_t)sy * W + sx) * CIN + c8 * 8);
+            *reinterpret_cast<uint4 *>(band + row * RS + col * Cs + c8 * 8) = v;
+        }
+        __syncthreads();
+        const int npx = (r1 - r0) * W, nblk = (npx + 15) / 16;
+        const size_t px0 = ((size_t)img * H + r0) * W;
+        for (int b = wi; b < nblk; b += wpc) {
+            const int px = 16 * b + li;
+            const bool pv = px < npx;
+            const int pc = pv ? px : 0;
+            const int pr = pc / W, pcl = pc - pr * W;
+            // top-left tap of the lane's pixel (band row / column of input pixel - 1)
+            const unsigned short *pb = band + pr * RS + pcl * Cs + lk;
+            float4_t acc = float4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int s = 0; s < NK; ++s) {
+                const int t = s / CPT, c = (s % CPT) * 16;
+                const uint2 bv = *reinterpret_cast<const uint2 *>(pb + (t / 3) * RS + (t % 3) * Cs + c);
+                acc = mfma_bf16(wa[s], bv, acc);
+            }
+            const int co = co0 + lk;
+            const float4 bb = *reinterpret_cast<const float4 *>(sbias + co);
+            float o[4] = {acc[0] + bb.x, acc[1] + bb.y, acc[2] + bb.z, acc[3] + bb.w};
+            if (relu) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) o[v] = fmaxf(o[v], 0.0f);
+            }
+            if (pv) *reinterpret_cast<uint2 *>(y + (px0 + px) * Cout + co) = pack4(o[0], o[1], o[2], o[3]);
         }
     }
 }
@@ -1079,8 +1148,58 @@ int isac_nn_conv_fwd(const uint16_t *x, int n, int H, int W, int Cin, const uint
         return fail(2, "isac_nn_conv_fwd: bad arguments (Cin, Cout multiples of 16; ks 1 or 3)");
     const size_t np = (size_t)n * H * W;
     if (np == 0) return 0;
-    dim3 grid(blocks_for(np, 256), (unsigned)((Cout + 63) / 64));
     hipStream_t st = (hipStream_t)stream;
+    if (ks == 3 && !out_f32 && (Cin == 32 || Cin == 64 || Cin == 128) &&
+        (Cout == 16 || Cout == 32 || Cout == 64)) {
+        // resident-weight form: band height with the fewest wavefront rounds of
+        // 16 pixels, plus half a round per band for its staging
+        const int Cs = Cin + 8, Wl = W + 2;
+        const int RS = Wl * Cs + ((64 - Wl * Cs % 128) + 128) % 128;
+        const int rmax = kResidentLdsMax / (RS * 2) - 2;
+        if (rmax >= 1) {
+            const int wpc = 4 / (Cout / 16);
+            int TH = 1;
+            double best = 1e30;
+            for (int t = 1; t <= std::min(rmax, H); ++t) {
+                double cost = 0.0;
+                for (int r0 = 0; r0 < H; r0 += t) {
+                    const int nblk = (std::min(t, H - r0) * W + 15) / 16;
+                    cost += (nblk + wpc - 1) / wpc + 0.5;
+                }
+                if (cost <= best) {
+                    best = cost;
+                    TH = t;
+                }
+            }
+            const int nband = (H + TH - 1) / TH;
+            const size_t lds = (size_t)(TH + 2) * RS * 2;
+            const long long items = (long long)n * nband;
+            if (items > INT_MAX) return fail(2, "isac_nn_conv_fwd: too many images");
+            const void *fn = Cin == 32   ? reinterpret_cast<const void *>(&k_conv_resident<32>)
+                             : Cin == 64 ? reinterpret_cast<const void *>(&k_conv_resident<64>)
+                                         : reinterpret_cast<const void *>(&k_conv_resident<128>);
+            int dev = 0, cus = 0, per_cu = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, lds) != hipSuccess ||
+                per_cu < 1)
+                return fail(3, "isac_nn_conv_fwd: occupancy query failed");
+            const unsigned grid = (unsigned)std::min<long long>(items, (long long)cus * per_cu);
+            const unsigned short *xw = x;
+            unsigned short *yw = static_cast<unsigned short *>(y);
+            if (Cin == 32)
+                hipLaunchKernelGGL(k_conv_resident<32>, dim3(grid), dim3(256), lds, st, xw, H, W, w, bias,
+                                   bias_stride, Cout, relu, TH, nband, (int)items, RS, yw);
+            else if (Cin == 64)
+                hipLaunchKernelGGL(k_conv_resident<64>, dim3(grid), dim3(256), lds, st, xw, H, W, w, bias,
+                                   bias_stride, Cout, relu, TH, nband, (int)items, RS, yw);
+            else
+                hipLaunchKernelGGL(k_conv_resident<128>, dim3(grid), dim3(256), lds, st, xw, H, W, w, bias,
+                                   bias_stride, Cout, relu, TH, nband, (int)items, RS, yw);
+            return launched("isac_nn_conv_fwd");
+        }
+    }
+    dim3 grid(blocks_for(np, 256), (unsigned)((Cout + 63) / 64));
     if (ks == 3 && !out_f32)
         hipLaunchKernelGGL((k_conv_mfma<3, false>), grid, dim3(256), 0, st, x, n, H, W, Cin, w, bias,
                            bias_stride, Cout, relu, y);

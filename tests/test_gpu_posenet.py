@@ -179,3 +179,33 @@ def test_pooled_conv_is_bit_identical_to_conv_then_pool(gpu, H, cin, cout):
                   bt.data_ptr(), 1, cout, got.data_ptr(), st)
     torch.cuda.synchronize()
     assert torch.equal(got.view(torch.int16), want.view(torch.int16))
+
+
+@pytest.mark.parametrize("H,cin,cout,relu,bias", [(9, 128, 32, 1, True), (38, 32, 64, 1, True),
+                                                  (19, 64, 32, 0, False), (19, 128, 64, 0, False),
+                                                  (11, 32, 16, 1, True), (3, 128, 16, 1, True)])
+def test_resident_conv_is_bit_identical(gpu, H, cin, cout, relu, bias):
+    """The unpooled 3x3 convolution in its resident-weight form (Cin 32 / 64 /
+    128, Cout 16 / 32 / 64, one or several bands) equals the unbanded MFMA
+    kernel bit for bit (run at Cout 80, which has no resident form)."""
+    import torch
+    from isac_amd import posenet
+    lib = posenet._lib()
+    rng = np.random.default_rng(H * 7 + cin + cout)
+    n = 3
+    x = torch.from_numpy(rng.standard_normal((n, H, H, cin)).astype(np.float32)).cuda().to(torch.bfloat16)
+    w = torch.from_numpy((rng.standard_normal((80, 9 * cin)) * 0.1).astype(np.float32)).cuda()
+    w = w.to(torch.bfloat16)
+    b = torch.from_numpy(rng.standard_normal(80).astype(np.float32)).cuda()
+    st = torch.cuda.current_stream().cuda_stream
+    bp = b.data_ptr() if bias else None
+    got = torch.empty((n, H, H, cout), dtype=torch.bfloat16, device="cuda")
+    posenet._call(lib, "isac_nn_conv_fwd", x.data_ptr(), n, H, H, cin, w.data_ptr(), bp, 1, cout, 3,
+                  relu, 0, got.data_ptr(), st)
+    # Cout 80 is not a resident shape: the unbanded kernel; its first Cout channels
+    # see the same weights and bias
+    ref = torch.empty((n, H, H, 80), dtype=torch.bfloat16, device="cuda")
+    posenet._call(lib, "isac_nn_conv_fwd", x.data_ptr(), n, H, H, cin, w.data_ptr(), bp, 1, 80,
+                  3, relu, 0, ref.data_ptr(), st)
+    torch.cuda.synchronize()
+    assert torch.equal(got.view(torch.int16), ref[..., :cout].contiguous().view(torch.int16))
