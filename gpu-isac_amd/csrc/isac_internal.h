@@ -10,6 +10,11 @@ namespace isac {
 
 constexpr int kRingLen = 256;   // samples per ring (fixed by every caller)
 constexpr int kSlots = 128;     // packed spectrum slots per ring: slot 0 = (X[0], X[128])
+
+// Position of spectrum slot k in a ring row of kSlots float2: slots k and k + 64
+// are adjacent, so a lane that owns slots l and l + 64 reads both with one
+// 16-byte access (float4 l of the row).
+__host__ __device__ constexpr int slot_pos(int k) { return 2 * (k & 63) + (k >> 6); }
 constexpr int kMaxImgDim = 128; // image staged whole in LDS by the polar kernel
 constexpr int kMaxRings = 127;  // rings must fit inside a 256-sample ring FFT budget
 

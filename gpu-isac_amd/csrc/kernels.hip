@@ -115,6 +115,14 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
     return (xcd < r) ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
 }
 
+// Slots lane and lane + 64 of one ring row (adjacent in memory, slot_pos): one
+// 16-byte load.
+__device__ __forceinline__ void load_slots(const float2 *row, int lane, float2 &lo, float2 &hi) {
+    const float4 v = reinterpret_cast<const float4 *>(row)[lane];
+    lo = make_float2(v.x, v.y);
+    hi = make_float2(v.z, v.w);
+}
+
 __device__ __forceinline__ void wave_argmax(float &v, int &t) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
@@ -221,13 +229,10 @@ __global__ __launch_bounds__(256) void k_polar_spectra(
                     al = f2(zl.x, v[2].x);
                     bl = f2(zl.y, v[2].y);
                 }
-                float2 *oa = spec + (((size_t)p * S + sg[g]) * (size_t)R + i) * kSlots;
-                oa[lane] = al;
-                oa[lane + 64] = ah;
-                if (hb) {
-                    oa[kSlots + lane] = bl;
-                    oa[kSlots + lane + 64] = bh;
-                }
+                // slots lane and lane + 64 are adjacent (slot_pos): one 16-byte store per ring
+                float4 *oa = reinterpret_cast<float4 *>(spec + (((size_t)p * S + sg[g]) * (size_t)R + i) * kSlots);
+                oa[lane] = make_float4(al.x, al.y, ah.x, ah.y);
+                if (hb) oa[kSlots / 2 + lane] = make_float4(bl.x, bl.y, bh.x, bh.y);
             }
         }
 #pragma unroll
@@ -256,7 +261,7 @@ __global__ __launch_bounds__(128) void k_ref_finalize(float2 *__restrict__ rspec
     const float2 nm = rnorm[n];
     double acc0 = 0.0;
     for (int i = 0; i < R; ++i) {
-        float2 *px = rspec + ((size_t)n * R + i) * kSlots + k;
+        float2 *px = rspec + ((size_t)n * R + i) * kSlots + slot_pos(k);
         float2 x = *px;
         const float gw = nm.y * (float)(i + 1);
         if (k == 0) {
@@ -417,10 +422,15 @@ __global__ __launch_bounds__(256) void k_ccf_items(
         for (int r = 0; r < 4; ++r)
             arow[r] = spec + (size_t)p * img_stride + (size_t)min(s0 + r, S - 1) * R * kSlots;
         for (int i = 0; i < R; ++i) {
-            const float2 b0v = rb0[i * kSlots + lane], b1v = rb0[i * kSlots + lane + 64];
+            float2 b0v, b1v;
+            load_slots(rb0 + i * kSlots, lane, b0v, b1v);
+            float2 a0s[4], a1s[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) load_slots(arow[r] + i * kSlots, lane, a0s[r], a1s[r]);
+            __builtin_amdgcn_sched_barrier(0);   // all of the ring's loads in flight before the FMAs
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const float2 a0 = arow[r][i * kSlots + lane], a1 = arow[r][i * kSlots + lane + 64];
+                const float2 a0 = a0s[r], a1 = a1s[r];
                 acc[r][0][0] += b0v.x * a0.x;
                 acc[r][0][1] += b0v.y * a0.y;
                 acc[r][0][2] += b0v.x * a0.y;
@@ -504,12 +514,12 @@ __global__ __launch_bounds__(256) void k_ccf_cand4(
             float2 a0[4], a1[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                a0[r] = arow[r][i * kSlots + lane];
-                a1[r] = arow[r][i * kSlots + lane + 64];
+                load_slots(arow[r] + i * kSlots, lane, a0[r], a1[r]);
             }
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const float2 b0v = rb[j][i * kSlots + lane], b1v = rb[j][i * kSlots + lane + 64];
+                float2 b0v, b1v;
+                load_slots(rb[j] + i * kSlots, lane, b0v, b1v);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     acc[r][j][0][0] += b0v.x * a0[r].x;
@@ -645,7 +655,7 @@ __global__ __launch_bounds__(256) void k_rescore_f64(
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int k = lane + 64 * h;
-            const float2 av = a[i * kSlots + k], bv = b[i * kSlots + k];
+            const float2 av = a[i * kSlots + slot_pos(k)], bv = b[i * kSlots + slot_pos(k)];
             const double x = ((k == 0) ? ((double)av.x - muL) : (double)av.x) * g;
             const double y = (double)av.y * g;
             P[h][0] += (double)bv.x * x;
@@ -785,14 +795,11 @@ __device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb, Mref
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const float2 *ar = abuf + (size_t)buf * (2 * CH4) + (ii * 4 + r) * kSlots;
-                    a[r][0] = ar[lane];
-                    a[r][1] = ar[lane + 64];
+                    load_slots(ar, lane, a[r][0], a[r][1]);
                 }
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    const float2 *rb = j.rspec + ((size_t)nref[c] * R + i) * kSlots;
-                    b[c][0] = rb[lane];
-                    b[c][1] = rb[lane + 64];
+                    load_slots(j.rspec + ((size_t)nref[c] * R + i) * kSlots, lane, b[c][0], b[c][1]);
                 }
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
@@ -1013,10 +1020,15 @@ __global__ __launch_bounds__(512) void k_ccf_prop(
             for (int r = 0; r < 3; ++r)
                 arow[r] = spec + (size_t)p * img_stride + (size_t)row(min(r0 + r, nr - 1)) * R * kSlots;
             for (int i = 0; i < R; ++i) {
-                const float2 b0v = rb0[i * kSlots + lane], b1v = rb0[i * kSlots + lane + 64];
+                float2 b0v, b1v;
+                load_slots(rb0 + i * kSlots, lane, b0v, b1v);
+                float2 a0s[3], a1s[3];
+#pragma unroll
+                for (int r = 0; r < 3; ++r) load_slots(arow[r] + i * kSlots, lane, a0s[r], a1s[r]);
+                __builtin_amdgcn_sched_barrier(0);   // all of the ring's loads in flight before the FMAs
 #pragma unroll
                 for (int r = 0; r < 3; ++r) {
-                    const float2 a0 = arow[r][i * kSlots + lane], a1 = arow[r][i * kSlots + lane + 64];
+                    const float2 a0 = a0s[r], a1 = a1s[r];
                     acc[r][0][0] += b0v.x * a0.x;
                     acc[r][0][1] += b0v.y * a0.y;
                     acc[r][0][2] += b0v.x * a0.y;

@@ -85,34 +85,38 @@ __global__ __launch_bounds__(256) void k_spec_to_mixed(const float2 *__restrict_
     if (row >= n_rows) return;
     const float2 nm = is_ref ? make_float2(0.0f, 1.0f) : norm[row];
     const float post = is_ref ? 1.0f / (float)R : 1.0f;
-    double ss = 0.0;
+    // one pass over the rings: slots lane and lane + 64 share one 16-byte word
+    // (slot_pos); the squared norm of each slot half accumulates on its own
+    double ssh[2] = {0.0, 0.0};
+    for (int q = 0; q < Q; ++q) {
+        unsigned short re[2][4], im[2][4];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int slot = lane + 64 * h;
-        for (int q = 0; q < Q; ++q) {
-            unsigned short re[4], im[4];
+        for (int e = 0; e < 4; ++e) {
+            const int i = 4 * q + e;
+            float4 v4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (i < R) v4 = reinterpret_cast<const float4 *>(spec + ((size_t)row * R + i) * kSlots)[lane];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int i = 4 * q + e;
-                float2 x = make_float2(0.0f, 0.0f);
-                if (i < R) {
-                    x = spec[((size_t)row * R + i) * kSlots + slot];
-                    if (!is_ref) {
-                        x.x = (slot == 0) ? (x.x - nm.x) * nm.y : x.x * nm.y;
-                        x.y *= nm.y;
-                    }
+            for (int h = 0; h < 2; ++h) {
+                float2 x = h ? make_float2(v4.z, v4.w) : make_float2(v4.x, v4.y);
+                if (i < R && !is_ref) {
+                    x.x = (lane == 0 && h == 0) ? (x.x - nm.x) * nm.y : x.x * nm.y;
+                    x.y *= nm.y;
                 }
-                ss += (double)x.x * x.x + (double)x.y * x.y;
+                ssh[h] += (double)x.x * x.x + (double)x.y * x.y;
                 x.x *= post;
                 x.y *= post;
-                re[e] = BF16 ? f32_to_bf16(x.x) : f32_to_f16(x.x);
-                im[e] = BF16 ? f32_to_bf16(x.y) : f32_to_f16(x.y);
+                re[h][e] = BF16 ? f32_to_bf16(x.x) : f32_to_f16(x.x);
+                im[h][e] = BF16 ? f32_to_bf16(x.y) : f32_to_f16(x.y);
             }
-            ushort4 *o = out + (((size_t)row * Q + q) * kSlots + slot) * 2;
-            o[0] = make_ushort4(re[0], re[1], re[2], re[3]);
-            o[1] = make_ushort4(im[0], im[1], im[2], im[3]);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            ushort4 *o = out + (((size_t)row * Q + q) * kSlots + lane + 64 * h) * 2;
+            o[0] = make_ushort4(re[h][0], re[h][1], re[h][2], re[h][3]);
+            o[1] = make_ushort4(im[h][0], im[h][1], im[h][2], im[h][3]);
         }
     }
+    double ss = ssh[0] + ssh[1];
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) ss += __shfl_xor(ss, off, kWave);
     if (lane == 0) row_norm[row] = (float)sqrt(ss);
