@@ -782,36 +782,57 @@ __device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb, Mref
                 for (int h = 0; h < 2; ++h)
 #pragma unroll
                     for (int e = 0; e < 4; ++e) acc[r][c][h][e] = 0.0f;
+        // Reference spectra of ring i + 1 load while ring i's FMAs run (ping-pong
+        // registers bA / bB, rings taken in pairs), and the next chunk's rows are
+        // staged from the chunk's last ring on: every load is issued after the
+        // LDS reads of the ring in flight, so the waits the compiler places
+        // before the next LDS reads find them a ring of FMAs old.
+        auto load_refs = [&](int i, float2 (&b)[4][2]) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                load_slots(j.rspec + ((size_t)nref[c] * R + i) * kSlots, lane, b[c][0], b[c][1]);
+        };
+        auto ring = [&](int ii, int buf, const float2 (&b)[4][2], auto &&issue) {
+            float2 a[4][2];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float2 *ar = abuf + (size_t)buf * (2 * CH4) + (ii * 4 + r) * kSlots;
+                load_slots(ar, lane, a[r][0], a[r][1]);
+            }
+            issue();
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        acc[r][c][h][0] += b[c][h].x * a[r][h].x;
+                        acc[r][c][h][1] += b[c][h].y * a[r][h].y;
+                        acc[r][c][h][2] += b[c][h].x * a[r][h].y;
+                        acc[r][c][h][3] += b[c][h].y * a[r][h].x;
+                    }
+        };
+        float2 bA[4][2], bB[4][2];
         load_chunk(0, 0);
+        load_refs(0, bA);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         for (int ch = 0; ch < nchunk; ++ch) {
             const int buf = ch & 1;
-            if (ch + 1 < nchunk) load_chunk(ch + 1, buf ^ 1);
-            const int i1 = min(R, (ch + 1) * kRingChunk);
-            for (int i = ch * kRingChunk; i < i1; ++i) {
-                const int ii = i - ch * kRingChunk;
-                float2 a[4][2], b[4][2];
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const float2 *ar = abuf + (size_t)buf * (2 * CH4) + (ii * 4 + r) * kSlots;
-                    load_slots(ar, lane, a[r][0], a[r][1]);
-                }
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    load_slots(j.rspec + ((size_t)nref[c] * R + i) * kSlots, lane, b[c][0], b[c][1]);
-                }
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-#pragma unroll
-                    for (int c = 0; c < 4; ++c)
-#pragma unroll
-                        for (int h = 0; h < 2; ++h) {
-                            acc[r][c][h][0] += b[c][h].x * a[r][h].x;
-                            acc[r][c][h][1] += b[c][h].y * a[r][h].y;
-                            acc[r][c][h][2] += b[c][h].x * a[r][h].y;
-                            acc[r][c][h][3] += b[c][h].y * a[r][h].x;
-                        }
+            const int i0 = ch * kRingChunk, i1 = min(R, i0 + kRingChunk);
+            // kRingChunk is even, so only the last chunk can end on an unpaired ring
+            for (int i = i0; i < i1; i += 2) {
+                const bool last_pair = i + 2 >= i1;
+                ring(i - i0, buf, bA, [&] {
+                    if (i + 1 < R) load_refs(i + 1, bB);
+                    if (last_pair && i + 1 >= i1 && ch + 1 < nchunk) load_chunk(ch + 1, buf ^ 1);
+                });
+                if (i + 1 < i1)
+                    ring(i + 1 - i0, buf, bB, [&] {
+                        if (i + 2 < R) load_refs(i + 2, bA);
+                        if (last_pair && ch + 1 < nchunk) load_chunk(ch + 1, buf ^ 1);
+                    });
             }
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
@@ -845,7 +866,7 @@ __device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb, Mref
 // 4 rows x 4 references (128 accumulators), then each (row, reference) pair goes
 // through psi_update.
 template <int NW>
-__global__ __launch_bounds__(NW * 64) void k_ccf_mref(MrefJob j, const float2 *__restrict__ twg) {
+__global__ __launch_bounds__(NW * 64, 2) void k_ccf_mref(MrefJob j, const float2 *__restrict__ twg) {
     __shared__ MrefLds<NW> sh;
     for (int i = threadIdx.x; i < 256; i += NW * 64) sh.tw[i] = twg[i];
     __syncthreads();
@@ -862,7 +883,7 @@ __global__ __launch_bounds__(NW * 64) void k_ccf_mref(MrefJob j, const float2 *_
 // claim is a vector atomic by one lane, broadcast through LDS.  Every workgroup
 // leaves the loop once its queue is empty.
 template <int NW>
-__global__ __launch_bounds__(NW * 64) void k_ccf_mref_queue(const MrefJob *__restrict__ jobs,
+__global__ __launch_bounds__(NW * 64, 2) void k_ccf_mref_queue(const MrefJob *__restrict__ jobs,
                                                             const int4 *__restrict__ items,
                                                             const int *__restrict__ qbeg,
                                                             int *__restrict__ counters,
