@@ -729,7 +729,10 @@ template <int NW>
 __device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb, MrefLds<NW> &sh) {
     constexpr int CH4 = 4 * kRingChunk * kSlots / 2;   // float4 per staged chunk
     const int S = j.S, R = j.R, N = j.N;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    // uniform over the wavefront: the reference indices and spectrum addresses
+    // built from it live in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int g = gb * NW + wave;
     const bool active = g < j.G;
     int nref[4];
