@@ -155,7 +155,7 @@ __global__ __launch_bounds__(256) void k_polar_spectra(
     float2 *tw = reinterpret_cast<float2 *>(smem4);
     float2 *fb = tw + 256;
     float *img = reinterpret_cast<float *>(fb + 4 * 512);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int p = blockIdx.x;
     if (MASKED && smask[p] == 0ull) return;   // uniform over the workgroup
     const float *src = imgs + (size_t)p * D * D;
@@ -388,7 +388,7 @@ __global__ __launch_bounds__(256) void k_ccf_items(
     __shared__ float2 fbuf[4][512];
     __shared__ float zbuf[4][256];
     __shared__ float nbuf[4][2][8];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     tw[tid] = twg[tid];
     __syncthreads();
     const int item = blockIdx.x * 4 + wave;
@@ -468,7 +468,7 @@ __global__ __launch_bounds__(256) void k_ccf_cand4(
     __shared__ float2 fbuf[4][512];
     __shared__ float zbuf[4][256];
     __shared__ float nbuf[4][4][2][8];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     tw[tid] = twg[tid];
     __syncthreads();
     const int G = K / 4;
