@@ -714,20 +714,22 @@ __global__ void k_resolve_ties(const int *__restrict__ cand, const double *__res
 // through psi_update.
 constexpr int kRingChunk = 4;
 
-template <int NW>
+// RW shift rows per group x RB references per wavefront (RW RB = 16: 128 accumulators).
+template <int NW, int RW = 4, int RB = 4>
 struct MrefLds {
     float2 tw[256];
-    float4 abuf4[2][4 * kRingChunk * kSlots / 2];   // [buf][ring][row][slot] float2 pairs
+    float4 abuf4[2][RW * kRingChunk * kSlots / 2];   // [buf][ring][row][slot pairs]
     float2 fbuf[NW][512];
     float zbuf[NW][256];
-    float nbuf[NW][4][2][8];
+    float nbuf[NW][RB][2][8];
 };
 
 // Work of one workgroup: particle p of job j, reference groups gb*NW .. gb*NW+NW-1
 // (the twiddles must already be in sh.tw).
-template <int NW>
-__device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb, MrefLds<NW> &sh) {
-    constexpr int CH4 = 4 * kRingChunk * kSlots / 2;   // float4 per staged chunk
+template <int NW, int RW = 4, int RB = 4>
+__device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb, MrefLds<NW, RW, RB> &sh) {
+    static_assert(RW * RB == 16, "128 accumulators per lane");
+    constexpr int CH4 = RW * kRingChunk * kSlots / 2;   // float4 per staged chunk
     const int S = j.S, R = j.R, N = j.N;
     const int tid = threadIdx.x, lane = tid & 63;
     // uniform over the wavefront: the reference indices and spectrum addresses
@@ -735,19 +737,19 @@ __device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb, Mref
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int g = gb * NW + wave;
     const bool active = g < j.G;
-    int nref[4];
-    bool nval[4];
+    int nref[RB];
+    bool nval[RB];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const int n = g * 4 + c;
+    for (int c = 0; c < RB; ++c) {
+        const int n = g * RB + c;
         nval[c] = active && n < N;
         nref[c] = min(n, N - 1);
     }
-    float bv[4][2];
-    int bf[4][2];
-    float rs0[4];
+    float bv[RB][2];
+    int bf[RB][2];
+    float rs0[RB];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < RB; ++c) {
         rs0[c] = j.rsum0[nref[c]];
         bv[c][0] = bv[c][1] = -3.40282e+38f;
         bf[c][0] = bf[c][1] = INT_MAX;
@@ -756,31 +758,31 @@ __device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb, Mref
     const int nchunk = (R + kRingChunk - 1) / kRingChunk;
     const float2 *abuf = reinterpret_cast<const float2 *>(&sh.abuf4[0][0]);
 
-    for (int s0 = 0; s0 < S; s0 += 4) {
-        const float4 *rows4[4];
+    for (int s0 = 0; s0 < S; s0 += RW) {
+        const float4 *rows4[RW];
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
+        for (int r = 0; r < RW; ++r)
             rows4[r] = reinterpret_cast<const float4 *>(j.spec + (size_t)p * img_stride +
                                                         (size_t)min(s0 + r, S - 1) * R * kSlots);
         // Stage one chunk (4 rows x kRingChunk rings, 1 KB per row-ring) with direct
         // global->LDS loads: one wavefront instruction moves one row-ring, each lane
         // 16 bytes to LDS base + 16*lane.
         auto load_chunk = [&](int ch, int buf) {
-            for (int rr = wave; rr < 4 * kRingChunk; rr += NW) {
-                const int ii = rr >> 2, r = rr & 3;
+            for (int rr = wave; rr < RW * kRingChunk; rr += NW) {
+                const int ii = rr / RW, r = rr % RW;
                 const int ring = ch * kRingChunk + ii;
                 if (ring >= R) continue;
                 __builtin_amdgcn_global_load_lds(
                     (const void *)(rows4[r] + ring * (kSlots / 2) + lane),
-                    (__attribute__((address_space(3))) void *)(&sh.abuf4[buf][(ii * 4 + r) * (kSlots / 2)]),
+                    (__attribute__((address_space(3))) void *)(&sh.abuf4[buf][(ii * RW + r) * (kSlots / 2)]),
                     16, 0, 0);
             }
         };
-        float acc[4][4][2][4];
+        float acc[RW][RB][2][4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
+        for (int r = 0; r < RW; ++r)
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
+            for (int c = 0; c < RB; ++c)
 #pragma unroll
                 for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -790,24 +792,24 @@ __device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb, Mref
         // staged from the chunk's last ring on: every load is issued after the
         // LDS reads of the ring in flight, so the waits the compiler places
         // before the next LDS reads find them a ring of FMAs old.
-        auto load_refs = [&](int i, float2 (&b)[4][2]) {
+        auto load_refs = [&](int i, float2 (&b)[RB][2]) {
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
+            for (int c = 0; c < RB; ++c)
                 load_slots(j.rspec + ((size_t)nref[c] * R + i) * kSlots, lane, b[c][0], b[c][1]);
         };
-        auto ring = [&](int ii, int buf, const float2 (&b)[4][2], auto &&issue) {
-            float2 a[4][2];
+        auto ring = [&](int ii, int buf, const float2 (&b)[RB][2], auto &&issue) {
+            float2 a[RW][2];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const float2 *ar = abuf + (size_t)buf * (2 * CH4) + (ii * 4 + r) * kSlots;
+            for (int r = 0; r < RW; ++r) {
+                const float2 *ar = abuf + (size_t)buf * (2 * CH4) + (ii * RW + r) * kSlots;
                 load_slots(ar, lane, a[r][0], a[r][1]);
             }
             issue();
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
+            for (int r = 0; r < RW; ++r)
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
+                for (int c = 0; c < RB; ++c)
 #pragma unroll
                     for (int h = 0; h < 2; ++h) {
                         acc[r][c][h][0] += b[c][h].x * a[r][h].x;
@@ -816,7 +818,7 @@ __device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb, Mref
                         acc[r][c][h][3] += b[c][h].y * a[r][h].x;
                     }
         };
-        float2 bA[4][2], bB[4][2];
+        float2 bA[RB][2], bB[RB][2];
         load_chunk(0, 0);
         load_refs(0, bA);
         __builtin_amdgcn_s_waitcnt(0);
@@ -842,12 +844,12 @@ __device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb, Mref
         }
         if (!active) continue;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
+        for (int r = 0; r < RW; ++r) {
             const int s = s0 + r;
             if (s >= S) break;
             const float2 nm = j.norm[(size_t)p * S + s];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
+            for (int c = 0; c < RB; ++c) {
                 if (!nval[c]) continue;
                 psi_update<false>(acc[r][c][0], acc[r][c][1], nm, rs0[c], s, lane, sh.fbuf[wave],
                                   sh.fbuf[wave] + 256, sh.tw, sh.zbuf[wave], sh.nbuf[wave][c],
@@ -856,7 +858,7 @@ __device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb, Mref
         }
     }
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
+    for (int c = 0; c < RB; ++c)
         if (nval[c])
             write_rec(j.rec + (size_t)p * N + nref[c], bv[c], bf[c], sh.nbuf[wave][c], lane);
 }
@@ -868,15 +870,15 @@ __device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb, Mref
 // vector L1.  Lane l accumulates, over rings, P1..P4 of slots l and l+64 for
 // 4 rows x 4 references (128 accumulators), then each (row, reference) pair goes
 // through psi_update.
-template <int NW>
+template <int NW, int RW = 4, int RB = 4>
 __global__ __launch_bounds__(NW * 64, 2) void k_ccf_mref(MrefJob j, const float2 *__restrict__ twg) {
-    __shared__ MrefLds<NW> sh;
+    __shared__ MrefLds<NW, RW, RB> sh;
     for (int i = threadIdx.x; i < 256; i += NW * 64) sh.tw[i] = twg[i];
     __syncthreads();
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     const int p = lb / j.GB;
     if (p >= j.n_img) return;  // uniform over the workgroup
-    mref_block<NW>(j, p, lb % j.GB, sh);
+    mref_block<NW, RW, RB>(j, p, lb % j.GB, sh);
 }
 
 // Persistent form over many jobs (batched subsets): work items (job, particle,
@@ -1723,10 +1725,15 @@ hipError_t launch_ref_finalize(float2 *rspec, const float2 *rnorm, int n_ref, in
     return hipGetLastError();
 }
 
+// Tile of the single-job search: kMrefNW wavefronts per workgroup, each with
+// kMrefRW shift rows x kMrefRB references.
+constexpr int kMrefNW = 4, kMrefRW = 4, kMrefRB = 4;
+
 template <int NW>
 static hipError_t launch_mref_nw(const MrefJob &j, const float2 *tw, hipStream_t st) {
     const long long blocks = (long long)j.n_img * j.GB;
-    hipLaunchKernelGGL((k_ccf_mref<NW>), dim3((unsigned)blocks), dim3(NW * 64), 0, st, j, tw);
+    hipLaunchKernelGGL((k_ccf_mref<NW, kMrefRW, kMrefRB>), dim3((unsigned)blocks), dim3(NW * 64), 0,
+                       st, j, tw);
     return hipGetLastError();
 }
 
@@ -1734,10 +1741,10 @@ hipError_t launch_ccf_mref(const float2 *spec, const float2 *norm, const float2 
                            const float *rsum0, int n_img, int n_shift, int n_ring, int n_ref,
                            const float2 *twiddle, PeakRec *rec, hipStream_t st) {
     if (n_img <= 0 || n_ref <= 0) return hipSuccess;
-    // Four waves (16 references) per workgroup: at ~250 VGPRs two waves fit per
-    // SIMD, so two workgroups share a CU; measured faster than 2, 3, 5 or 8 waves.
-    MrefJob j = mref_job(spec, norm, rspec, rsum0, rec, n_img, n_shift, n_ring, n_ref, 4);
-    return launch_mref_nw<4>(j, twiddle, st);
+    MrefJob j = mref_job(spec, norm, rspec, rsum0, rec, n_img, n_shift, n_ring, n_ref, kMrefNW);
+    j.G = (n_ref + kMrefRB - 1) / kMrefRB;   // groups of kMrefRB references, one per wavefront
+    j.GB = (j.G + kMrefNW - 1) / kMrefNW;
+    return launch_mref_nw<kMrefNW>(j, twiddle, st);
 }
 
 MrefJob mref_job(const float2 *spec, const float2 *norm, const float2 *rspec, const float *rsum0,
