@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256, 2) void k_mref_mixed(
     __shared__ ushort4 zim[4][4][64];
     __shared__ float4 zc[4][64];        // psi samples of one mirror for neighbour capture
     __shared__ float nbuf[4][4][2][8];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     const int p = lb / GB;
     if (p >= n_img) return;   // uniform over the workgroup
@@ -247,12 +247,22 @@ __global__ __launch_bounds__(256, 2) void k_mref_mixed(
                 const int qq = q - qc * kQChunk;
                 const uint2 *ar0 = &abuf[buf][qq][jj >> 1][(jj & 1)];
                 const uint2 *ar1 = &abuf[buf][qq][2 + (jj >> 1)][(jj & 1)];
+                // the quad's reference operands are all issued before its first LDS
+                // read (the compiler waits for every outstanding global load there),
+                // so the quad pays one round trip instead of one per slot block
+                uint2 bq[8][2];
+#pragma unroll
+                for (int g = 0; g < 8; ++g) {
+                    const size_t off = ((size_t)q * kSlots + 16 * g + blk) * 2;
+                    bq[g][0] = brow[0][off];
+                    bq[g][1] = brow[1][off];
+                }
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int g = 0; g < 8; ++g) {
                     const int so = (16 * g + blk) * 2;
                     const uint2 a0 = ar0[so], a1 = ar1[so];
-                    const size_t off = ((size_t)q * kSlots + 16 * g + blk) * 2;
-                    const uint2 b0 = brow[0][off], b1 = brow[1][off];
+                    const uint2 b0 = bq[g][0], b1 = bq[g][1];
                     acc[g][0][0] = mfma4<BF16>(a0, b0, acc[g][0][0]);
                     acc[g][0][1] = mfma4<BF16>(a0, b1, acc[g][0][1]);
                     acc[g][1][0] = mfma4<BF16>(a1, b0, acc[g][1][0]);
