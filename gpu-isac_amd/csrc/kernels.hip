@@ -107,6 +107,19 @@ __device__ __forceinline__ float bilinear(const float *img, int D, float x, floa
     return (1.0f - fy) * ((1.0f - fx) * v00 + fx * v01) + fy * ((1.0f - fx) * v10 + fx * v11);
 }
 
+// The same sample from an LDS copy of the image with a one-pixel border that
+// repeats the edge (row / column c of the copy holds pixel clamp(c - 1)): the
+// left / upper tap is clamped to -1 .. D - 1 and both taps of a row are adjacent
+// words (one ds_read2 per row), with exactly the values and weights above.
+__device__ __forceinline__ float bilinear_bordered(const float *pimg, int D, float x, float y) {
+    const float x0 = floorf(x), y0 = floorf(y);
+    const float fx = x - x0, fy = y - y0;
+    const int cx = min(max((int)x0, -1), D - 1) + 1, cy = min(max((int)y0, -1), D - 1) + 1;
+    const float *q = pimg + cy * (D + 2) + cx;
+    const float v00 = q[0], v01 = q[1], v10 = q[D + 2], v11 = q[D + 3];
+    return (1.0f - fy) * ((1.0f - fx) * v00 + fx * v01) + fy * ((1.0f - fx) * v10 + fx * v11);
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs; remap so that consecutive
 // logical blocks (which share particle spectra) run on the same XCD and L2.
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
@@ -159,7 +172,10 @@ __global__ __launch_bounds__(256) void k_polar_spectra(
     const int p = blockIdx.x;
     if (MASKED && smask[p] == 0ull) return;   // uniform over the workgroup
     const float *src = imgs + (size_t)p * D * D;
-    for (int i = tid; i < D * D; i += 256) img[i] = src[i];
+    for (int i = tid; i < (D + 2) * (D + 2); i += 256) {   // bordered copy (bilinear_bordered)
+        const int r = i / (D + 2), c = i - r * (D + 2);
+        img[i] = src[min(max(r - 1, 0), D - 1) * D + min(max(c - 1, 0), D - 1)];
+    }
     tw[tid] = twg[tid];
     if (MASKED && tid == 0) {
         const unsigned long long mk = smask[p];
@@ -209,8 +225,8 @@ __global__ __launch_bounds__(256) void k_polar_spectra(
                 float2 v[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const float va = bilinear(img, D, cxg[g] + ta[q].x, cyg[g] + ta[q].y);
-                    const float vb = hb ? bilinear(img, D, cxg[g] + tb[q].x, cyg[g] + tb[q].y) : 0.0f;
+                    const float va = bilinear_bordered(img, D, cxg[g] + ta[q].x, cyg[g] + ta[q].y);
+                    const float vb = hb ? bilinear_bordered(img, D, cxg[g] + tb[q].x, cyg[g] + tb[q].y) : 0.0f;
                     v[q] = f2(va, vb);
                     sA[g] += wa * (double)va + wb * (double)vb;
                     sQ[g] += wa * (double)va * (double)va + wb * (double)vb * (double)vb;
@@ -1679,7 +1695,8 @@ __global__ void k_row_c2r(float *__restrict__ refs, int D, const float2 *__restr
 hipError_t prepare_launches() {
     static bool lds_raised = false;
     if (!lds_raised) {
-        const int lds = (int)((256 + 4 * 512) * sizeof(float2) + kMaxImgDim * kMaxImgDim * sizeof(float));
+        const int lds = (int)((256 + 4 * 512) * sizeof(float2) +
+                              (kMaxImgDim + 2) * (kMaxImgDim + 2) * sizeof(float));
         hipError_t e = hipFuncSetAttribute((const void *)k_polar_spectra<false>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
         if (e != hipSuccess) return e;
@@ -1696,7 +1713,7 @@ hipError_t launch_polar_spectra(const float *imgs, int n_img, int img_dim, Shift
                                 const float2 *twiddle, float2 *spec, float2 *norm,
                                 hipStream_t st) {
     if (n_img <= 0 || n_shift <= 0) return hipSuccess;
-    const size_t lds = (256 + 4 * 512) * sizeof(float2) + (size_t)img_dim * img_dim * sizeof(float);
+    const size_t lds = (256 + 4 * 512) * sizeof(float2) + (size_t)(img_dim + 2) * (img_dim + 2) * sizeof(float);
     if (hipError_t e = prepare_launches()) return e;
     dim3 grid(n_img);
     hipLaunchKernelGGL(k_polar_spectra<false>, grid, dim3(256), lds, st, imgs, img_dim, acc,
@@ -1711,7 +1728,7 @@ hipError_t launch_polar_spectra_masked(const float *imgs, int n_img, int img_dim
                                        hipStream_t st) {
     if (n_img <= 0 || n_shift <= 0) return hipSuccess;
     if (n_shift > 64) return hipErrorInvalidValue;
-    const size_t lds = (256 + 4 * 512) * sizeof(float2) + (size_t)img_dim * img_dim * sizeof(float);
+    const size_t lds = (256 + 4 * 512) * sizeof(float2) + (size_t)(img_dim + 2) * (img_dim + 2) * sizeof(float);
     if (hipError_t e = prepare_launches()) return e;
     hipLaunchKernelGGL(k_polar_spectra<true>, dim3(n_img), dim3(256), lds, st, imgs, img_dim, acc,
                        shift_i, n_shift, tmpl, n_ring, twiddle, spec, norm, smask);
