@@ -57,16 +57,36 @@ __device__ __forceinline__ void radix4(float2 (&v)[4]) {
     }
 }
 
+// Position of element a of a wave's 256-float2 exchange buffer: a ^ ((a >> 2) & 31).
+// Every access pattern of fft256_wave (4 l + q, l + 64 q, and the two stage
+// writes) then hits 32 distinct bank pairs per half-wavefront.
+__device__ __forceinline__ int fswz(int a) { return a ^ ((a >> 2) & 31); }
+
+// Twiddles of the three Stockham stages, in the order the stages read them
+// (entry (q - 1) NS + k of stage NS is e^{-2 pi i q k (64 / NS) / 256}): the
+// reads of a stage are consecutive or broadcast, so free of bank conflicts.
+constexpr int kFftTw = 3 * (4 + 16 + 64);   // 252 entries
+__device__ __forceinline__ void fill_fft_twiddles(float2 *tw, const float2 *twg, int tid,
+                                                  int nthreads) {
+    for (int i = tid; i < kFftTw; i += nthreads) {
+        const int NS = i < 12 ? 4 : i < 60 ? 16 : 64;
+        const int j = i - (NS == 4 ? 0 : NS == 16 ? 12 : 60);
+        const int q = j / NS + 1, k = j % NS;
+        tw[i] = twg[q * k * (64 / NS)];
+    }
+}
+
 // One Stockham radix-4 stage (Govindaraju et al., SC'08) with sub-transform size NS.
 template <bool INV, int NS, bool WRITE>
 __device__ __forceinline__ void stockham_stage(float2 (&v)[4], const float2 *in, float2 *out,
                                                const float2 *tw, int lane) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) v[q] = in[lane + 64 * q];
+    for (int q = 0; q < 4; ++q) v[q] = in[fswz(lane + 64 * q)];
     const int k = lane & (NS - 1);
+    constexpr int off = NS == 4 ? 0 : NS == 16 ? 12 : 60;
 #pragma unroll
     for (int q = 1; q < 4; ++q) {
-        float2 w = tw[q * k * (64 / NS)];
+        float2 w = tw[off + (q - 1) * NS + k];
         if (INV) w.y = -w.y;
         v[q] = cmul(v[q], w);
     }
@@ -74,20 +94,21 @@ __device__ __forceinline__ void stockham_stage(float2 (&v)[4], const float2 *in,
     if (WRITE) {
         const int base = (lane - k) * 4 + k;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) out[base + q * NS] = v[q];
+        for (int q = 0; q < 4; ++q) out[fswz(base + q * NS)] = v[q];
     }
 }
 
 // 256-point complex FFT by one wavefront.  In: v[q] = x[lane + 64 q].
 // Out: v[q] = X[lane + 64 q] (natural order).  b0/b1: 256-float2 wave-private
-// LDS buffers; tw: e^{-2 pi i m/256}, m < 256, in LDS.  Unnormalised.
+// LDS buffers (swizzled, fswz); tw: the stage twiddles of fill_fft_twiddles, in
+// LDS.  Unnormalised.
 template <bool INV>
 __device__ __forceinline__ void fft256_wave(float2 (&v)[4], float2 *b0, float2 *b1,
                                             const float2 *tw, int lane) {
     wave_lds_sync();
     radix4<INV>(v);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) b0[4 * lane + q] = v[q];
+    for (int q = 0; q < 4; ++q) b0[fswz(4 * lane + q)] = v[q];
     wave_lds_sync();
     stockham_stage<INV, 4, true>(v, b0, b1, tw, lane);
     wave_lds_sync();
@@ -176,7 +197,7 @@ __global__ __launch_bounds__(256) void k_polar_spectra(
         const int r = i / (D + 2), c = i - r * (D + 2);
         img[i] = src[min(max(r - 1, 0), D - 1) * D + min(max(c - 1, 0), D - 1)];
     }
-    tw[tid] = twg[tid];
+    fill_fft_twiddles(tw, twg, tid, 256);
     if (MASKED && tid == 0) {
         const unsigned long long mk = smask[p];
         int c = 0;
@@ -405,7 +426,7 @@ __global__ __launch_bounds__(256) void k_ccf_items(
     __shared__ float zbuf[4][256];
     __shared__ float nbuf[4][2][8];
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    tw[tid] = twg[tid];
+    fill_fft_twiddles(tw, twg, tid, 256);
     __syncthreads();
     const int item = blockIdx.x * 4 + wave;
     if (item >= n_items) return;
@@ -485,7 +506,7 @@ __global__ __launch_bounds__(256) void k_ccf_cand4(
     __shared__ float zbuf[4][256];
     __shared__ float nbuf[4][4][2][8];
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    tw[tid] = twg[tid];
+    fill_fft_twiddles(tw, twg, tid, 256);
     __syncthreads();
     const int G = K / 4;
     const int grp = blockIdx.x * 4 + wave;
@@ -889,7 +910,7 @@ __device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb, Mref
 template <int NW, int RW = 4, int RB = 4>
 __global__ __launch_bounds__(NW * 64, 2) void k_ccf_mref(MrefJob j, const float2 *__restrict__ twg) {
     __shared__ MrefLds<NW, RW, RB> sh;
-    for (int i = threadIdx.x; i < 256; i += NW * 64) sh.tw[i] = twg[i];
+    fill_fft_twiddles(sh.tw, twg, threadIdx.x, NW * 64);
     __syncthreads();
     const int lb = xcd_remap(blockIdx.x, gridDim.x);
     const int p = lb / j.GB;
@@ -911,7 +932,7 @@ __global__ __launch_bounds__(NW * 64, 2) void k_ccf_mref_queue(const MrefJob *__
                                                             const float2 *__restrict__ twg) {
     __shared__ MrefLds<NW> sh;
     __shared__ int claim;
-    for (int i = threadIdx.x; i < 256; i += NW * 64) sh.tw[i] = twg[i];
+    fill_fft_twiddles(sh.tw, twg, threadIdx.x, NW * 64);
     const int q = blockIdx.x & 7;
     const int q0 = qbeg[q], q1 = qbeg[q + 1];
     for (;;) {
@@ -1020,7 +1041,7 @@ __global__ __launch_bounds__(512) void k_ccf_prop(
     __shared__ float zbuf[kMaxProposals][256];
     __shared__ float nbuf[kMaxProposals][2][8];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int i = tid; i < 256; i += blockDim.x) tw[i] = twg[i];
+    fill_fft_twiddles(tw, twg, tid, blockDim.x);
     __syncthreads();
     const int item = blockIdx.x * K + wave;
     if (item >= n_items) return;
