@@ -151,15 +151,23 @@ __device__ __forceinline__ half4_t to_h4(float a, float b, float c, float d) {
 // reference operands come through the vector L1.
 constexpr int kQChunk = 3;
 
+// Position of Z-form slot u (0..63) of pair pr in a wavefront's zre / zim: pairs
+// 65 apart and bit 1 of slots 32..63 flipped.
+__device__ __forceinline__ int zpos(int pr, int u) { return pr * 65 + (u ^ (((u >> 5) & 1) << 1)); }
+
 template <bool BF16>
 __global__ __launch_bounds__(256, 2) void k_mref_mixed(
     const uint2 *__restrict__ pop, const float *__restrict__ pnorm, const uint2 *__restrict__ rop,
     const float *__restrict__ rnorm, int n_img, int S, int Q, int N, int G, int GB, float rscale,
     PeakRec *__restrict__ rec) {
     // staged particle operands: [buf][qq][row 0..3][slot 0..127][part] uint2
-    __shared__ uint2 abuf[2][kQChunk][4][kSlots * 2];
-    __shared__ ushort4 zre[4][4][64];   // [wave][pair][b*16+g] fp16 bits, 4 per ushort4
-    __shared__ ushort4 zim[4][4][64];
+    // rows padded by 16 uint2 (128 bytes): the two rows one MFMA operand read
+    // touches fall in different bank halves
+    __shared__ uint2 abuf[2][kQChunk][4][kSlots * 2 + 16];
+    // [wave][pair * 65 + zslot(b*16+g)] fp16 bits, 4 per ushort4 (stride 65 and the
+    // slot swizzle keep the Z-form writes and the stage-1 reads free of bank conflicts)
+    __shared__ ushort4 zre[4][4 * 65];
+    __shared__ ushort4 zim[4][4 * 65];
     __shared__ float4 zc[4][64];        // psi samples of one mirror for neighbour capture
     __shared__ float nbuf[4][4][2][8];
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -330,18 +338,19 @@ __global__ __launch_bounds__(256, 2) void k_mref_mixed(
                     for (int ri = 0; ri < 2; ++ri) {
                         const int pr = 2 * ri + ci;
                         if (bpart == 0) {
-                            zre[wave][pr][blk * 4 + gh] = make_ushort4(hre[ri][0], hre[ri][1], hre[ri][2], hre[ri][3]);
-                            zim[wave][pr][blk * 4 + gh] = make_ushort4(him[ri][0], him[ri][1], him[ri][2], him[ri][3]);
+                            zre[wave][zpos(pr, blk * 4 + gh)] = make_ushort4(hre[ri][0], hre[ri][1], hre[ri][2], hre[ri][3]);
+                            zim[wave][zpos(pr, blk * 4 + gh)] = make_ushort4(him[ri][0], him[ri][1], him[ri][2], him[ri][3]);
                         } else if (blk) {
                             // g = 4gh..4gh+3 -> g' = 15-g: gh 0 -> 15..12 (ushort4 3), gh 1 -> 11..8 (2)
                             const int u = (16 - blk) * 4 + 3 - gh;
-                            zre[wave][pr][u] = make_ushort4(hre[ri][3], hre[ri][2], hre[ri][1], hre[ri][0]);
-                            zim[wave][pr][u] = make_ushort4(him[ri][3], him[ri][2], him[ri][1], him[ri][0]);
+                            zre[wave][zpos(pr, u)] = make_ushort4(hre[ri][3], hre[ri][2], hre[ri][1], hre[ri][0]);
+                            zim[wave][zpos(pr, u)] = make_ushort4(him[ri][3], him[ri][2], him[ri][1], him[ri][0]);
                         } else {
                             // blk 0: g' = 16-g (g >= 1), Nyquist (g = 0) at 8:
                             // gh 0: g 0..3 -> 8, 15, 14, 13 ; gh 1: g 4..7 -> 12, 11, 10, 9
-                            unsigned short *zr16 = reinterpret_cast<unsigned short *>(zre[wave][pr]);
-                            unsigned short *zi16 = reinterpret_cast<unsigned short *>(zim[wave][pr]);
+                            // slots 2 and 3 (< 32): zpos leaves them in place
+                            unsigned short *zr16 = reinterpret_cast<unsigned short *>(&zre[wave][pr * 65]);
+                            unsigned short *zi16 = reinterpret_cast<unsigned short *>(&zim[wave][pr * 65]);
 #pragma unroll
                             for (int gg = 0; gg < 4; ++gg) {
                                 const int g = 4 * gh + gg;
@@ -361,8 +370,8 @@ __global__ __launch_bounds__(256, 2) void k_mref_mixed(
                     const int ci2 = pr & 1;
                     const float inv = 1.0f / __shfl(sc[ri], 2 * ci2, kWave);
                     // stage 1: rows b = l%16, K = g = 4(l/16)+v (re plane, then im plane)
-                    const ushort4 ar = zre[wave][pr][(lane & 15) * 4 + (lane >> 4)];
-                    const ushort4 ai = zim[wave][pr][(lane & 15) * 4 + (lane >> 4)];
+                    const ushort4 ar = zre[wave][zpos(pr, (lane & 15) * 4 + (lane >> 4))];
+                    const ushort4 ai = zim[wave][zpos(pr, (lane & 15) * 4 + (lane >> 4))];
                     const half4_t Are = __builtin_bit_cast(half4_t, ar);
                     const half4_t Aim = __builtin_bit_cast(half4_t, ai);
                     const float4_t z4 = {0.f, 0.f, 0.f, 0.f};
