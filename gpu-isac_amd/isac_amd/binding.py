@@ -87,6 +87,14 @@ def load(path: Optional[str] = None):
     p = path or LIB_PATH
     if not os.path.exists(p):
         raise FileNotFoundError(f"{p} not built; run `make -C {PKG_DIR}` (or __graft_entry__.build())")
+    # PyTorch-ROCm wheels carry their own HIP runtime (soname libamdhip64.so.7, as
+    # the library needs).  Loaded after torch, the library binds to that runtime;
+    # loaded first, it binds /opt/rocm's and torch later loads a second runtime
+    # into the process, and the library's first device call then finds no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(p)
     c_uint, c_int, c_float, c_bool = ctypes.c_uint, ctypes.c_int, ctypes.c_float, ctypes.c_bool
     lib.print_gpu_info.argtypes = [c_uint]
