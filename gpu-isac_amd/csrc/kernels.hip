@@ -923,13 +923,12 @@ __global__ __launch_bounds__(NW * 64, 2) void k_ccf_mref(MrefJob j, const float2
 // queue, so its spectra stay in one L2); each workgroup takes items from the
 // queue of the XCD it runs on (blockIdx % 8) until the queue is exhausted.  The
 // claim is a vector atomic by one lane, broadcast through LDS.  Every workgroup
-// leaves the loop once its queue is empty.
+// leaves the loop once its queue is empty.  (The one-wavefront tile asks for no
+// occupancy target: its registers leave one wavefront per SIMD either way.)
 template <int NW>
-__global__ __launch_bounds__(NW * 64, 2) void k_ccf_mref_queue(const MrefJob *__restrict__ jobs,
-                                                            const int4 *__restrict__ items,
-                                                            const int *__restrict__ qbeg,
-                                                            int *__restrict__ counters,
-                                                            const float2 *__restrict__ twg) {
+__global__ __launch_bounds__(NW * 64, NW == 1 ? 1 : 2) void k_ccf_mref_queue(
+    const MrefJob *__restrict__ jobs, const int4 *__restrict__ items, const int *__restrict__ qbeg,
+    int *__restrict__ counters, const float2 *__restrict__ twg) {
     __shared__ MrefLds<NW> sh;
     __shared__ int claim;
     fill_fft_twiddles(sh.tw, twg, threadIdx.x, NW * 64);
