@@ -133,3 +133,23 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode != 0 and "one GPU" in r.stderr
+
+
+def test_load_binds_the_hip_runtime_torch_carries():
+    # PyTorch-ROCm wheels ship their own libamdhip64 (same soname as /opt/rocm's):
+    # binding.load() imports torch first, so the process maps one HIP runtime.
+    import subprocess
+    import sys
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "from isac_amd import binding\n"
+        "binding.load()\n"
+        "maps = open('/proc/self/maps').read().split('\\n')\n"
+        "libs = sorted({l.split()[-1] for l in maps if 'libamdhip64' in l})\n"
+        "print('\\n'.join(libs))\n" % os.path.join(ROOT, "gpu-isac_amd"))
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stderr[-2000:]
+    libs = out.stdout.split()
+    assert len(libs) == 1, libs
+    import torch
+    assert os.path.dirname(libs[0]) == os.path.join(os.path.dirname(torch.__file__), "lib"), libs
