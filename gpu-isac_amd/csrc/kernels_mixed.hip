@@ -21,7 +21,7 @@
 // DFT16 intermediate stays below the FP16 maximum.
 //
 // The results are approximate (FP16/BF16 rounding); candidates are re-scored
-// exactly in FP32 by k_ccf_items (kernels.hip) and near-ties in FP64 here.
+// exactly in FP32 by k_ccf_cand4 (kernels.hip) and near-ties in FP64 here.
 #include "isac_internal.h"
 
 #include <climits>
