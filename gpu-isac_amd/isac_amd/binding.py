@@ -522,14 +522,20 @@ def assign_groups(d):
 
 def classes_from_assignment(groups, n_ref):
     """Class of each particle from assign_groups' n_img + 1 layout, as the driver
-    builds ``belongsto`` (bin/sp_isac2_gpu.py:1122-1135)."""
+    builds ``belongsto`` (bin/sp_isac2_gpu.py:1122-1135): the first cap * n_ref
+    entries fill the classes in blocks of cap = n_img // n_ref; only the first
+    n_img % cap leftovers join the class in the last entry, and every particle
+    the driver does not place keeps its default class 0 (``belongsto = [0]*nima``).
+    That default applies whenever n_img - cap * n_ref >= cap, i.e. n_ref > cap."""
     g = np.asarray(groups)
     P = g.size - 1
     cap = P // n_ref
-    cls = np.empty(P, np.int32)
+    if cap < 1:
+        raise ValueError(f"{P} particles cannot fill {n_ref} classes (the driver divides by n_img // n_ref)")
+    cls = np.zeros(P, np.int32)
     for i in range(cap * n_ref):
         cls[g[i]] = i // cap
-    for i in range(P - cap * n_ref):
+    for i in range(P % cap):
         cls[g[cap * n_ref + i]] = g[-1]
     return cls
 

@@ -24,6 +24,9 @@ TIE_REL = 1e-5
 # Gaussian targets over neighbouring psi bins (sigma 1 bin) and shifts (0.7 step).
 NET = dict(channels=(32, 64, 128, 32), hidden=384, pool4=False)
 TRAIN = dict(batch=256, lr=2e-3, wd=1e-4, warmup=1000, sigma_psi=1.0, sigma_shift=0.7)
+# Agreement with the exhaustive search that the fallback threshold is calibrated
+# for, on the calibration set (the held-out agreement comes out a little lower).
+TARGET = 0.998
 
 
 def exhaustive(sbj, ref, ring_num, shift_rng, shift_step=1.0, chunk=20000):

@@ -188,14 +188,50 @@ def test_pure_mirror_selects_mirror():
 
 # --------------------------------------------------------- P8 tie-break
 
-def test_first_max_wins_order():
-    # the oracle's decode scans (m, s, t) flattened: the first maximum wins
-    tab = np.zeros((1, 1, 2, 3, 8))
-    tab[0, 0, 1, 0, 2] = 5.0     # mirror 1, shift 0
-    tab[0, 0, 0, 2, 7] = 5.0     # mirror 0, shift 2  (earlier in (m, s, t) order)
-    flat = tab.reshape(1, 1, -1)
-    idx = np.argmax(flat, axis=-1)[0, 0]
-    assert idx == (0 * 3 + 2) * 8 + 7
+def first_of_class(shifts):
+    """For each grid shift, the first grid index with the same int-truncated
+    shift (quirk Q1: the kernel samples at the truncated shift, so all members
+    of a class produce identical polar rows and exactly tied CCF values)."""
+    tr = [tuple(v) for v in np.trunc(shifts).astype(int).tolist()]
+    first = {}
+    for i, k in enumerate(tr):
+        first.setdefault(k, i)
+    return np.array([first[k] for k in tr]), np.array([tr.count(k) for k in tr])
+
+
+def test_first_max_wins_on_exact_ties():
+    """Exact CCF ties built from the int truncation of sub-pixel grid shifts:
+    with step 0.5 the shifts -0.5, 0 and 0.5 sample the same rows.  Every
+    decoder must return the FIRST tied shift in grid order (strict '>' scans,
+    cuda/gpu_aln_noref.cu:1729-1741 for multi-ref, :1638-1664 for the row
+    argmax); a last-max-wins ('>=') decoder returns the last one and fails."""
+    ds = synth.make_dataset(12, 3, 48, 20.0, 5.0, 1, True)
+    acc = np.zeros((12, 2), np.float32)
+    shifts = O.shift_grid(1.0, 1.0, 0.5)
+    first, size = first_of_class(shifts)
+    a = O.multi_ref_align(ds.sbj, ds.ref, acc, 20, 1.0, 1.0, 0.5)
+    b = coracle.mref(ds.sbj, ds.ref, acc, 20, 1.0, 1.0, 0.5)
+    sa = a["shift_idx"].astype(int)
+    dup = size[sa] > 1
+    assert dup.sum() >= 10, "the data must put winners in duplicated classes"
+    # the tie is exact (identical rows), not merely within tolerance
+    assert np.all(a["runner"][dup] == a["d"][dup])
+    assert np.array_equal(sa, first[sa]), "numpy oracle: not the first tied shift"
+    assert np.array_equal(b["shift_idx"], sa), "C oracle differs from the numpy oracle"
+    # the decoded shift is the first member's (its float shift, not the class's)
+    assert np.array_equal(b["sx"], shifts[sa, 0].astype(np.float64))
+    assert np.array_equal(b["sy"], shifts[sa, 1].astype(np.float64))
+    # pre-align / reference-free row argmax (decode S13) on the same rows
+    tab = O.single_ref_table(ds.sbj, ds.ref, np.zeros(12, int), acc, 20, shifts)
+    prm = O.decode_single_ref(tab, acc, shifts, 48, 20)
+    s1 = prm["shift_idx"].astype(int)
+    assert (size[s1] > 1).sum() >= 4
+    assert np.array_equal(s1, first[s1]), "row argmax: not the first tied shift"
+    # planted mistake: a last-max-wins scan picks a different member
+    flat = tab.reshape(12, -1)
+    last = flat.shape[1] - 1 - np.argmax(flat[:, ::-1], axis=-1)
+    s_last = (last // 256) % shifts.shape[0]
+    assert np.any(s_last != s1)
 
 
 def test_c_oracle_matches_numpy_oracle():
