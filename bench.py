@@ -418,6 +418,9 @@ def main():
                     help="skip the compact precision / batched / learned summaries")
     ap.add_argument("--no-host-abi", action="store_true",
                     help="skip the host-pointer ABI timing (multi_ref_align, PCIe included)")
+    ap.add_argument("--probe-form", choices=["fp32", "fp16", "bf16", "fp16_best"], default="fp32",
+                    help="the form --probe runs: the FP32 search (default), the FP16 / BF16 MFMA "
+                         "search, or the FP16 MFMA search with FP32 re-scoring of the best")
     ap.add_argument("--probe", action="store_true",
                     help="one step of the workload plus a 1 GiB device copy, nothing printed: the "
                          "target of rocprofv3 --pmc runs (the copy calibrates the byte counters)")
@@ -468,7 +471,13 @@ def main():
         binding.mref_align_device(sbj, ref, R, 1.0, rng, rng, acc_xy=acc, out=(peak, dval))
 
     if args.probe:
-        step()
+        if args.probe_form == "fp32":
+            step()
+        elif args.probe_form == "fp16_best":
+            binding.mref_best_mixed_device(sbj, ref, R, 1.0, rng, rng, "fp16", acc_xy=acc)
+        else:
+            binding.mref_align_mixed_device(sbj, ref, R, 1.0, rng, rng, args.probe_form,
+                                            acc_xy=acc, out=(peak2, dval2))
         src = torch.ones(1 << 28, dtype=torch.float32, device=dev)   # 1 GiB
         dst = torch.empty_like(src)
         dst.copy_(src)

@@ -390,16 +390,26 @@ int run_mref_mixed(const float *d_sbj, const float *d_ref, ShiftView acc, int P,
     if (int e = ensure_template(R)) return e;
     if (int e = ensure_shifts(shifts)) return e;
     if (int e = ref_spectra(d_ref, N, D, R, st)) return e;
-    const int Q = (R + 3) / 4;
-    const size_t op_row = (size_t)Q * kSlots * 2 * 8;   // bytes of 16-bit operands per row
-    HIP_OK(ws.mix_r.reserve((size_t)N * op_row));
-    HIP_OK(ws.mix_rn.reserve((size_t)N * sizeof(float)));
-    HIP_OK(isac::launch_spec_to_mixed(ws.rspec.as<float2>(), nullptr, N, R, 1, bf16, ws.mix_r.p,
-                                      ws.mix_rn.as<float>(), st));
-    const size_t per = (size_t)S * R * kSlots * sizeof(float2) + (size_t)S * sizeof(float2) +
-                       (size_t)S * (op_row + sizeof(float)) + (size_t)N * sizeof(PeakRec) +
+    if (S > 1024) {
+        set_error(kErrUnsupported, "mixed-precision search: %d shifts (at most 1024)", S);
+        return kErrUnsupported;
+    }
+    const isac::GemmLayout gl = isac::gemm_layout(R);
+    const int NB = (N + 15) / 16;
+    HIP_OK(ws.mix_r.reserve((size_t)NB * gl.block_bytes));
+    HIP_OK(ws.mix_rn.reserve((size_t)N * sizeof(float)));   // power-of-two scale per reference
+    HIP_OK(ws.mix_pn.reserve((size_t)N * sizeof(float)));   // 1 / (c_p c_n) per reference
+    HIP_OK(isac::launch_frag_refs(ws.rspec.as<float2>(), N, R, bf16, ws.mix_rn.as<float>(),
+                                  ws.mix_pn.as<float>(), ws.mix_r.p, st));
+    const size_t op_part = gl.block_bytes / 16 * (size_t)isac::gemm_rows_per_particle(S);
+    // operands straight from the images when the rings fit the direct kernel; the
+    // FP32 spectra are then only needed for the exact re-scoring of the best form
+    const bool direct = isac::polar_frag_supported(R);
+    const bool need_spec = bo != nullptr || !direct;
+    const size_t per = (need_spec ? (size_t)S * R * kSlots * sizeof(float2) + (size_t)S * sizeof(float2) : 0) +
+                       op_part + (size_t)N * sizeof(PeakRec) +
                        kCandidates * (sizeof(PeakRec) + sizeof(int)) + 16 * sizeof(int);
-    const int pb = batch_for(per, 0, mem_frac, P);
+    const int pb = batch_for(per, (size_t)16 * op_part, mem_frac, P);
     if (bo) {
         HIP_OK(ws.cand.reserve((size_t)pb * kCandidates * sizeof(int)));
         HIP_OK(ws.rec_c.reserve((size_t)pb * kCandidates * sizeof(PeakRec)));
@@ -408,26 +418,32 @@ int run_mref_mixed(const float *d_sbj, const float *d_ref, ShiftView acc, int P,
         HIP_OK(ws.tie.reserve((size_t)pb * sizeof(int)));
         HIP_OK(ws.val64.reserve((size_t)pb * 2 * sizeof(double)));
     }
-    HIP_OK(ws.spec.reserve((size_t)pb * S * R * kSlots * sizeof(float2)));
-    HIP_OK(ws.norm.reserve((size_t)pb * S * sizeof(float2)));
-    HIP_OK(ws.mix_p.reserve((size_t)pb * S * op_row));
-    HIP_OK(ws.mix_pn.reserve((size_t)pb * S * sizeof(float)));
+    if (need_spec) {
+        HIP_OK(ws.spec.reserve((size_t)pb * S * R * kSlots * sizeof(float2)));
+        HIP_OK(ws.norm.reserve((size_t)pb * S * sizeof(float2)));
+    }
+    HIP_OK(ws.mix_p.reserve(isac::gemm_particle_operand_bytes(pb, S, R)));
     HIP_OK(ws.rec.reserve((size_t)pb * N * sizeof(PeakRec)));
     for (int p0 = 0; p0 < P; p0 += pb) {
         const int n = std::min(pb, P - p0);
         ShiftView a = acc;
         if (a.base) a.base += (size_t)p0 * a.stride;
-        HIP_OK(isac::launch_polar_spectra(d_sbj + (size_t)p0 * D * D, n, D, a,
-                                          ws.shifts_i.as<int2>(), S, ws.tmpl.as<float2>(), R,
-                                          ws.twiddle.as<float2>(), ws.spec.as<float2>(),
-                                          ws.norm.as<float2>(), st));
-        HIP_OK(isac::launch_spec_to_mixed(ws.spec.as<float2>(), ws.norm.as<float2>(), n * S, R, 0,
-                                          bf16, ws.mix_p.p, ws.mix_pn.as<float>(), st));
+        if (need_spec)
+            HIP_OK(isac::launch_polar_spectra(d_sbj + (size_t)p0 * D * D, n, D, a,
+                                              ws.shifts_i.as<int2>(), S, ws.tmpl.as<float2>(), R,
+                                              ws.twiddle.as<float2>(), ws.spec.as<float2>(),
+                                              ws.norm.as<float2>(), st));
+        if (direct)
+            HIP_OK(isac::launch_polar_frag(d_sbj + (size_t)p0 * D * D, n, D, a,
+                                           ws.shifts_i.as<int2>(), S, R, bf16,
+                                           ws.twiddle.as<float2>(), ws.mix_p.p, st));
+        else
+            HIP_OK(isac::launch_frag_rows(ws.spec.as<float2>(), ws.norm.as<float2>(), n, S, R, bf16,
+                                          ws.mix_p.p, st));
         {
             ProfScope ps(g.prof, st);
-            HIP_OK(isac::launch_mref_mixed(ws.mix_p.p, ws.mix_pn.as<float>(), ws.mix_r.p,
-                                           ws.mix_rn.as<float>(), n, S, R, N, bf16,
-                                           ws.rec.as<PeakRec>(), st));
+            HIP_OK(isac::launch_mref_gemm(ws.mix_p.p, ws.mix_r.p, ws.mix_pn.as<float>(), n, S, R,
+                                          N, bf16, ws.rec.as<PeakRec>(), st));
             if (g.prof) { g.prof_ccf_ms += ps.finish(); g.prof_ccf_n++; }
         }
         HIP_OK(isac::launch_decode_mref(ws.rec.as<PeakRec>(), n, N, a, ws.shifts_f.as<float2>(), S,

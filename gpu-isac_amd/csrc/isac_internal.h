@@ -240,14 +240,43 @@ hipError_t launch_update_references(const float *imgs, int P, int D, const float
                                    const int *cls, int N, float cutoff, float falloff,
                                    UpdateScratch w, float *refs, float4 *params, hipStream_t st);
 
-// Mixed-precision (MFMA) multi-reference search: FP32 spectra -> 16-bit MFMA
-// operands [row][ceil(R/4)][kSlots][re,im][4 rings] (+ Frobenius norm per row).
-hipError_t launch_spec_to_mixed(const float2 *spec, const float2 *norm, int n_rows, int n_ring,
-                                int is_ref, bool bf16, void *out, float *row_norm, hipStream_t st);
+// Mixed-precision (MFMA) multi-reference search.  Operands are FP16 / BF16 rows
+// (particle shift rows, or references): per spectrum slot the two parts over
+// K = rings padded to k_pad, grouped in the MFMA K-steps; blocks of 16 rows.
+struct GemmLayout {
+    int full16;          // 16-ring K-steps
+    int tail;            // rings of the last step: 0, 8 or 16
+    int k_pad;           // 16 full16 + tail
+    int slot_bytes;      // 4 k_pad: one slot of one row
+    size_t row_bytes;    // kSlots slot_bytes
+    size_t block_bytes;  // 16 rows
+    float cp;            // power-of-two scale of the particle operands
+};
+GemmLayout gemm_layout(int n_ring);
+int gemm_index_bits(int n_shift);
 
+// Particle operand rows: particle p, shift s at row p * S_pad + s with
+// S_pad = gemm_rows_per_particle(S) = max(S, 16) (so a block of 16 rows holds
+// rows of at most two particles); padding rows are zero.
+int gemm_rows_per_particle(int n_shift);
+size_t gemm_particle_operand_bytes(int n_img, int n_shift, int n_ring);
+// Particle operands from the raw FP32 spectra [n_img][n_shift] + normalisation.
+hipError_t launch_frag_rows(const float2 *spec, const float2 *norm, int n_img, int n_shift,
+                            int n_ring, bool bf16, void *frag, hipStream_t st);
+// Particle operands straight from the images (no FP32 spectra): the mixed
+// search's stage A, for n_ring <= 40 (polar_frag_supported).
+bool polar_frag_supported(int n_ring);
+hipError_t launch_polar_frag(const float *imgs, int n_img, int img_dim, ShiftView acc,
+                             const int2 *shift_i, int n_shift, int n_ring, bool bf16,
+                             const float2 *twiddle, void *frag, hipStream_t st);
+// Reference operands (normalised, ring-weighted spectra), padded to a multiple
+// of 16 references; cn[n] = the power-of-two scale of reference n, rinv[n] =
+// 1 / (c_p cn[n]) (the factor that undoes both scales).
+hipError_t launch_frag_refs(const float2 *rspec, int n_ref, int n_ring, bool bf16, float *cn,
+                            float *rinv, void *frag, hipStream_t st);
 // Approximate peaks of every (particle, reference) pair, rec[p*n_ref + n].
-hipError_t launch_mref_mixed(const void *pop, const float *pnorm, const void *rop,
-                             const float *rnorm, int n_img, int n_shift, int n_ring, int n_ref,
-                             bool bf16, PeakRec *rec, hipStream_t st);
+hipError_t launch_mref_gemm(const void *pfrag, const void *rfrag, const float *rinv, int n_img,
+                            int n_shift, int n_ring, int n_ref, bool bf16, PeakRec *rec,
+                            hipStream_t st);
 
 }  // namespace isac

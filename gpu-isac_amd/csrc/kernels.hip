@@ -291,6 +291,230 @@ __global__ __launch_bounds__(256) void k_polar_spectra(
     }  // row groups
 }
 
+// Stage A of the mixed-precision search: the normalised ring spectra of every
+// image at every grid shift, written directly as 16-bit MFMA operand rows (the
+// GEMM layout of kernels_mixed.hip): row p * S_pad + s holds, per slot, the
+// parts a = g (Re S + Im S) c_p and b = g (Re S - Im S) c_p over the rings (slot
+// 0: a = g (S[0] - mu L) c_p, b = g S[128] c_p), K padded with zeros.  Rows
+// s >= S and images p >= n_img (padding of the last group of 16) are zero.
+// Workgroup = one image, staged in LDS as (image - m) / s with its own mean and
+// standard deviation (so the raw spectra of a row fit FP16 and the DC term needs
+// no large cancellation); wavefront w computes rows w, w + 4, ...  A row's raw
+// parts (Re + Im, Re - Im) stay in registers as packed FP16 (2 slots x R rings
+// per lane) until the row's normalisation sums are complete; the exact row
+// normalisation is applied in FP32 at the final conversion, and every slot is
+// one contiguous store.  Sampling positions are the centre plus (i + 1)(cos,
+// sin) of the lane's four angles; the normalisation sums add FP32 partial sums
+// of four samples in FP64.
+template <bool BF16>
+__device__ __forceinline__ unsigned short frag16(float x) {
+    if (BF16) {
+        unsigned u = __float_as_uint(x);
+        u += 0x7fffu + ((u >> 16) & 1u);
+        return (unsigned short)(u >> 16);
+    }
+    return __builtin_bit_cast(unsigned short, (_Float16)x);
+}
+
+__device__ __forceinline__ unsigned h2pack(float lo, float hi) {
+    return (unsigned)__builtin_bit_cast(unsigned short, (_Float16)lo) |
+           ((unsigned)__builtin_bit_cast(unsigned short, (_Float16)hi) << 16);
+}
+__device__ __forceinline__ float h2lo(unsigned v) {
+    return (float)__builtin_bit_cast(_Float16, (unsigned short)(v & 0xffffu));
+}
+__device__ __forceinline__ float h2hi(unsigned v) {
+    return (float)__builtin_bit_cast(_Float16, (unsigned short)(v >> 16));
+}
+
+constexpr int kFragMaxRings = 40;
+
+template <bool BF16>
+__global__ __launch_bounds__(256, 2) void k_polar_frag(
+    const float *__restrict__ imgs, int n_img, int D, ShiftView acc,
+    const int2 *__restrict__ shift_i, int S, int S_pad, int R, int full16, int tail,
+    int slot_bytes, float cp, const float2 *__restrict__ twg, unsigned char *__restrict__ frag) {
+    extern __shared__ float4 smem4[];
+    __shared__ double red[2][4];
+    float2 *tw = reinterpret_cast<float2 *>(smem4);
+    float2 *fb = tw + 256;
+    float *img = reinterpret_cast<float *>(fb + 4 * 512);
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int p = blockIdx.x;
+    const size_t row_bytes = (size_t)kSlots * slot_bytes;
+    unsigned char *prow = frag + (size_t)p * S_pad * row_bytes;
+    const int n_rows = (p < n_img) ? S : 0;
+    // zero rows: shifts S .. S_pad - 1 (and every row of a padding image)
+    {
+        const size_t zb = (size_t)(S_pad - n_rows) * row_bytes;
+        uint4 *z = reinterpret_cast<uint4 *>(prow + (size_t)n_rows * row_bytes);
+        for (size_t i = tid; i < zb / 16; i += 256) z[i] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    if (n_rows == 0) return;   // uniform over the workgroup
+    const float *src = imgs + (size_t)p * D * D;
+    // image mean and standard deviation (FP64)
+    double s1 = 0.0, s2 = 0.0;
+    for (int i = tid; i < D * D; i += 256) {
+        const double v = src[i];
+        s1 += v;
+        s2 += v * v;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        s1 += __shfl_xor(s1, off, kWave);
+        s2 += __shfl_xor(s2, off, kWave);
+    }
+    if (lane == 0) {
+        red[0][wave] = s1;
+        red[1][wave] = s2;
+    }
+    __syncthreads();
+    const double t1 = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    const double t2 = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    const double im = t1 / (double)(D * D);
+    double isd = sqrt(fmax(t2 / (double)(D * D) - im * im, 0.0));
+    if (!(isd > 1e-30)) isd = 1.0;
+    const float mimg = (float)im, rs = (float)(1.0 / isd);
+    for (int i = tid; i < (D + 2) * (D + 2); i += 256) {   // bordered copy (bilinear_bordered)
+        const int r = i / (D + 2), c = i - r * (D + 2);
+        img[i] = (src[min(max(r - 1, 0), D - 1) * D + min(max(c - 1, 0), D - 1)] - mimg) * rs;
+    }
+    fill_fft_twiddles(tw, twg, tid, 256);
+    __syncthreads();
+    float ax = 0.0f, ay = 0.0f;
+    if (acc.base) {
+        ax = acc.base[(size_t)p * acc.stride];
+        ay = acc.base[(size_t)p * acc.stride + 1];
+    }
+    float cq[4], sq[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sincospif((float)(lane + 64 * q) / 128.0f, &sq[q], &cq[q]);
+    float2 *b0 = fb + wave * 512;
+    float2 *b1 = b0 + 256;
+    const int pl = (64 - lane) & 63;
+    for (int j = wave; j < n_rows; j += 4) {
+        const int2 si = shift_i[j];
+        const float cx = (float)(D / 2 + si.x) + ax, cy = (float)(D / 2 + si.y) + ay;
+        // ring i, slot lane + 64 h: packed FP16 (Re + Im, Re - Im); slot 0: (S[0], S[128])
+        unsigned sp[kFragMaxRings][2];
+        double sA = 0.0, sQ = 0.0;
+#pragma unroll
+        for (int i = 0; i < kFragMaxRings; i += 2) {
+            if (i < R) {   // uniform
+                const bool hb = (i + 1) < R;
+                const float ra = (float)(i + 1), rb = (float)(i + 2);
+                float2 v[4];
+                float pa = 0.f, qa = 0.f, pb = 0.f, qb = 0.f;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float va = bilinear_bordered(img, D, cx + ra * cq[q], cy + ra * sq[q]);
+                    const float vb = hb ? bilinear_bordered(img, D, cx + rb * cq[q], cy + rb * sq[q]) : 0.0f;
+                    v[q] = f2(va, vb);
+                    pa += va;
+                    qa += va * va;
+                    pb += vb;
+                    qb += vb * vb;
+                }
+                sA += (double)ra * pa + (double)rb * pb;
+                sQ += (double)ra * qa + (double)rb * qb;
+                fft256_wave<false>(v, b0, b1, tw, lane);
+                const float2 u2 = shfl2(v[2], pl), u3 = shfl2(v[3], pl);
+                const float2 zl = v[0], zlp = lane ? u3 : v[0];
+                const float2 zh = v[1], zhp = lane ? u2 : v[3];
+                float2 al = f2(0.5f * (zl.x + zlp.x), 0.5f * (zl.y - zlp.y));
+                float2 bl = f2(0.5f * (zl.y + zlp.y), 0.5f * (zlp.x - zl.x));
+                const float2 ah = f2(0.5f * (zh.x + zhp.x), 0.5f * (zh.y - zhp.y));
+                const float2 bh = f2(0.5f * (zh.y + zhp.y), 0.5f * (zhp.x - zh.x));
+                if (lane == 0) {
+                    // slot 0 keeps the two real bins (S[0], S[128]) instead of (Re+Im, Re-Im)
+                    sp[i][0] = h2pack(zl.x, v[2].x);
+                    if (i + 1 < kFragMaxRings) sp[i + 1][0] = h2pack(zl.y, v[2].y);
+                } else {
+                    sp[i][0] = h2pack(al.x + al.y, al.x - al.y);
+                    if (i + 1 < kFragMaxRings) sp[i + 1][0] = h2pack(bl.x + bl.y, bl.x - bl.y);
+                }
+                sp[i][1] = h2pack(ah.x + ah.y, ah.x - ah.y);
+                if (i + 1 < kFragMaxRings) sp[i + 1][1] = h2pack(bh.x + bh.y, bh.x - bh.y);
+            }
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            sA += __shfl_xor(sA, off, kWave);
+            sQ += __shfl_xor(sQ, off, kWave);
+        }
+        // row statistics of the standardised samples; the row's normalisation of
+        // the original image is g = 1 / (isd sd + 1e-7), applied to spectra / rs
+        const double nn = (double)kRingLen * 0.5 * (double)R * (double)(R + 1);
+        const double mu = sA / nn;
+        const double sd = sqrt(fabs(sQ / nn - mu * mu));
+        const float gf = (float)(isd / (isd * sd + 1e-7)) * cp;
+        const float muL = (float)(mu * kRingLen);
+        unsigned char *rowp = prow + (size_t)j * row_bytes;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {   // slot lane (h = 0) and lane + 64 (h = 1)
+            const int k = lane + 64 * h;
+            unsigned char *dst = rowp + (size_t)k * slot_bytes;
+            const float dc = (k == 0) ? muL : 0.0f;
+#pragma unroll
+            for (int ks = 0; ks < (kFragMaxRings + 15) / 16; ++ks) {
+                if (ks < full16 + (tail == 16 ? 1 : 0)) {
+#pragma unroll
+                    for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+                        for (int hh = 0; hh < 2; ++hh) {
+                            unsigned e[4];
+#pragma unroll
+                            for (int jj = 0; jj < 4; ++jj) {
+                                unsigned short lo = 0, hi = 0;
+                                const int i0 = 16 * ks + 8 * hh + 2 * jj;
+                                if (i0 < kFragMaxRings && i0 < R) {
+                                    const unsigned v = sp[i0][h];
+                                    lo = frag16<BF16>(pt ? h2hi(v) * gf : (h2lo(v) - dc) * gf);
+                                }
+                                if (i0 + 1 < kFragMaxRings && i0 + 1 < R) {
+                                    const unsigned v = sp[i0 + 1][h];
+                                    hi = frag16<BF16>(pt ? h2hi(v) * gf : (h2lo(v) - dc) * gf);
+                                }
+                                e[jj] = lo | ((unsigned)hi << 16);
+                            }
+                            *reinterpret_cast<uint4 *>(dst + ks * 64 + pt * 32 + hh * 16) =
+                                make_uint4(e[0], e[1], e[2], e[3]);
+                        }
+                }
+            }
+            if (tail == 8) {
+#pragma unroll
+                for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+                    for (int hh = 0; hh < 2; ++hh) {
+                        unsigned e[2];
+#pragma unroll
+                        for (int jj = 0; jj < 2; ++jj) {
+                            unsigned short lo = 0, hi = 0;
+                            // full16 <= 2 when R <= 40: ring indices below are static per full16
+#pragma unroll
+                            for (int f16 = 0; f16 <= 2; ++f16) {
+                                if (f16 != full16) continue;
+                                const int i0 = 16 * f16 + 4 * hh + 2 * jj;
+                                if (i0 < kFragMaxRings && i0 < R) {
+                                    const unsigned v = sp[i0][h];
+                                    lo = frag16<BF16>(pt ? h2hi(v) * gf : (h2lo(v) - dc) * gf);
+                                }
+                                if (i0 + 1 < kFragMaxRings && i0 + 1 < R) {
+                                    const unsigned v = sp[i0 + 1][h];
+                                    hi = frag16<BF16>(pt ? h2hi(v) * gf : (h2lo(v) - dc) * gf);
+                                }
+                            }
+                            e[jj] = lo | ((unsigned)hi << 16);
+                        }
+                        *reinterpret_cast<uint2 *>(dst + full16 * 64 + pt * 16 + hh * 8) =
+                            make_uint2(e[0], e[1]);
+                    }
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(128) void k_ref_finalize(float2 *__restrict__ rspec,
                                                       const float2 *__restrict__ rnorm, int R,
                                                       float *__restrict__ rsum0) {
@@ -1723,6 +1947,12 @@ hipError_t prepare_launches() {
         e = hipFuncSetAttribute((const void *)k_polar_spectra<true>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, lds);
         if (e != hipSuccess) return e;
+        e = hipFuncSetAttribute((const void *)k_polar_frag<false>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        if (e != hipSuccess) return e;
+        e = hipFuncSetAttribute((const void *)k_polar_frag<true>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        if (e != hipSuccess) return e;
         lds_raised = true;
     }
     return hipSuccess;
@@ -1738,6 +1968,29 @@ hipError_t launch_polar_spectra(const float *imgs, int n_img, int img_dim, Shift
     dim3 grid(n_img);
     hipLaunchKernelGGL(k_polar_spectra<false>, grid, dim3(256), lds, st, imgs, img_dim, acc,
                        shift_i, n_shift, tmpl, n_ring, twiddle, spec, norm, nullptr);
+    return hipGetLastError();
+}
+
+bool polar_frag_supported(int n_ring) { return n_ring >= 1 && n_ring <= kFragMaxRings; }
+
+hipError_t launch_polar_frag(const float *imgs, int n_img, int img_dim, ShiftView acc,
+                             const int2 *shift_i, int n_shift, int n_ring, bool bf16,
+                             const float2 *twiddle, void *frag, hipStream_t st) {
+    if (n_img <= 0 || n_shift <= 0) return hipSuccess;
+    if (!polar_frag_supported(n_ring)) return hipErrorInvalidValue;
+    const GemmLayout L = gemm_layout(n_ring);
+    const int S_pad = gemm_rows_per_particle(n_shift);
+    const size_t lds = (256 + 4 * 512) * sizeof(float2) + (size_t)(img_dim + 2) * (img_dim + 2) * sizeof(float);
+    if (hipError_t e = prepare_launches()) return e;
+    const dim3 grid((unsigned)((n_img + 15) / 16 * 16));
+    if (bf16)
+        hipLaunchKernelGGL(k_polar_frag<true>, grid, dim3(256), lds, st, imgs, n_img, img_dim, acc,
+                           shift_i, n_shift, S_pad, n_ring, L.full16, L.tail, L.slot_bytes, L.cp,
+                           twiddle, (unsigned char *)frag);
+    else
+        hipLaunchKernelGGL(k_polar_frag<false>, grid, dim3(256), lds, st, imgs, n_img, img_dim, acc,
+                           shift_i, n_shift, S_pad, n_ring, L.full16, L.tail, L.slot_bytes, L.cp,
+                           twiddle, (unsigned char *)frag);
     return hipGetLastError();
 }
 

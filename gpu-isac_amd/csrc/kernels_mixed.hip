@@ -1,45 +1,61 @@
 // Mixed-precision (MFMA) form of the multi-reference search for CDNA4 (gfx950).
 //
-// Ring contraction: FP16 or BF16 operands on v_mfma_f32_4x4x4 (16 blocks), FP32
-// accumulation.  Each instruction batches 16 spectrum slots (one per 4x4 block):
-// block rows = 2 shift rows x (re, im) of the particle spectrum, block columns =
-// 2 references x (w*Re, w*Im), K = 4 rings.  The 4x4 block holds the four real
-// products P1..P4 of 4 (row, reference) pairs at that slot.
+// Ring contraction as a batched GEMM, one per spectrum slot k.  With the
+// particle operand a = sr + si, b = sr - si (normalised spectrum S of one shift
+// row) and the reference operand u = rr + ri, v = rr - ri (normalised,
+// ring-weighted spectrum R), the four real products over the rings are exactly
+// the inputs of the psi transform:
+//     Z[k]       = C0[k] + i C1[k]                         = (a.u) + i (b.u)
+//     Z[256 - k] = conj(C0[k]) + i conj(C1[k])              = (b.v) + i (a.v)
+// where C0 = sum_i w_i R conj(S), C1 = sum_i w_i conj(R S) are the reference's
+// two mirror cross-spectra (cuda/gpu_aln_noref.cu:1508-1536) and z = IDFT(Z)
+// holds c0[t] + i c1[t], the psi CCF of both mirrors (:2393, :2459).  Slot 0
+// packs the two real bins (X[0], X[128]): a = S[0], b = S[128], u = R[0],
+// v = R[128], so Z[0] = (a.u)(1 + i) and Z[128] = (b.v)(1 + i).
 //
-// Psi transform: the 256-point inverse DFT of Z = C0 + i C1 runs on
-// v_mfma_f32_16x16x16_f16 as DFT16 (over g, k = b + 16 g) -> twiddle
-// e^{+2 pi i b t1 / 256} -> DFT16 (over b); the accumulator layout of the first
-// stage (lane l: b = 4(l/16)+v, t1 = l%16) is exactly the A-operand layout of the
-// second, so no data moves between the two stages.  Output: lane l holds
-// z[t1 + 16 t2] for t1 = 4(l/16)+v, t2 = l%16.
+// Search kernel (k_mref_gemm): workgroup = 16 particles x 16 references, four
+// wavefronts.  For each block of 16 consecutive (particle, shift) rows:
+//  * contraction: per slot one 32 x 32 output tile (rows = 16 rows x {a, b},
+//    columns = 16 references x {u, v}) on v_mfma_f32_32x32x16 (+ 32x32x8 for
+//    the last rings), K = rings; each wavefront computes a quarter of the slots;
+//  * the tiles are converted to FP16 Z values and transposed through LDS into
+//    the psi operand layout, which every wavefront keeps in registers for its
+//    64 (row, reference) pairs (4 VGPRs per pair);
+//  * psi transform per pair: the 256-point inverse DFT as DFT16 -> twiddle ->
+//    DFT16 on v_mfma_f32_16x16x32_f16 (k = 16 k1 + k2, t = t1 + 16 t2); the
+//    first stage's accumulator layout is the second stage's B operand;
+//  * search: each psi sample carries its (shift, sample) index in the low
+//    mantissa bits (first-max-wins order: a smaller index gets a larger code),
+//    so the running maximum per (reference, mirror) is one v_max3 per three
+//    samples; a wave-wide reduction per particle decodes the winner.
+// The values are approximate (FP16/BF16 operands, FP16 psi transform, and the
+// index bits perturb a value by at most 2^-(23 - index bits) relative); the
+// best candidates are re-scored exactly in FP32 (k_ccf_cand4, kernels.hip) and
+// near-ties in FP64 (kernels.hip).  The approximate records carry the peak
+// sample only (no sub-sample refinement).
 //
-// Scaling: particle operands are normalised spectra (the DC correction already
-// applied); reference operands carry the ring weight times 1/R ("per-ring"
-// scaling, undone by multiplying the accumulator by R).  Before the FP16 psi
-// transform each (row, reference) pair is scaled by a power of two chosen from
-// the Cauchy-Schwarz bound |C[k]| <= ||S||_F ||wR||_F so that |Z| <= 2048 and the
-// DFT16 intermediate stays below the FP16 maximum.
-//
-// The results are approximate (FP16/BF16 rounding); candidates are re-scored
-// exactly in FP32 by k_ccf_cand4 (kernels.hip) and near-ties in FP64 here.
+// Scaling: the particle operands are the normalised spectra times a global
+// power of two c_p; each reference's operands are scaled by a power of two c_n
+// so that its Frobenius norm lies in [0.7, 1.42).  For a normalised row the
+// Frobenius norm of (a, b) is at most 256 sqrt(R (R + 1)), so |Z|, and the
+// DFT16 intermediates, are bounded by c_p 256 sqrt(R (R + 1)) 1.42, which c_p
+// keeps below 32768 (FP16 max 65504).  Results are multiplied by 1 / (c_p c_n).
 #include "isac_internal.h"
 
+#include <cfloat>
 #include <climits>
 
 namespace isac {
 namespace {
 
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef float float16_t __attribute__((ext_vector_type(16)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kWave = 64;
-
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 __device__ __forceinline__ unsigned short f32_to_bf16(float x) {
     unsigned u = __float_as_uint(x);
@@ -47,9 +63,49 @@ __device__ __forceinline__ unsigned short f32_to_bf16(float x) {
     return (unsigned short)(u >> 16);
 }
 
-__device__ __forceinline__ unsigned short f32_to_f16(float x) {
-    _Float16 h = (_Float16)x;
-    return __builtin_bit_cast(unsigned short, h);
+__device__ __forceinline__ unsigned short f32_to_h(float x) {
+    return __builtin_bit_cast(unsigned short, (_Float16)x);
+}
+
+template <bool BF16>
+__device__ __forceinline__ unsigned short to16(float x) {
+    return BF16 ? f32_to_bf16(x) : f32_to_h(x);
+}
+
+__device__ __forceinline__ unsigned pack_h2(float lo, float hi) {
+    return (unsigned)f32_to_h(lo) | ((unsigned)f32_to_h(hi) << 16);
+}
+
+template <bool BF16>
+__device__ __forceinline__ float16_t mma32x16(uint4 a, uint4 b, float16_t c) {
+    if (BF16)
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, a),
+                                                       __builtin_bit_cast(bf16x8_t, b), c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(half8_t, a),
+                                                  __builtin_bit_cast(half8_t, b), c, 0, 0, 0);
+}
+
+template <bool BF16>
+__device__ __forceinline__ float16_t mma32x8(uint2 a, uint2 b, float16_t c) {
+    if (BF16)
+        return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(__builtin_bit_cast(short4_t, a),
+                                                        __builtin_bit_cast(short4_t, b), c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_32x32x8f16(__builtin_bit_cast(half4_t, a),
+                                                __builtin_bit_cast(half4_t, b), c, 0, 0, 0);
+}
+
+__device__ __forceinline__ float4_t mma16x32(uint4 a, uint4 b, float4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(half8_t, a),
+                                                  __builtin_bit_cast(half8_t, b), c, 0, 0, 0);
+}
+
+// max of three finite values without the NaN-quieting copies that fmaxf inserts
+// for operands assembled from bits (the embedded psi values are finite by the
+// scaling; -inf only initialises the running maxima)
+__device__ __forceinline__ float vmax3(float a, float b, float c) {
+    float d;
+    asm("v_max3_f32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
 }
 
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
@@ -58,423 +114,592 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
     return (xcd < r) ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
 }
 
-__device__ __forceinline__ void wave_argmax(float &v, int &t) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        const float ov = __shfl_xor(v, off, kWave);
-        const int ot = __shfl_xor(t, off, kWave);
-        if (ov > v || (ov == v && ot < t)) {
-            v = ov;
-            t = ot;
-        }
-    }
-}
+// ----------------------------------------------------------- operand layout
+// Rows (particle shift rows or references) are contiguous: row r, slot k at
+// r * row_bytes + k * slot_bytes, slot_bytes = 4 K_pad (two parts, K_pad rings,
+// 16-bit).  Inside a slot, each K-step of 16 rings is 64 bytes: part (a / b or
+// u / v) x half h, 16 bytes = rings 16 ks + 8 h .. + 7; a tail of 8 rings is 32
+// bytes: part x h, 8 bytes = rings 16 ks + 4 h .. + 3.  An MFMA operand of slot
+// k, step ks for a block of 16 rows is then lane l = 2 idx + part + 32 h ->
+// row idx's 16 (or 8) bytes at part / h: four lanes read 64 contiguous bytes.
+struct GemmGeom {
+    int full16;      // number of 16-ring steps
+    int tail;        // 0, 8 or 16 rings in the last step
+    int slot_bytes;  // 4 K_pad
+};
 
-// FP32 spectra -> MFMA operand layout [row][q][slot][part][4 rings] (16-bit), rings
-// padded with zeros to 4Q.  Particles: normalisation applied (g, DC correction).
-// References: the input is already normalised and ring-weighted; scaled by 1/R.
-// Also writes the Frobenius norm of the (unscaled) spectrum of each row.
-template <bool BF16>
-__global__ __launch_bounds__(256) void k_spec_to_mixed(const float2 *__restrict__ spec,
-                                                       const float2 *__restrict__ norm, int n_rows,
-                                                       int R, int Q, int is_ref,
-                                                       ushort4 *__restrict__ out,
-                                                       float *__restrict__ row_norm) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int row = blockIdx.x * 4 + wave;
-    if (row >= n_rows) return;
-    const float2 nm = is_ref ? make_float2(0.0f, 1.0f) : norm[row];
-    const float post = is_ref ? 1.0f / (float)R : 1.0f;
-    // one pass over the rings: slots lane and lane + 64 share one 16-byte word
-    // (slot_pos); the squared norm of each slot half accumulates on its own
-    double ssh[2] = {0.0, 0.0};
-    for (int q = 0; q < Q; ++q) {
-        unsigned short re[2][4], im[2][4];
+// Write the 2 x K_pad operand values of one row at slot k: val(part, i) for i < R,
+// zero beyond.  row_base points at the row.
+template <bool BF16, class F>
+__device__ __forceinline__ void write_operand(unsigned char *row_base, int k, int R, GemmGeom g,
+                                              F val) {
+    unsigned char *base = row_base + (size_t)k * g.slot_bytes;
+    const int n16 = g.full16 + (g.tail == 16 ? 1 : 0);
+    for (int ks = 0; ks < n16; ++ks) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int i = 4 * q + e;
-            float4 v4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (i < R) v4 = reinterpret_cast<const float4 *>(spec + ((size_t)row * R + i) * kSlots)[lane];
+        for (int part = 0; part < 2; ++part) {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                float2 x = h ? make_float2(v4.z, v4.w) : make_float2(v4.x, v4.y);
-                if (i < R && !is_ref) {
-                    x.x = (lane == 0 && h == 0) ? (x.x - nm.x) * nm.y : x.x * nm.y;
-                    x.y *= nm.y;
+                unsigned short e[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int i = 16 * ks + 8 * h + j;
+                    e[j] = to16<BF16>(i < R ? val(part, i) : 0.0f);
                 }
-                ssh[h] += (double)x.x * x.x + (double)x.y * x.y;
-                x.x *= post;
-                x.y *= post;
-                re[h][e] = BF16 ? f32_to_bf16(x.x) : f32_to_f16(x.x);
-                im[h][e] = BF16 ? f32_to_bf16(x.y) : f32_to_f16(x.y);
+                uint4 v;
+                v.x = e[0] | ((unsigned)e[1] << 16);
+                v.y = e[2] | ((unsigned)e[3] << 16);
+                v.z = e[4] | ((unsigned)e[5] << 16);
+                v.w = e[6] | ((unsigned)e[7] << 16);
+                *reinterpret_cast<uint4 *>(base + ks * 64 + part * 32 + h * 16) = v;
             }
         }
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            ushort4 *o = out + (((size_t)row * Q + q) * kSlots + lane + 64 * h) * 2;
-            o[0] = make_ushort4(re[h][0], re[h][1], re[h][2], re[h][3]);
-            o[1] = make_ushort4(im[h][0], im[h][1], im[h][2], im[h][3]);
-        }
     }
-    double ss = ssh[0] + ssh[1];
+    if (g.tail == 8) {
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) ss += __shfl_xor(ss, off, kWave);
-    if (lane == 0) row_norm[row] = (float)sqrt(ss);
-}
-
-template <bool BF16>
-__device__ __forceinline__ float4_t mfma4(uint2 a, uint2 b, float4_t c) {
-    if (BF16)
-        return __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(__builtin_bit_cast(short4_t, a),
-                                                      __builtin_bit_cast(short4_t, b), c, 0, 0, 0);
-    return __builtin_amdgcn_mfma_f32_4x4x4f16(__builtin_bit_cast(half4_t, a),
-                                              __builtin_bit_cast(half4_t, b), c, 0, 0, 0);
-}
-
-__device__ __forceinline__ float4_t mfma16(half4_t a, half4_t b, float4_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
-}
-
-__device__ __forceinline__ half4_t to_h4(float a, float b, float c, float d) {
-    half4_t h;
-    h[0] = (_Float16)a;
-    h[1] = (_Float16)b;
-    h[2] = (_Float16)c;
-    h[3] = (_Float16)d;
-    return h;
-}
-
-// One workgroup = one particle x 16 references (4 wavefronts x 4 references).
-// For each block of 4 shift rows the particle operands are staged in LDS in
-// double-buffered chunks of kQChunk ring-quads with direct global->LDS loads
-// (one wavefront instruction moves 1 KB = 64 slots x (re, im) of one row);
-// reference operands come through the vector L1.
-constexpr int kQChunk = 3;
-
-// Position of Z-form slot u (0..63) of pair pr in a wavefront's zre / zim: pairs
-// 65 apart and bit 1 of slots 32..63 flipped.
-__device__ __forceinline__ int zpos(int pr, int u) { return pr * 65 + (u ^ (((u >> 5) & 1) << 1)); }
-
-template <bool BF16>
-__global__ __launch_bounds__(256, 2) void k_mref_mixed(
-    const uint2 *__restrict__ pop, const float *__restrict__ pnorm, const uint2 *__restrict__ rop,
-    const float *__restrict__ rnorm, int n_img, int S, int Q, int N, int G, int GB, float rscale,
-    PeakRec *__restrict__ rec) {
-    // staged particle operands: [buf][qq][row 0..3][slot 0..127][part] uint2
-    // rows padded by 16 uint2 (128 bytes): the two rows one MFMA operand read
-    // touches fall in different bank halves
-    __shared__ uint2 abuf[2][kQChunk][4][kSlots * 2 + 16];
-    // [wave][pair * 65 + zslot(b*16+g)] fp16 bits, 4 per ushort4 (stride 65 and the
-    // slot swizzle keep the Z-form writes and the stage-1 reads free of bank conflicts)
-    __shared__ ushort4 zre[4][4 * 65];
-    __shared__ ushort4 zim[4][4 * 65];
-    __shared__ float4 zc[4][64];        // psi samples of one mirror for neighbour capture
-    __shared__ float nbuf[4][4][2][8];
-    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lb = xcd_remap(blockIdx.x, gridDim.x);
-    const int p = lb / GB;
-    if (p >= n_img) return;   // uniform over the workgroup
-    const int g4 = (lb % GB) * 4 + wave;
-    const bool active = g4 < G;
-
-    int nref[4];
-    bool nval[4];
-    float rn[4];
+        for (int part = 0; part < 2; ++part) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const int n = g4 * 4 + c;
-        nval[c] = active && n < N;
-        nref[c] = min(n, N - 1);
-        rn[c] = rnorm[nref[c]];
-    }
-    // DFT16 operands (inverse): lane l holds B[k = 4(l/16)+v][j = l%16]
-    half4_t Bc, Bs, Bns;
-    float tw_re[4], tw_im[4];
-    {
-        const int j = lane & 15, kb = 4 * (lane >> 4);
+            for (int h = 0; h < 2; ++h) {
+                unsigned short e[4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int m = ((kb + v) * j) & 15;
-            float sn, cs;
-            __sincosf(6.283185307179586f * (float)m / 16.0f, &sn, &cs);
-            Bc[v] = (_Float16)cs;
-            Bs[v] = (_Float16)sn;
-            Bns[v] = (_Float16)(-sn);
-            // stage-2 twiddle for b = kb + v, t1 = j: e^{+2 pi i b t1 / 256}
-            const float a = 6.283185307179586f * (float)(((kb + v) * j) & 255) / 256.0f;
-            __sincosf(a, &tw_im[v], &tw_re[v]);
-        }
-    }
-    float bv[4][2];
-    int bf[4][2];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        bv[c][0] = bv[c][1] = -3.40282e+38f;
-        bf[c][0] = bf[c][1] = INT_MAX;
-    }
-    const int blk = lane >> 2, jj = lane & 3;
-    const size_t row_stride = (size_t)Q * kSlots * 2;   // uint2 per (image, shift)
-    const int nqc = (Q + kQChunk - 1) / kQChunk;
-    const uint2 *brow[2];
-#pragma unroll
-    for (int cp = 0; cp < 2; ++cp)
-        brow[cp] = rop + (size_t)nref[2 * cp + (jj >> 1)] * row_stride + (jj & 1);
-
-    for (int s0 = 0; s0 < S; s0 += 4) {
-        const uint2 *rows[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-            rows[r] = pop + ((size_t)p * S + min(s0 + r, S - 1)) * row_stride;
-        // one chunk = kQChunk ring-quads x 4 rows x 2 halves (1 KB each)
-        auto load_chunk = [&](int qc, int buf) {
-            for (int u = wave; u < kQChunk * 4 * 2; u += 4) {
-                const int qq = u >> 3, r = (u >> 1) & 3, half = u & 1;
-                const int q = qc * kQChunk + qq;
-                if (q >= Q) continue;
-                __builtin_amdgcn_global_load_lds(
-                    (const void *)(rows[r] + (size_t)q * kSlots * 2 + half * kSlots + 2 * lane),
-                    (__attribute__((address_space(3))) void *)(&abuf[buf][qq][r][half * kSlots]),
-                    16, 0, 0);
+                for (int j = 0; j < 4; ++j) {
+                    const int i = 16 * g.full16 + 4 * h + j;
+                    e[j] = to16<BF16>(i < R ? val(part, i) : 0.0f);
+                }
+                uint2 v;
+                v.x = e[0] | ((unsigned)e[1] << 16);
+                v.y = e[2] | ((unsigned)e[3] << 16);
+                *reinterpret_cast<uint2 *>(base + g.full16 * 64 + part * 16 + h * 8) = v;
             }
-        };
-        float4_t acc[8][2][2];
-#pragma unroll
-        for (int g = 0; g < 8; ++g)
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b) acc[g][a][b] = float4_t{0.f, 0.f, 0.f, 0.f};
-        load_chunk(0, 0);
-        __builtin_amdgcn_s_waitcnt(0);
+        }
+    }
+}
+
+// Particle operands from the raw FP32 spectra: thread = (operand row, slot).
+// Operand row p * S_pad + s holds spectra row p * S + s; rows with s >= S or
+// p >= n_img (padding) are zero.
+template <bool BF16>
+__global__ __launch_bounds__(256) void k_frag_rows(const float2 *__restrict__ spec,
+                                                   const float2 *__restrict__ norm, int n_img,
+                                                   int S, int S_pad, int n_rows_pad, int R,
+                                                   GemmGeom g, float cp,
+                                                   unsigned char *__restrict__ frag) {
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int orow = (int)(tid / kSlots), k = (int)(tid % kSlots);
+    if (orow >= n_rows_pad) return;
+    unsigned char *rowp = frag + (size_t)orow * kSlots * g.slot_bytes;
+    const int p = orow / S_pad, s = orow - p * S_pad;
+    if (p >= n_img || s >= S) {
+        write_operand<BF16>(rowp, k, R, g, [](int, int) { return 0.0f; });
+        return;
+    }
+    const int row = p * S + s;
+    const float2 nm = norm[row];
+    const float gs = nm.y * cp;
+    const float2 *src = spec + (size_t)row * R * kSlots + slot_pos(k);
+    write_operand<BF16>(rowp, k, R, g, [&](int part, int i) {
+        const float2 x = src[(size_t)i * kSlots];
+        if (k == 0) return part ? x.y * gs : (x.x - nm.x) * gs;   // S[0] - mu L, S[128]
+        return part ? (x.x - x.y) * gs : (x.x + x.y) * gs;
+    });
+}
+
+// Per-reference scale: c_n = 2^-round(log2 ||(u, v)||_F).  Block = one reference.
+__global__ __launch_bounds__(256) void k_ref_scale(const float2 *__restrict__ rspec, int R,
+                                                   float cp, float *__restrict__ cn,
+                                                   float *__restrict__ rinv) {
+    const int n = blockIdx.x;
+    double ss = 0.0;
+    for (int e = threadIdx.x; e < R * kSlots; e += blockDim.x) {
+        const float2 x = rspec[(size_t)n * R * kSlots + e];
+        const int k = e % kSlots;   // slot_pos(0) == 0: position 0 is slot 0
+        if (k == 0) ss += (double)x.x * x.x + (double)x.y * x.y;
+        else ss += 2.0 * ((double)x.x * x.x + (double)x.y * x.y);
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = ss;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
         __syncthreads();
-        for (int qc = 0; qc < nqc; ++qc) {
-            const int buf = qc & 1;
-            if (qc + 1 < nqc) load_chunk(qc + 1, buf ^ 1);
-            const int q1 = min(Q, (qc + 1) * kQChunk);
-            for (int q = qc * kQChunk; q < q1; ++q) {
-                const int qq = q - qc * kQChunk;
-                const uint2 *ar0 = &abuf[buf][qq][jj >> 1][(jj & 1)];
-                const uint2 *ar1 = &abuf[buf][qq][2 + (jj >> 1)][(jj & 1)];
-                // the quad's reference operands are all issued before its first LDS
-                // read (the compiler waits for every outstanding global load there),
-                // so the quad pays one round trip instead of one per slot block
-                uint2 bq[8][2];
+    }
+    if (threadIdx.x == 0) {
+        const double f = sqrt(red[0]);
+        const float c = (f > 0.0) ? ldexpf(1.0f, -(int)lrint(log2(f))) : 1.0f;
+        cn[n] = c;
+        rinv[n] = 1.0f / (c * cp);
+    }
+}
+
+// Reference operands from the normalised, ring-weighted spectra: thread = (ref, slot);
+// references at or beyond n_ref (padding of the last block) are zero.
+template <bool BF16>
+__global__ __launch_bounds__(256) void k_frag_refs(const float2 *__restrict__ rspec, int n_ref,
+                                                   int n_ref_pad, int R, GemmGeom g,
+                                                   const float *__restrict__ cn,
+                                                   unsigned char *__restrict__ frag) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = tid / kSlots, k = tid % kSlots;
+    if (n >= n_ref_pad) return;
+    unsigned char *rowp = frag + (size_t)n * kSlots * g.slot_bytes;
+    if (n >= n_ref) {
+        write_operand<BF16>(rowp, k, R, g, [](int, int) { return 0.0f; });
+        return;
+    }
+    const float c = cn[n];
+    const float2 *src = rspec + (size_t)n * R * kSlots + slot_pos(k);
+    write_operand<BF16>(rowp, k, R, g, [&](int part, int i) {
+        const float2 x = src[(size_t)i * kSlots];
+        if (k == 0) return part ? x.y * c : x.x * c;              // R[0], R[128]
+        return part ? (x.x - x.y) * c : (x.x + x.y) * c;
+    });
+}
+
+// ------------------------------------------------------------ search kernel
+// psi operand layout of a pair (16x16x32 A operand, K = (k1, re/im)): lane l =
+// k2 + 16 lg holds in register j the Z value at k1 = k1_of(lg, j).  A chunk c of
+// 32 slots fills register j = c of every lane except lane 32 (k2 = 0, lg 2),
+// whose four values (Z[224], Z[192], Z[160], Z[128]) go through a separate buffer.
+__host__ __device__ constexpr int k1_of(int lg, int j) {
+    return lg == 0 ? 2 * j : lg == 1 ? 2 * j + 1 : lg == 2 ? 14 - 2 * j : 15 - 2 * j;
+}
+
+// Workgroup: 8 wavefronts (two per SIMD); wavefront w owns the pairs (row r,
+// reference 2 w + q), r < 16, q < 2, and computes the contraction tiles of 16
+// of the 128 slots of each row block (4 per chunk of 32).
+constexpr int kGemmWaves = 8;
+constexpr int kGemmThreads = 64 * kGemmWaves;
+constexpr int kQW = 16 / kGemmWaves;   // references per wavefront
+constexpr int kRing = 2;                // slots of operand fragments in flight per wavefront
+
+// LDS swizzle of a pair's 64 consumer lanes: owner wave and bit 1 of the row
+// (spreads the 32 same-position writes of a tile over distinct banks).
+__device__ __forceinline__ int zswz(int wo, int row) { return wo | (((row >> 1) & 1) << 3); }
+
+// One slot's operand fragments: two 16-ring steps and a tail of 8 rings (the
+// layout of any R <= 40; larger R loops over further 16-ring steps).
+struct Frag {
+    uint4 a0, b0, a1, b1;
+    uint2 a8, b8;
+};
+
+// Buffer resource of a byte range (raw buffer loads: 32-bit lane offset in a VGPR,
+// uniform offset in an SGPR, no 64-bit per-lane addresses).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t byte_rsrc(const unsigned char *base,
+                                                           unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<unsigned char *>(base), (short)0,
+                                             (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ uint4 bload16(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+
+__device__ __forceinline__ uint2 bload8(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+// One slot's fragments: pa / pb = uniform byte offsets of the slot in the particle
+// and reference resources.
+__device__ __forceinline__ void load_frag(Frag &f, __amdgpu_buffer_rsrc_t ra, unsigned pa,
+                                          __amdgpu_buffer_rsrc_t rbr, unsigned pb, unsigned off16,
+                                          unsigned off8, GemmGeom g) {
+    if (g.full16 > 0) {
+        f.a0 = bload16(ra, off16, pa);
+        f.b0 = bload16(rbr, off16, pb);
+    }
+    if (g.full16 > 1 || (g.full16 == 1 && g.tail == 16)) {
+        f.a1 = bload16(ra, off16, pa + 64);
+        f.b1 = bload16(rbr, off16, pb + 64);
+    }
+    if (g.tail == 8) {
+        f.a8 = bload8(ra, off8, pa + g.full16 * 64);
+        f.b8 = bload8(rbr, off8, pb + g.full16 * 64);
+    }
+}
+
+template <bool BF16>
+__device__ __forceinline__ float16_t contract(const Frag &f, __amdgpu_buffer_rsrc_t ra, unsigned pa,
+                                              __amdgpu_buffer_rsrc_t rbr, unsigned pb,
+                                              unsigned off16, GemmGeom g) {
+    float16_t acc = {};
+    if (g.full16 > 0) acc = mma32x16<BF16>(f.a0, f.b0, acc);
+    if (g.full16 > 1 || (g.full16 == 1 && g.tail == 16)) acc = mma32x16<BF16>(f.a1, f.b1, acc);
+    // rings beyond 32 (R > 40): further steps, loaded here
+    for (int ks = 2; ks < g.full16 + (g.tail == 16 ? 1 : 0); ++ks)
+        acc = mma32x16<BF16>(bload16(ra, off16, pa + ks * 64), bload16(rbr, off16, pb + ks * 64), acc);
+    if (g.tail == 8) acc = mma32x8<BF16>(f.a8, f.b8, acc);
+    return acc;
+}
+
+template <bool BF16>
+__global__ __launch_bounds__(kGemmThreads, 1) void k_mref_gemm(
+    const unsigned char *__restrict__ pfrag, const unsigned char *__restrict__ rfrag,
+    const float *__restrict__ rinv, int P, int S, int S_pad, int N, GemmGeom g, int nbits,
+    PeakRec *__restrict__ rec) {
+    // [buffer][owner wave][row][consumer lane ^ swizzle] -> the owner's kQW pair
+    // values (u32 = FP16 (re, im) each)
+    __shared__ uint2 zc[2][kGemmWaves][16][64];
+    // [row-block parity][owner wave][row * kQW + q] -> lane 32's four values
+    __shared__ uint4 zl[2][kGemmWaves][16 * kQW];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // 1D grid, logical block = particle group x reference block (fastest): the
+    // reference blocks of a particle group are consecutive and share an XCD / L2
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int NB = (N + 15) / 16;
+    const int pg = lb / NB, nb = lb % NB;
+    const unsigned row_bytes = (unsigned)kSlots * g.slot_bytes;
+    const unsigned blk_bytes = 16 * row_bytes;
+    // the workgroup's particle rows and its reference block as buffer resources
+    const __amdgpu_buffer_rsrc_t rsp = byte_rsrc(pfrag + (size_t)pg * S_pad * blk_bytes,
+                                                 (unsigned)S_pad * blk_bytes);
+    const __amdgpu_buffer_rsrc_t rsr = byte_rsrc(rfrag + (size_t)nb * blk_bytes, blk_bytes);
+    // this lane's operand bytes in a block: row (l % 32) / 2, part l % 2, half l / 32
+    // (32-bit: a block is at most 16 * 128 * 4 * 128 bytes)
+    const unsigned off16 = ((lane & 31) >> 1) * row_bytes + (lane & 1) * 32 + (lane >> 5) * 16;
+    const unsigned off8 = ((lane & 31) >> 1) * row_bytes + (lane & 1) * 16 + (lane >> 5) * 8;
+
+    // ---- constant MFMA operands of the psi transform, kept in LDS (read per row
+    // block: their registers serve the fragment pipeline during the contraction)
+    __shared__ uint4 pcst[4][64];     // B1 re, B1 im, A2 re, A2 im per lane
+    __shared__ float4 ptw[2][64];     // twiddle cos, sin per lane
+    if (w == 0) {
+        const int c16 = lane & 15, lg = lane >> 4;
+        unsigned short e1r[8], e1i[8], e2r[8], e2i[8];
 #pragma unroll
-                for (int g = 0; g < 8; ++g) {
-                    const size_t off = ((size_t)q * kSlots + 16 * g + blk) * 2;
-                    bq[g][0] = brow[0][off];
-                    bq[g][1] = brow[1][off];
+        for (int e = 0; e < 8; ++e) {
+            const int part = e & 1;
+            // stage 1: B[(k1, part)][t1], t1 = c16
+            const int k1 = k1_of(lg, e >> 1);
+            float s1, c1;
+            sincospif((float)((k1 * c16) & 15) / 8.0f, &s1, &c1);
+            e1r[e] = f32_to_h(part ? -s1 : c1);
+            e1i[e] = f32_to_h(part ? c1 : s1);
+            // stage 2: A[t2][(k2, part)], t2 = c16, k2 = 4 lg + e / 2
+            const int k2 = 4 * lg + (e >> 1);
+            float s2, c2;
+            sincospif((float)((k2 * c16) & 15) / 8.0f, &s2, &c2);
+            e2r[e] = f32_to_h(part ? -s2 : c2);
+            e2i[e] = f32_to_h(part ? c2 : s2);
+        }
+        auto pk = [](const unsigned short *e) {
+            return make_uint4(e[0] | ((unsigned)e[1] << 16), e[2] | ((unsigned)e[3] << 16),
+                              e[4] | ((unsigned)e[5] << 16), e[6] | ((unsigned)e[7] << 16));
+        };
+        pcst[0][lane] = pk(e1r);
+        pcst[1][lane] = pk(e1i);
+        pcst[2][lane] = pk(e2r);
+        pcst[3][lane] = pk(e2i);
+        // twiddle e^{+2 pi i k2 t1 / 256} of the stage-1 outputs (k2 = 4 lg + v, t1 = c16)
+        float twc[4], tws[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            sincospif((float)(((4 * lg + v) * c16) & 255) / 128.0f, &tws[v], &twc[v]);
+        ptw[0][lane] = make_float4(twc[0], twc[1], twc[2], twc[3]);
+        ptw[1][lane] = make_float4(tws[0], tws[1], tws[2], tws[3]);
+    }
+    __syncthreads();
+
+    // ---- search state: running maximum per (reference, mirror), index in the low bits
+    const unsigned lowmask = (1u << nbits) - 1u;
+    const int code_top = 4 * S - 1;
+    // rmA: the particle of the block's first row; rmB: the next particle, whose
+    // rows may start inside the block (S_pad >= 16: at most two per block)
+    float rmA[kQW][2], rmB[kQW][2];
+#pragma unroll
+    for (int q = 0; q < kQW; ++q) rmA[q][0] = rmA[q][1] = rmB[q][0] = rmB[q][1] = -INFINITY;
+
+    uint4 za[16 * kQW];
+#pragma unroll
+    for (int i = 0; i < 16 * kQW; ++i) za[i] = make_uint4(0u, 0u, 0u, 0u);
+
+    unsigned *zc32 = reinterpret_cast<unsigned *>(&zc[0][0][0][0]);
+    unsigned *zl32 = reinterpret_cast<unsigned *>(&zl[0][0][0]);
+
+    // fragment pipeline: a ring of kRing slots.  An entry is refilled with the slot
+    // kRing positions later (wrapping into the next row block) right after its
+    // contraction, so kRing - 1 to kRing slots of operands are in flight.
+    Frag fr[kRing];
+    auto slot_of = [](int i, int wv) { return 32 * (i >> 2) + 4 * wv + (i & 3); };   // i-th of 16
+#pragma unroll
+    for (int s0 = 0; s0 < kRing; ++s0) {
+        const unsigned k = slot_of(s0, w);
+        load_frag(fr[s0], rsp, k * g.slot_bytes, rsr, k * g.slot_bytes, off16, off8, g);
+    }
+
+    for (int jb = 0; jb < S_pad; ++jb) {
+        const size_t rb = (size_t)pg * S_pad + jb;
+        const unsigned pblk = (unsigned)jb * blk_bytes;
+        const int par = jb & 1;
+        // opaque copies: what the slots and lanes select is recomputed per row block
+        // instead of being kept live across the loop (16 slots x lane values)
+        int wv = w, ln = lane;
+        asm volatile("" : "+s"(wv));
+        asm volatile("" : "+v"(ln));
+        const int col = ln & 31, nl = col >> 1, uv = col & 1, hw = ln >> 5;
+        const int wo = nl / kQW, qo = nl % kQW;
+        // every row this lane writes has (row >> 1) & 1 == hw: one swizzle per lane
+        const int sw = wo | (hw << 3);
+        // lane part of the LDS addresses (u32 units); rows add compile-time offsets
+        const int zc_lane = ((wo * 16 + 2 * hw) * 64) * kQW + qo;
+        const int zl_lane = ((wo * (16 * kQW)) + 2 * hw * kQW + qo) * 4;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int buf = c & 1;
+            // ---- produce: slots 32 c + 4 w + s
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int i = 4 * c + s;
+                const int k = slot_of(i, wv);
+                const float16_t acc = contract<BF16>(fr[i % kRing], rsp, pblk + k * g.slot_bytes,
+                                                     rsr, k * g.slot_bytes, off16, g);
+                // refill this ring entry: slot i + kRing of this row block or the next
+                {
+                    const bool wrap = (i + kRing >= 16);
+                    if (!wrap || jb + 1 < S_pad) {
+                        const unsigned pn = wrap ? pblk + blk_bytes : pblk;
+                        const unsigned kn = slot_of((i + kRing) & 15, wv);
+                        load_frag(fr[i % kRing], rsp, pn + kn * g.slot_bytes, rsr,
+                                  kn * g.slot_bytes, off16, off8, g);
+                    }
                 }
-                __builtin_amdgcn_sched_barrier(0);
+                // consumer position of this lane's Z values (uniform over half the lanes)
+                const int k1 = k >> 4, k2 = k & 15;
+                int lp = 0, late = -1;
+                if (uv == 0) lp = k2 + 16 * (k1 & 1);
+                else if (k2 != 0) lp = (16 - k2) + 16 * (2 + ((15 - k1) & 1));
+                else if (k == 0) late = 3;                 // Z[128]
+                else if ((k1 & 1) == 0) late = k1 / 2 - 1; // Z[256 - 16 k1], k1 even
+                else lp = 48;                              // Z[256 - 16 k1], k1 odd
+                unsigned *dst = (late >= 0)
+                    ? zl32 + par * (kGemmWaves * 16 * kQW * 4) + zl_lane + late
+                    : zc32 + buf * (kGemmWaves * 16 * 64 * kQW) + zc_lane + (lp ^ sw) * kQW;
+                const int rstride = (late >= 0) ? kQW * 4 : 64 * kQW;   // u32 per row
 #pragma unroll
-                for (int g = 0; g < 8; ++g) {
-                    const int so = (16 * g + blk) * 2;
-                    const uint2 a0 = ar0[so], a1 = ar1[so];
-                    const uint2 b0 = bq[g][0], b1 = bq[g][1];
-                    acc[g][0][0] = mfma4<BF16>(a0, b0, acc[g][0][0]);
-                    acc[g][0][1] = mfma4<BF16>(a0, b1, acc[g][0][1]);
-                    acc[g][1][0] = mfma4<BF16>(a1, b0, acc[g][1][0]);
-                    acc[g][1][1] = mfma4<BF16>(a1, b1, acc[g][1][1]);
+                for (int q = 0; q < 8; ++q) {
+                    const int rq = 4 * (q >> 1) + (q & 1);   // row - 2 hw
+                    const float x0 = acc[2 * q], x1 = acc[2 * q + 1];   // (a.col, b.col)
+                    float zr, zi;
+                    if (uv == 0) {
+                        zr = x0;
+                        zi = (k == 0) ? x0 : x1;
+                    } else {
+                        zr = x1;
+                        zi = (k == 0) ? x1 : x0;
+                    }
+                    dst[rq * rstride] = pack_h2(zr, zi);
                 }
             }
-            __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
+            // ---- consume: register c of this wavefront's pairs (row r, reference q)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const uint2 v = zc[buf][w][r][lane ^ zswz(w, r)];
+                const unsigned vv[2] = {v.x, v.y};
+#pragma unroll
+                for (int q = 0; q < kQW; ++q) {
+                    if (c == 0) za[kQW * r + q].x = vv[q];
+                    if (c == 1) za[kQW * r + q].y = vv[q];
+                    if (c == 2) za[kQW * r + q].z = vv[q];
+                    if (c == 3) za[kQW * r + q].w = vv[q];
+                }
+            }
         }
-        if (!active) continue;
-        // psi stage, four (row, reference) pairs per round: pair (ri, ci) of row pair rp
-        // and reference pair cp; lane j = 2ci + bpart writes Z-forms (bpart 0) or the
-        // Hermitian partners (bpart 1) of its 8 slots 16g + blk.
+        if (lane == 32) {
 #pragma unroll
-        for (int rp = 0; rp < 2; ++rp) {
+            for (int i = 0; i < 16 * kQW; ++i) za[i] = zl[par][w][i];
+        }
+
+        // ---- psi transform and search, row by row
+        const uint4 b1r = pcst[0][lane], b1i = pcst[1][lane];
+        const uint4 a2r = pcst[2][lane], a2i = pcst[3][lane];
+        const float4 twc4 = ptw[0][lane], tws4 = ptw[1][lane];
+        const float twc[4] = {twc4.x, twc4.y, twc4.z, twc4.w};
+        const float tws[4] = {tws4.x, tws4.y, tws4.z, tws4.w};
+        const long long row0 = (long long)rb * 16;
+        const int p_first = (int)(row0 / S_pad);
+        const int sg0 = (int)(row0 - (long long)p_first * S_pad);
 #pragma unroll
-            for (int cp = 0; cp < 2; ++cp) {
-                const int ci = jj >> 1, bpart = jj & 1;
-                float sc[2];
+        for (int r = 0; r < 16; ++r) {
+            // one row at a time: the scheduler must not interleave rows (their
+            // temporaries on top of the pair registers would spill)
+            __builtin_amdgcn_sched_barrier(0);
+            // S_pad >= 16: the block's rows span at most two particles
+            const bool second = sg0 + r >= S_pad;
+            const int p = p_first + (second ? 1 : 0);
+            const int sg = sg0 + r - (second ? S_pad : 0);
+            if (p < P && sg < S) {
+                const float4_t z4 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int ri = 0; ri < 2; ++ri) {
-                    const int s = min(s0 + 2 * rp + ri, S - 1);
-                    const float bound = pnorm[(size_t)p * S + s] * rn[2 * cp + ci];
-                    int e;
-                    frexpf(2048.0f / fmaxf(bound, 1e-30f), &e);
-                    sc[ri] = ldexpf(1.0f, e - 1);
-                }
-#pragma unroll
-                for (int gh = 0; gh < 2; ++gh) {   // slots g = 4gh .. 4gh+3
-                    unsigned short hre[2][4], him[2][4];
-#pragma unroll
-                    for (int gg = 0; gg < 4; ++gg) {
-                        const int g = 4 * gh + gg;
-                        const float4_t d = acc[g][rp][cp];
-                        float4_t o;
-                        o[0] = __shfl_xor(d[0], 1, kWave);
-                        o[1] = __shfl_xor(d[1], 1, kWave);
-                        o[2] = __shfl_xor(d[2], 1, kWave);
-                        o[3] = __shfl_xor(d[3], 1, kWave);
-#pragma unroll
-                        for (int ri = 0; ri < 2; ++ri) {
-                            const float P1 = (bpart ? o[2 * ri] : d[2 * ri]) * rscale;
-                            const float P3 = (bpart ? o[2 * ri + 1] : d[2 * ri + 1]) * rscale;
-                            const float P4 = (bpart ? d[2 * ri] : o[2 * ri]) * rscale;
-                            const float P2 = (bpart ? d[2 * ri + 1] : o[2 * ri + 1]) * rscale;
-                            const float A = P1 + P2, B = P4 - P3, Cc = P1 - P2, Dd = -(P3 + P4);
-                            float zr, zi;
-                            if (bpart == 0) {
-                                zr = A - Dd;
-                                zi = B + Cc;
-                                if (g == 0 && blk == 0) zr = zi = P1;   // Z[0] = C[0](1 + i)
-                            } else {
-                                zr = A + Dd;
-                                zi = Cc - B;
-                                if (g == 0 && blk == 0) zr = zi = P2;   // Z[128] = C[128](1 + i)
-                            }
-                            hre[ri][gg] = f32_to_f16(zr * sc[ri]);
-                            him[ri][gg] = f32_to_f16(zi * sc[ri]);
-                        }
-                    }
-                    // Z[k], k = 16 g + blk -> half index b*16 + g' (b = k%16, g' = k/16):
-                    //   bpart 0: blk*16 + g;  bpart 1 (partner 256-k): blk >= 1 ->
-                    //   (16-blk)*16 + 15-g; blk == 0 -> 16-g, with the Nyquist term (g=0) at 8.
-#pragma unroll
-                    for (int ri = 0; ri < 2; ++ri) {
-                        const int pr = 2 * ri + ci;
-                        if (bpart == 0) {
-                            zre[wave][zpos(pr, blk * 4 + gh)] = make_ushort4(hre[ri][0], hre[ri][1], hre[ri][2], hre[ri][3]);
-                            zim[wave][zpos(pr, blk * 4 + gh)] = make_ushort4(him[ri][0], him[ri][1], him[ri][2], him[ri][3]);
-                        } else if (blk) {
-                            // g = 4gh..4gh+3 -> g' = 15-g: gh 0 -> 15..12 (ushort4 3), gh 1 -> 11..8 (2)
-                            const int u = (16 - blk) * 4 + 3 - gh;
-                            zre[wave][zpos(pr, u)] = make_ushort4(hre[ri][3], hre[ri][2], hre[ri][1], hre[ri][0]);
-                            zim[wave][zpos(pr, u)] = make_ushort4(him[ri][3], him[ri][2], him[ri][1], him[ri][0]);
-                        } else {
-                            // blk 0: g' = 16-g (g >= 1), Nyquist (g = 0) at 8:
-                            // gh 0: g 0..3 -> 8, 15, 14, 13 ; gh 1: g 4..7 -> 12, 11, 10, 9
-                            // slots 2 and 3 (< 32): zpos leaves them in place
-                            unsigned short *zr16 = reinterpret_cast<unsigned short *>(&zre[wave][pr * 65]);
-                            unsigned short *zi16 = reinterpret_cast<unsigned short *>(&zim[wave][pr * 65]);
-#pragma unroll
-                            for (int gg = 0; gg < 4; ++gg) {
-                                const int g = 4 * gh + gg;
-                                const int idx = g ? 16 - g : 8;
-                                zr16[idx] = hre[ri][gg];
-                                zi16[idx] = him[ri][gg];
-                            }
-                        }
-                    }
-                }
-                wave_lds_sync();
-#pragma unroll
-                for (int pr = 0; pr < 4; ++pr) {
-                    const int ri = pr >> 1, c = 2 * cp + (pr & 1);
-                    const int s = s0 + 2 * rp + ri;
-                    if (s >= S || !nval[c]) continue;
-                    const int ci2 = pr & 1;
-                    const float inv = 1.0f / __shfl(sc[ri], 2 * ci2, kWave);
-                    // stage 1: rows b = l%16, K = g = 4(l/16)+v (re plane, then im plane)
-                    const ushort4 ar = zre[wave][zpos(pr, (lane & 15) * 4 + (lane >> 4))];
-                    const ushort4 ai = zim[wave][zpos(pr, (lane & 15) * 4 + (lane >> 4))];
-                    const half4_t Are = __builtin_bit_cast(half4_t, ar);
-                    const half4_t Aim = __builtin_bit_cast(half4_t, ai);
-                    const float4_t z4 = {0.f, 0.f, 0.f, 0.f};
-                    float4_t y_re = mfma16(Are, Bc, z4);
-                    y_re = mfma16(Aim, Bns, y_re);
-                    float4_t y_im = mfma16(Are, Bs, z4);
-                    y_im = mfma16(Aim, Bc, y_im);
-                    float tr[4], ti[4];
+                for (int q = 0; q < kQW; ++q) {
+                    const uint4 a1 = za[kQW * r + q];
+                    const float4_t yr = mma16x32(a1, b1r, z4);
+                    const float4_t yi = mma16x32(a1, b1i, z4);
+                    unsigned bb[4];
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                        tr[v] = y_re[v] * tw_re[v] - y_im[v] * tw_im[v];
-                        ti[v] = y_re[v] * tw_im[v] + y_im[v] * tw_re[v];
+                        const float tr = yr[v] * twc[v] - yi[v] * tws[v];
+                        const float ti = yr[v] * tws[v] + yi[v] * twc[v];
+                        bb[v] = pack_h2(tr, ti);
                     }
-                    const half4_t A2r = to_h4(tr[0], tr[1], tr[2], tr[3]);
-                    const half4_t A2i = to_h4(ti[0], ti[1], ti[2], ti[3]);
-                    float4_t c0 = mfma16(A2r, Bc, z4);
-                    c0 = mfma16(A2i, Bns, c0);
-                    float4_t c1 = mfma16(A2r, Bs, z4);
-                    c1 = mfma16(A2i, Bc, c1);
-                    // lane l holds t = 4(l/16) + v + 16 (l%16)
-                    const int tb = 4 * (lane >> 4) + 16 * (lane & 15);
+                    const uint4 b2 = make_uint4(bb[0], bb[1], bb[2], bb[3]);
+                    const float4_t zr = mma16x32(a2r, b2, z4);
+                    const float4_t zi = mma16x32(a2i, b2, z4);
 #pragma unroll
                     for (int m = 0; m < 2; ++m) {
-                        float x[4];
+                        const float4_t x = m ? zi : zr;
+                        float e[4];
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) x[v] = (m ? c1[v] : c0[v]) * inv;
-                        float bvl = x[0];
-                        int bt = tb;
-#pragma unroll
-                        for (int v = 1; v < 4; ++v)
-                            if (x[v] > bvl) {
-                                bvl = x[v];
-                                bt = tb + v;
-                            }
-                        if (!__any(bvl > bv[c][m])) continue;
-                        wave_argmax(bvl, bt);
-                        const int flat = s * kRingLen + bt;
-                        if (bvl > bv[c][m] || (bvl == bv[c][m] && flat < bf[c][m])) {
-                            bv[c][m] = bvl;
-                            bf[c][m] = flat;
-                            wave_lds_sync();
-                            zc[wave][(lane >> 4) + 4 * (lane & 15)] = make_float4(x[0], x[1], x[2], x[3]);
-                            wave_lds_sync();
-                            const float *zf = reinterpret_cast<const float *>(zc[wave]);
-                            if (lane < 7) nbuf[wave][c][m][lane] = zf[(bt + lane - 3) & 255];
-                            wave_lds_sync();
+                        for (int v = 0; v < 4; ++v) {
+                            const unsigned code = (unsigned)(code_top - (4 * sg + v));
+                            e[v] = __uint_as_float((__float_as_uint(x[v]) & ~lowmask) | code);
                         }
+                        const float t3 = vmax3(e[0], e[1], e[2]);
+                        if (second) rmB[q][m] = vmax3(rmB[q][m], t3, e[3]);
+                        else rmA[q][m] = vmax3(rmA[q][m], t3, e[3]);
                     }
                 }
-                wave_lds_sync();
             }
         }
-    }
+        // particle p_first complete in this block: decode its records, shift the state
+        const long long last_row = (long long)(p_first + 1) * S_pad - 1;
+        if (last_row < row0 + 16) {
+            const int p = p_first;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        if (!nval[c]) continue;
-        const int m = (bv[c][1] > bv[c][0]) ? 1 : 0;
-        const int f = (bf[c][m] == INT_MAX) ? 0 : bf[c][m];   // never updated (all values NaN)
-        PeakRec *o = rec + (size_t)p * N + nref[c];
-        if (lane == 0) {
-            o->val = bv[c][m];
-            o->m = m;
-            o->s = f / kRingLen;
-            o->t = f % kRingLen;
-            o->pad = 0;
+            for (int q = 0; q < kQW; ++q) {
+                const int n = nb * 16 + kQW * w + q;
+                float bv[2];
+                int bt[2], bs[2];
+#pragma unroll
+                for (int m = 0; m < 2; ++m) {
+                    float v = rmA[q][m];
+                    const unsigned u = __float_as_uint(v);
+                    const int idx = code_top - (int)(u & lowmask);
+                    int t = (lane & 15) + 64 * (lane >> 4) + 16 * (idx & 3);
+                    int s_ = idx >> 2;
+                    if (!(v > -INFINITY)) {   // no finite value in any sample
+                        t = INT_MAX;
+                        s_ = 0;
+                    }
+#pragma unroll
+                    for (int off = 32; off >= 1; off >>= 1) {
+                        const float ov = __shfl_xor(v, off, kWave);
+                        const int ot = __shfl_xor(t, off, kWave);
+                        const int os = __shfl_xor(s_, off, kWave);
+                        if (ov > v || (ov == v && ot < t)) {
+                            v = ov;
+                            t = ot;
+                            s_ = os;
+                        }
+                    }
+                    bv[m] = __uint_as_float(__float_as_uint(v) & ~lowmask);
+                    bt[m] = (t == INT_MAX) ? 0 : t;
+                    bs[m] = min(max(s_, 0), S - 1);
+                    rmA[q][m] = rmB[q][m];
+                    rmB[q][m] = -INFINITY;
+                }
+                if (p < P && n < N && lane == 0) {
+                    const int m = (bv[1] > bv[0]) ? 1 : 0;
+                    const float val = bv[m] * rinv[n];
+                    PeakRec o;
+                    o.val = val;
+                    o.m = m;
+                    o.s = bs[m];
+                    o.t = bt[m];
+#pragma unroll
+                    for (int e = 0; e < 7; ++e) o.x[e] = val;
+                    o.pad = 0;
+                    rec[(size_t)p * N + n] = o;
+                }
+            }
         }
-        if (lane < 7) o->x[lane] = nbuf[wave][c][m][lane];
     }
 }
 
 }  // namespace
 
-hipError_t launch_spec_to_mixed(const float2 *spec, const float2 *norm, int n_rows, int n_ring,
-                                int is_ref, bool bf16, void *out, float *row_norm, hipStream_t st) {
-    if (n_rows <= 0) return hipSuccess;
-    const int Q = (n_ring + 3) / 4;
-    const dim3 grid((n_rows + 3) / 4);
+GemmLayout gemm_layout(int n_ring) {
+    GemmLayout L;
+    L.full16 = n_ring / 16;
+    const int rem = n_ring % 16;
+    L.tail = rem == 0 ? 0 : (rem <= 8 ? 8 : 16);
+    L.k_pad = 16 * L.full16 + L.tail;
+    L.slot_bytes = 4 * L.k_pad;
+    L.row_bytes = (size_t)kSlots * L.slot_bytes;
+    L.block_bytes = 16 * L.row_bytes;
+    // |Z| <= c_p 256 sqrt(R (R + 1)) * 1.42 must stay below 32768
+    const double bound = 256.0 * sqrt((double)n_ring * (n_ring + 1)) * 1.42;
+    L.cp = ldexpf(1.0f, (int)floor(log2(32768.0 / bound)));
+    return L;
+}
+
+int gemm_index_bits(int n_shift) {
+    int b = 0;
+    while ((1 << b) < n_shift) ++b;
+    return 2 + b;
+}
+
+int gemm_rows_per_particle(int n_shift) { return n_shift < 16 ? 16 : n_shift; }
+
+size_t gemm_particle_operand_bytes(int n_img, int n_shift, int n_ring) {
+    const size_t rows = (size_t)((n_img + 15) / 16) * 16 * gemm_rows_per_particle(n_shift);
+    return rows / 16 * gemm_layout(n_ring).block_bytes;
+}
+
+hipError_t launch_frag_rows(const float2 *spec, const float2 *norm, int n_img, int n_shift,
+                            int n_ring, bool bf16, void *frag, hipStream_t st) {
+    if (n_img <= 0) return hipSuccess;
+    const GemmLayout L = gemm_layout(n_ring);
+    const GemmGeom g{L.full16, L.tail, L.slot_bytes};
+    const int S_pad = gemm_rows_per_particle(n_shift);
+    const int n_rows_pad = (n_img + 15) / 16 * 16 * S_pad;
+    const long long threads = (long long)n_rows_pad * kSlots;
+    const dim3 grid((unsigned)((threads + 255) / 256));
     if (bf16)
-        hipLaunchKernelGGL((k_spec_to_mixed<true>), grid, dim3(256), 0, st, spec, norm, n_rows,
-                           n_ring, Q, is_ref, (ushort4 *)out, row_norm);
+        hipLaunchKernelGGL((k_frag_rows<true>), grid, dim3(256), 0, st, spec, norm, n_img, n_shift,
+                           S_pad, n_rows_pad, n_ring, g, L.cp, (unsigned char *)frag);
     else
-        hipLaunchKernelGGL((k_spec_to_mixed<false>), grid, dim3(256), 0, st, spec, norm, n_rows,
-                           n_ring, Q, is_ref, (ushort4 *)out, row_norm);
+        hipLaunchKernelGGL((k_frag_rows<false>), grid, dim3(256), 0, st, spec, norm, n_img,
+                           n_shift, S_pad, n_rows_pad, n_ring, g, L.cp, (unsigned char *)frag);
     return hipGetLastError();
 }
 
-hipError_t launch_mref_mixed(const void *pop, const float *pnorm, const void *rop,
-                             const float *rnorm, int n_img, int n_shift, int n_ring, int n_ref,
-                             bool bf16, PeakRec *rec, hipStream_t st) {
-    if (n_img <= 0 || n_ref <= 0) return hipSuccess;
-    const int Q = (n_ring + 3) / 4;
-    const int G = (n_ref + 3) / 4;
-    const int GB = (G + 3) / 4;
-    const long long blocks = (long long)n_img * GB;
-    const float rscale = (float)n_ring;
+hipError_t launch_frag_refs(const float2 *rspec, int n_ref, int n_ring, bool bf16, float *cn,
+                            float *rinv, void *frag, hipStream_t st) {
+    if (n_ref <= 0) return hipSuccess;
+    const GemmLayout L = gemm_layout(n_ring);
+    const GemmGeom g{L.full16, L.tail, L.slot_bytes};
+    hipLaunchKernelGGL(k_ref_scale, dim3(n_ref), dim3(256), 0, st, rspec, n_ring, L.cp, cn, rinv);
+    const int n_pad = (n_ref + 15) / 16 * 16;
+    const dim3 grid((unsigned)((n_pad * kSlots + 255) / 256));
     if (bf16)
-        hipLaunchKernelGGL((k_mref_mixed<true>), dim3((unsigned)blocks), dim3(256), 0, st,
-                           (const uint2 *)pop, pnorm, (const uint2 *)rop, rnorm, n_img, n_shift, Q,
-                           n_ref, G, GB, rscale, rec);
+        hipLaunchKernelGGL((k_frag_refs<true>), grid, dim3(256), 0, st, rspec, n_ref, n_pad,
+                           n_ring, g, cn, (unsigned char *)frag);
     else
-        hipLaunchKernelGGL((k_mref_mixed<false>), dim3((unsigned)blocks), dim3(256), 0, st,
-                           (const uint2 *)pop, pnorm, (const uint2 *)rop, rnorm, n_img, n_shift, Q,
-                           n_ref, G, GB, rscale, rec);
+        hipLaunchKernelGGL((k_frag_refs<false>), grid, dim3(256), 0, st, rspec, n_ref, n_pad,
+                           n_ring, g, cn, (unsigned char *)frag);
+    return hipGetLastError();
+}
+
+hipError_t launch_mref_gemm(const void *pfrag, const void *rfrag, const float *rinv, int n_img,
+                            int n_shift, int n_ring, int n_ref, bool bf16, PeakRec *rec,
+                            hipStream_t st) {
+    if (n_img <= 0 || n_ref <= 0) return hipSuccess;
+    const GemmLayout L = gemm_layout(n_ring);
+    const GemmGeom g{L.full16, L.tail, L.slot_bytes};
+    const int nbits = gemm_index_bits(n_shift);
+    if (nbits > 12) return hipErrorInvalidValue;   // more than 1024 shifts
+    const int S_pad = gemm_rows_per_particle(n_shift);
+    const dim3 grid((unsigned)(((n_img + 15) / 16) * ((n_ref + 15) / 16)));
+    if (bf16)
+        hipLaunchKernelGGL((k_mref_gemm<true>), grid, dim3(kGemmThreads), 0, st,
+                           (const unsigned char *)pfrag, (const unsigned char *)rfrag, rinv, n_img,
+                           n_shift, S_pad, n_ref, g, nbits, rec);
+    else
+        hipLaunchKernelGGL((k_mref_gemm<false>), grid, dim3(kGemmThreads), 0, st,
+                           (const unsigned char *)pfrag, (const unsigned char *)rfrag, rinv, n_img,
+                           n_shift, S_pad, n_ref, g, nbits, rec);
     return hipGetLastError();
 }
 
