@@ -18,6 +18,7 @@
 #include <functional>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1178,6 +1179,156 @@ size_t spec_bytes_per_particle(const AlignConfig *c) {
     return S * c->ring_num * kSlots * sizeof(float2) + S * sizeof(float2) + sizeof(PeakRec);
 }
 
+// ------------------------------------------------- allocation-free entry point
+// isac_mref_align_device_noalloc works in one fixed device array per GPU
+// (`scratch` below, kNoAllocBytes): tables, reference spectra and one particle
+// batch of spectra and records at a time.  Its device address is looked up once
+// per device with hipGetSymbolAddress (the array's name in host code is only a
+// host-side handle) and kept in g_scratch_dev.
+constexpr size_t kNoAllocBytes = ISAC_NOALLOC_SCRATCH_BYTES;
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_dev[kNoAllocMaxDevices] = {};
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Byte offsets of the regions of the scratch array for one call.
+struct NoAllocLayout {
+    size_t twiddle, tmpl, zero_shift, shifts_i, shifts_f, rspec, rnorm, rsum0;
+    size_t spec, norm, rec;   // one particle batch of pb particles
+    size_t end;
+    int pb;
+};
+
+// Places the regions; pb = the largest batch that fits (0 when not even one does).
+NoAllocLayout noalloc_layout(int N, int R, int S, int P) {
+    NoAllocLayout L{};
+    size_t o = 0;
+    auto take = [&](size_t bytes) { const size_t at = o; o += align256(bytes); return at; };
+    L.twiddle = take(256 * sizeof(float2));
+    L.tmpl = take((size_t)R * kRingLen * sizeof(float2));
+    L.zero_shift = take(sizeof(int2));
+    L.shifts_i = take((size_t)S * sizeof(int2));
+    L.shifts_f = take((size_t)S * sizeof(float2));
+    L.rspec = take((size_t)N * R * kSlots * sizeof(float2));
+    L.rnorm = take((size_t)N * sizeof(float2));
+    L.rsum0 = take((size_t)N * sizeof(float));
+    const size_t fixed = o;
+    auto batch_bytes = [&](size_t pb) {
+        return align256(pb * S * R * kSlots * sizeof(float2)) + align256(pb * S * sizeof(float2)) +
+               align256(pb * N * sizeof(PeakRec));
+    };
+    size_t pb = 0;
+    if (fixed < kNoAllocBytes) {
+        const size_t per = (size_t)S * R * kSlots * sizeof(float2) + (size_t)S * sizeof(float2) +
+                           (size_t)N * sizeof(PeakRec);
+        pb = std::min<size_t>((kNoAllocBytes - fixed) / per, (size_t)P);
+        while (pb > 0 && fixed + batch_bytes(pb) > kNoAllocBytes) --pb;   // alignment slack
+    }
+    L.pb = (int)pb;
+    L.spec = take(pb * S * R * kSlots * sizeof(float2));
+    L.norm = take(pb * S * sizeof(float2));
+    L.rec = take(pb * N * sizeof(PeakRec));
+    L.end = o;
+    return L;
+}
+
+}  // namespace
+
+// The allocation-free entry point's device memory (one copy per GPU, from module
+// load until the process ends).
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+// Device address of `scratch` on the current device (looked up on the first call
+// there, with that device current).
+int scratch_on_current_device(unsigned char **out) {
+    int dev = -1;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices) {
+        set_error(kErrUnsupported, "device %d: the allocation-free entry supports devices 0..%d",
+                  dev, kNoAllocMaxDevices - 1);
+        return kErrUnsupported;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            set_error(kErrHip, "hipGetSymbolAddress returned a null address for the scratch array");
+            return kErrHip;
+        }
+        g_scratch_dev[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return 0;
+}
+
+// run_mref's pipeline in the scratch array: the same kernels on the same inputs
+// (results bit-identical to isac_mref_align_device), particle batches of at most
+// NoAllocLayout::pb particles, no device allocation, tables uploaded per call.
+int run_mref_noalloc(const float *d_sbj, const float *d_ref, ShiftView acc, int P, int N, int D,
+                     int R, const std::vector<float2> &shifts, float *d_peak, int peak_row,
+                     float *d_d, int d_pitch, hipStream_t st) {
+    const int S = (int)shifts.size();
+    unsigned char *base = nullptr;
+    if (int e = scratch_on_current_device(&base)) return e;
+    const NoAllocLayout L = noalloc_layout(N, R, S, P);
+    if (L.pb < 1) {
+        set_error(kErrUnsupported,
+                  "isac_mref_align_device_noalloc: %d references x %d rings x %d shifts need more "
+                  "than its %zu-byte scratch array for one particle",
+                  N, R, S, kNoAllocBytes);
+        return kErrUnsupported;
+    }
+    std::vector<float2> tw(256);
+    for (int m = 0; m < 256; ++m) {   // the twiddle table of ensure_device_ws
+        const double a = -2.0 * M_PI * (double)m / 256.0;
+        tw[m] = make_float2((float)cos(a), (float)sin(a));
+    }
+    const std::vector<float2> tm = make_template(R);
+    std::vector<int2> si(shifts.size());
+    for (size_t i = 0; i < shifts.size(); ++i) si[i] = make_int2((int)shifts[i].x, (int)shifts[i].y);
+    const int2 zero{0, 0};
+    auto at = [&](size_t off) { return static_cast<void *>(base + off); };
+    HIP_OK(hipMemcpyAsync(at(L.twiddle), tw.data(), 256 * sizeof(float2), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(at(L.tmpl), tm.data(), tm.size() * sizeof(float2), hipMemcpyHostToDevice,
+                          st));
+    HIP_OK(hipMemcpyAsync(at(L.zero_shift), &zero, sizeof(int2), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(at(L.shifts_i), si.data(), si.size() * sizeof(int2),
+                          hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(at(L.shifts_f), shifts.data(), shifts.size() * sizeof(float2),
+                          hipMemcpyHostToDevice, st));
+    const float2 *twd = static_cast<const float2 *>(at(L.twiddle));
+    const float2 *tmpl = static_cast<const float2 *>(at(L.tmpl));
+    float2 *rspec = static_cast<float2 *>(at(L.rspec));
+    float2 *rnorm = static_cast<float2 *>(at(L.rnorm));
+    float *rsum0 = static_cast<float *>(at(L.rsum0));
+    float2 *spec = static_cast<float2 *>(at(L.spec));
+    float2 *norm = static_cast<float2 *>(at(L.norm));
+    PeakRec *rec = static_cast<PeakRec *>(at(L.rec));
+    const int2 *shi = static_cast<const int2 *>(at(L.shifts_i));
+    const float2 *shf = static_cast<const float2 *>(at(L.shifts_f));
+    HIP_OK(isac::launch_polar_spectra(d_ref, N, D, ShiftView{nullptr, 0},
+                                      static_cast<const int2 *>(at(L.zero_shift)), 1, tmpl, R, twd,
+                                      rspec, rnorm, st));
+    HIP_OK(isac::launch_ref_finalize(rspec, rnorm, N, R, rsum0, st));
+    for (int p0 = 0; p0 < P; p0 += L.pb) {
+        const int n = std::min(L.pb, P - p0);
+        ShiftView a = acc;
+        if (a.base) a.base += (size_t)p0 * a.stride;
+        HIP_OK(isac::launch_polar_spectra(d_sbj + (size_t)p0 * D * D, n, D, a, shi, S, tmpl, R, twd,
+                                          spec, norm, st));
+        HIP_OK(isac::launch_ccf_mref(spec, norm, rspec, rsum0, n, S, R, N, twd, rec, st));
+        HIP_OK(isac::launch_decode_mref(rec, n, N, a, shf, S, d_peak, peak_row, p0, d_d, d_pitch,
+                                        st));
+    }
+    // the host tables above must outlive their copies; the call returns on completion
+    HIP_OK(hipStreamSynchronize(st));
+    return 0;
+}
+
 }  // namespace
 
 namespace isac {
@@ -1349,6 +1500,24 @@ int isac_mref_align_device(const float *d_sbj, const float *d_ref, const float *
     ShiftView acc{d_acc_xy, 2};
     return run_mref(d_sbj, d_ref, acc, (int)n_sbj, (int)n_ref, (int)img_dim, (int)ring_num, sh,
                     d_peak_list, 4 * (int)n_sbj, d_d, (int)d_pitch, 0.9f, st);
+}
+
+int isac_mref_align_device_noalloc(const float *d_sbj, const float *d_ref, const float *d_acc_xy,
+                                   unsigned int n_sbj, unsigned int n_ref, unsigned int img_dim,
+                                   unsigned int ring_num, unsigned int ring_len, float shift_step,
+                                   float shift_rng_x, float shift_rng_y, float *d_peak_list,
+                                   float *d_d, unsigned int d_pitch, void *stream) {
+    std::vector<float2> sh = make_shift_grid(shift_rng_x, shift_rng_y, shift_step);
+    if (int e = check_shape((int)img_dim, (int)ring_num, (int)ring_len, sh.size())) return e;
+    if (n_sbj == 0 || n_ref == 0) return 0;
+    if (!d_sbj || !d_ref || !d_peak_list || !d_d || d_pitch < n_sbj) {
+        set_error(kErrArg, "isac_mref_align_device_noalloc: bad pointers or d_pitch < n_sbj");
+        return kErrArg;
+    }
+    ShiftView acc{d_acc_xy, 2};
+    return run_mref_noalloc(d_sbj, d_ref, acc, (int)n_sbj, (int)n_ref, (int)img_dim,
+                            (int)ring_num, sh, d_peak_list, 4 * (int)n_sbj, d_d, (int)d_pitch,
+                            (hipStream_t)stream);
 }
 
 int isac_mref_align_mixed_device(const float *d_sbj, const float *d_ref, const float *d_acc_xy,

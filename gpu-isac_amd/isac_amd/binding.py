@@ -34,8 +34,11 @@ EXTRA_SYMBOLS = (
     "isac_mref_align_mixed_device", "isac_mref_best_mixed_device", "isac_profile_enable",
     "isac_profile_read", "isac_mref_align_batched_device", "isac_batched_report",
     "isac_mref_proposal_device", "isac_assign_groups_device", "isac_assign_groups",
-    "isac_update_references_device",
+    "isac_update_references_device", "isac_mref_align_device_noalloc",
 )
+
+# Size of the static device array of isac_mref_align_device_noalloc (ISAC_NOALLOC_SCRATCH_BYTES).
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 BATCH_PER_SUBSET = 0   # ISAC_BATCH_PER_SUBSET
 BATCH_GRAPH = 1        # ISAC_BATCH_GRAPH
@@ -136,6 +139,8 @@ def load(path: Optional[str] = None):
     lib.isac_mref_align_device.argtypes = [vp, vp, vp, c_uint, c_uint, c_uint, c_uint, c_uint,
                                            c_float, c_float, c_float, vp, vp, c_uint, vp]
     lib.isac_mref_align_device.restype = c_int
+    lib.isac_mref_align_device_noalloc.argtypes = lib.isac_mref_align_device.argtypes
+    lib.isac_mref_align_device_noalloc.restype = c_int
     lib.isac_mref_align_mixed_device.argtypes = [vp, vp, vp, c_uint, c_uint, c_uint, c_uint, c_uint,
                                                  c_float, c_float, c_float, c_int, vp, vp, c_uint, vp]
     lib.isac_mref_align_mixed_device.restype = c_int
@@ -374,6 +379,35 @@ def mref_align_device(sbj, ref, ring_num: int, shift_step: float, shift_rng_x: f
     if rc:
         check(lib)
         raise GpuAlignError(f"isac_mref_align_device returned {rc}")
+    return peak, d
+
+
+def mref_align_device_noalloc(sbj, ref, ring_num: int, shift_step: float, shift_rng_x: float,
+                              shift_rng_y: float, peak, d, acc_xy=None, stream=None):
+    """mref_align_device without any device allocation inside the library: the
+    caller passes the outputs peak [N, 4P] and d [N, P] (float32, cuda); the work
+    runs in the library's static scratch array (NOALLOC_SCRATCH_BYTES per GPU).
+    Calls that overlap on one GPU race on that array."""
+    import torch
+    lib = load()
+    P, D = sbj.shape[0], sbj.shape[-1]
+    N = ref.shape[0]
+    assert sbj.is_cuda and ref.is_cuda and sbj.dtype == torch.float32 and ref.dtype == torch.float32
+    assert sbj.is_contiguous() and ref.is_contiguous()
+    assert tuple(peak.shape) == (N, 4 * P) and tuple(d.shape) == (N, P)
+    assert peak.dtype == torch.float32 and d.dtype == torch.float32
+    assert peak.is_contiguous() and d.is_contiguous()
+    acc_ptr = None
+    if acc_xy is not None:
+        assert acc_xy.is_cuda and acc_xy.dtype == torch.float32 and acc_xy.is_contiguous()
+        acc_ptr = acc_xy.data_ptr()
+    st = (stream or torch.cuda.current_stream()).cuda_stream
+    rc = lib.isac_mref_align_device_noalloc(sbj.data_ptr(), ref.data_ptr(), acc_ptr, P, N, D,
+                                            ring_num, RING_LEN, shift_step, shift_rng_x,
+                                            shift_rng_y, peak.data_ptr(), d.data_ptr(), P, st)
+    if rc:
+        check(lib)
+        raise GpuAlignError(f"isac_mref_align_device_noalloc returned {rc}")
     return peak, d
 
 

@@ -159,6 +159,25 @@ int isac_mref_align_device(const float *d_sbj, const float *d_ref, const float *
                            float *d_peak_list, float *d_d, unsigned int d_pitch,
                            void *stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  isac_mref_align_device without any device allocation or
+ * free: same arguments, same kernels, bit-identical results.  It works in one
+ * static device array of ISAC_NOALLOC_SCRATCH_BYTES per GPU (held from library
+ * load until the process ends), which must hold the tables, the reference
+ * spectra (n_ref * ring_num * 1 KiB) and the spectra of at least one particle
+ * (shifts * ring_num * 1 KiB); larger jobs return error 4.  Particles are
+ * processed in batches that fit.  Calls that overlap on one GPU (other threads,
+ * other streams) share the array and race: serialise them.  Queued on `stream`;
+ * returns after completion. */
+#define ISAC_NOALLOC_SCRATCH_BYTES 67108864 /* 64 MiB */
+int isac_mref_align_device_noalloc(const float *d_sbj, const float *d_ref, const float *d_acc_xy,
+                                   unsigned int n_sbj, unsigned int n_ref, unsigned int img_dim,
+                                   unsigned int ring_num, unsigned int ring_len,
+                                   float shift_step, float shift_rng_x, float shift_rng_y,
+                                   float *d_peak_list, float *d_d, unsigned int d_pitch,
+                                   void *stream);
+
 /* No counterpart in the reference (mixed-precision form of
  * isac_mref_align_device): the ring contraction runs on FP16 (bf16 = 0) or BF16
  * (bf16 = 1) MFMA with FP32 accumulation and the psi transform on FP16 MFMA.
