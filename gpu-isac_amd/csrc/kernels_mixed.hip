@@ -54,6 +54,7 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef short short4_t __attribute__((ext_vector_type(4)));
 typedef float float16_t __attribute__((ext_vector_type(16)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
 
 constexpr int kWave = 64;
 
@@ -72,8 +73,11 @@ __device__ __forceinline__ unsigned short to16(float x) {
     return BF16 ? f32_to_bf16(x) : f32_to_h(x);
 }
 
+// two FP32 values as an FP16 pair (one v_cvt_pk_f16_f32, round to nearest even)
 __device__ __forceinline__ unsigned pack_h2(float lo, float hi) {
-    return (unsigned)f32_to_h(lo) | ((unsigned)f32_to_h(hi) << 16);
+    typedef _Float16 hp_t __attribute__((ext_vector_type(2)));
+    const hp_t v = {(_Float16)lo, (_Float16)hi};
+    return __builtin_bit_cast(unsigned, v);
 }
 
 template <bool BF16>
@@ -322,13 +326,16 @@ template <bool BF16>
 __device__ __forceinline__ float16_t contract(const Frag &f, __amdgpu_buffer_rsrc_t ra, unsigned pa,
                                               __amdgpu_buffer_rsrc_t rbr, unsigned pb,
                                               unsigned off16, GemmGeom g) {
+    // each step from a zero accumulator, summed on the VALU (no MFMA reads another
+    // one's result as its accumulator input)
+    const float16_t z0 = {};
     float16_t acc = {};
-    if (g.full16 > 0) acc = mma32x16<BF16>(f.a0, f.b0, acc);
-    if (g.full16 > 1 || (g.full16 == 1 && g.tail == 16)) acc = mma32x16<BF16>(f.a1, f.b1, acc);
+    if (g.full16 > 0) acc = mma32x16<BF16>(f.a0, f.b0, z0);
+    if (g.full16 > 1 || (g.full16 == 1 && g.tail == 16)) acc += mma32x16<BF16>(f.a1, f.b1, z0);
     // rings beyond 32 (R > 40): further steps, loaded here
     for (int ks = 2; ks < g.full16 + (g.tail == 16 ? 1 : 0); ++ks)
-        acc = mma32x16<BF16>(bload16(ra, off16, pa + ks * 64), bload16(rbr, off16, pb + ks * 64), acc);
-    if (g.tail == 8) acc = mma32x8<BF16>(f.a8, f.b8, acc);
+        acc += mma32x16<BF16>(bload16(ra, off16, pa + ks * 64), bload16(rbr, off16, pb + ks * 64), z0);
+    if (g.tail == 8) acc += mma32x8<BF16>(f.a8, f.b8, z0);
     return acc;
 }
 
@@ -617,6 +624,339 @@ __global__ __launch_bounds__(kGemmThreads, 1) void k_mref_gemm(
     }
 }
 
+// ------------------------------------- search kernel, reference-resident tiles
+// k_mref_rt (rings <= 48): workgroup = 16 particles x 8 references, eight
+// wavefronts.  Wavefront w holds the operand fragments of the 8 references for
+// its 16 slots (k = 8 i + w) in registers for the whole workgroup, so per block
+// of 16 (particle, shift) rows only the particle operands are read.  Per block:
+//  * contraction: per slot two 16 x 16 tiles (rows = 8 particle rows x {a, b},
+//    columns = 8 references x {u, v}) on v_mfma_f32_16x16x32 over rings 0..31
+//    and v_mfma_f32_16x16x16 over a further 16-ring step and / or the 8-ring
+//    tail; particle fragments stream through a ring of kRtRing slots;
+//  * every Z value of the block (16 rows x 8 references x 256 bins, FP16 re/im,
+//    128 KB) goes to LDS at its psi operand position, so the psi transform of a
+//    pair loads its whole A operand with one ds_read_b128 per lane;
+//  * psi transform and search as in k_mref_gemm, wavefront w on reference w.
+// The v part of slot k is stored as (a.v, b.v) = (Im, Re) of Z[256 - k]; the
+// constant stage-1 matrix swaps re / im for the lane groups that hold those
+// bins (2 and 3), so no lane selects between its two products.
+constexpr int kRtWaves = 8;
+constexpr int kRtThreads = 64 * kRtWaves;
+constexpr int kRtRefs = 8;
+constexpr int kRtRing = 4;   // divides the 16 slots of a wavefront: entry i % kRtRing across blocks
+constexpr int kRtMaxRings = 48;   // one 16x16x32 step plus one 16-ring or 8-ring 16x16x16 step
+
+struct RtFrag {   // one slot's operand fragment of 8 rows (or references) x {parts}
+    uint4 m;      // 16x16x32 step, rings 0..31
+    uint2 a;      // 16x16x16 step over 16-ring step 2 * NX32
+    uint2 b;      // 16x16x16 step over the 8-ring tail (lane groups 0, 1)
+};
+
+template <bool BF16>
+__device__ __forceinline__ float4_t mmag32(uint4 a, uint4 b, float4_t c) {
+    if (BF16)
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a),
+                                                       __builtin_bit_cast(bf16x8_t, b), c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(half8_t, a),
+                                                  __builtin_bit_cast(half8_t, b), c, 0, 0, 0);
+}
+
+template <bool BF16>
+__device__ __forceinline__ float4_t mmag16(uint2 a, uint2 b, float4_t c) {
+    if (BF16)
+        return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(short4_t, a),
+                                                         __builtin_bit_cast(short4_t, b), c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(half4_t, a),
+                                                 __builtin_bit_cast(half4_t, b), c, 0, 0, 0);
+}
+
+// psi operand position of bin kk (1..255, or 128): lane k2 + 16 lg, register j
+// with k1_of(lg, j) = kk >> 4; returned as 4 * lane + j
+__device__ __forceinline__ int psi_pos(int kk) {
+    const int k1 = kk >> 4, k2 = kk & 15;
+    const int lg = (k1 < 8) ? (k1 & 1) : 2 + (k1 & 1);
+    const int j = (k1 < 8) ? (k1 >> 1) : ((14 + (k1 & 1) - k1) >> 1);
+    return 4 * (k2 + 16 * lg) + j;
+}
+
+template <bool BF16, int KP>
+__global__ __launch_bounds__(kRtThreads, 1) void k_mref_rt(
+    const unsigned char *__restrict__ pfrag, const unsigned char *__restrict__ rfrag,
+    const float *__restrict__ rinv, int P, int S, int S_pad, int N, int nbits,
+    PeakRec *__restrict__ rec) {
+    // operand geometry from the padded ring count KP (8, 16, ..., 48)
+    constexpr int FULL16 = KP / 16, TAIL = KP % 16;                 // 16-ring steps, 8-ring tail
+    constexpr int NKS = FULL16;                                      // KP = 16 NKS + TAIL
+    constexpr int NX32 = NKS >= 2 ? 1 : 0, X16A = NKS & 1, X16B = TAIL ? 1 : 0;
+    constexpr unsigned SB = 4 * KP, RB = kSlots * SB, BB = 16 * RB;  // slot, row, block bytes
+    // Z of the block: [reference][row][psi lane ^ reference] -> 4 registers of FP16 (re, im)
+    __shared__ uint4 zs[kRtRefs * 16 * 64];
+    __shared__ uint4 pcst[4][64];     // stage-1 B (re, im outputs), stage-2 A (re, im)
+    __shared__ uint2 ptw[4][64];      // twiddle (c, s) and (-s, c) as FP16 pairs, per v
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int NB = (N + kRtRefs - 1) / kRtRefs;
+    const int pg = lb / NB, nb = lb % NB;
+    const __amdgpu_buffer_rsrc_t rsp = byte_rsrc(pfrag + (size_t)pg * S_pad * BB,
+                                                 (unsigned)S_pad * BB);
+    const __amdgpu_buffer_rsrc_t rsr = byte_rsrc(rfrag + (size_t)nb * kRtRefs * RB, kRtRefs * RB);
+    // operand lanes: row (or reference) idx = (l % 16) / 2, part l % 2, K group gq = l / 16;
+    // the wavefront's slot offset w * SB is folded into the lane offsets
+    const int gq = lane >> 4, idx = (lane & 15) >> 1, part = lane & 1;
+    const unsigned vb = idx * RB + w * SB;
+    const unsigned o32 = vb + (gq >> 1) * 64 + part * 32 + (gq & 1) * 16;
+    const unsigned o16a = vb + (2 * NX32) * 64 + part * 32 + (gq >> 1) * 16 + (gq & 1) * 8;
+    const unsigned o16b = vb + FULL16 * 64 + part * 16 + (gq & 1) * 8;
+
+    if (w == 0) {
+        const int c16 = lane & 15, lg = lane >> 4;
+        unsigned short e1r[8], e1i[8], e2r[8], e2i[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            // lane groups 2, 3 hold (Im, Re) pairs: their K rows of B1 are swapped
+            const int p1 = (e & 1) ^ (lg >= 2 ? 1 : 0);
+            const int k1 = k1_of(lg, e >> 1);
+            float s1, c1;
+            sincospif((float)((k1 * c16) & 15) / 8.0f, &s1, &c1);
+            e1r[e] = f32_to_h(p1 ? -s1 : c1);
+            e1i[e] = f32_to_h(p1 ? c1 : s1);
+            const int p2 = e & 1;
+            const int k2 = 4 * lg + (e >> 1);
+            float s2, c2;
+            sincospif((float)((k2 * c16) & 15) / 8.0f, &s2, &c2);
+            e2r[e] = f32_to_h(p2 ? -s2 : c2);
+            e2i[e] = f32_to_h(p2 ? c2 : s2);
+        }
+        auto pk = [](const unsigned short *e) {
+            return make_uint4(e[0] | ((unsigned)e[1] << 16), e[2] | ((unsigned)e[3] << 16),
+                              e[4] | ((unsigned)e[5] << 16), e[6] | ((unsigned)e[7] << 16));
+        };
+        pcst[0][lane] = pk(e1r);
+        pcst[1][lane] = pk(e1i);
+        pcst[2][lane] = pk(e2r);
+        pcst[3][lane] = pk(e2i);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            float ts, tc;
+            sincospif((float)(((4 * lg + v) * c16) & 255) / 128.0f, &ts, &tc);
+            ptw[v][lane] = make_uint2(pack_h2(tc, ts), pack_h2(-ts, tc));
+        }
+    }
+
+    // resident reference fragments of this wavefront's slots k = 8 i + w; the tail
+    // step's K rows 8..15 (lane groups 2, 3) do not exist and stay zero, which also
+    // cancels whatever the particle operand holds there
+    RtFrag rf[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        if (NX32) rf[i].m = bload16(rsr, o32, 8 * i * SB);
+        if (X16A) rf[i].a = bload8(rsr, o16a, 8 * i * SB);
+        if (X16B) {
+            const uint2 t = bload8(rsr, o16b, 8 * i * SB);
+            rf[i].b = (gq < 2) ? t : make_uint2(0u, 0u);
+        }
+    }
+
+    // LDS position (u32) of this lane's Z values of slot i: Z[k] (u lanes) or
+    // Z[256 - k] (v lanes) of row 2 gq (+ 8 T + pp), reference q, at the psi lane
+    // swizzled by the reference
+    const int q = (lane & 15) >> 1, uv = lane & 1;
+    unsigned dz[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int k = 8 * i + w;
+        const int pos = uv ? psi_pos(k == 0 ? 128 : 256 - k) : psi_pos(k);
+        dz[i] = (unsigned)(((q * 16 + 2 * gq) * 64) * 4 + (pos ^ (q << 2))) * 4u;
+    }
+
+    // particle fragments of one slot for the block's two 8-row tiles
+    struct PF {
+        RtFrag t[2];
+    };
+    auto pload = [&](PF &f, unsigned blk_off, int i) {
+#pragma unroll
+        for (int T = 0; T < 2; ++T) {
+            const unsigned so = blk_off + 8 * i * SB + T * 8 * RB;
+            if (NX32) f.t[T].m = bload16(rsp, o32, so);
+            if (X16A) f.t[T].a = bload8(rsp, o16a, so);
+            if (X16B) f.t[T].b = bload8(rsp, o16b, so);
+        }
+    };
+    PF pf[kRtRing];
+#pragma unroll
+    for (int s0 = 0; s0 < kRtRing; ++s0) pload(pf[s0], 0u, s0);
+
+    __syncthreads();   // the constant tables
+    const uint4 b1r = pcst[0][lane], b1i = pcst[1][lane];
+    const uint4 a2r = pcst[2][lane], a2i = pcst[3][lane];
+    uint2 tw[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) tw[v] = ptw[v][lane];
+
+    // search state of reference nb * 8 + w: running maxima per mirror of the open
+    // particle; the index code sits in the low nbits of each value
+    const int code_top = 4 * S - 1;
+    unsigned keep = ~((1u << nbits) - 1u);
+    asm volatile("" : "+v"(keep));   // a VGPR operand: one bit-select per value
+    float rm0 = -INFINITY, rm1 = -INFINITY;
+    const int n_ref = nb * kRtRefs + w;
+    const float rsc = (n_ref < N) ? rinv[n_ref] : 0.0f;
+    int p_open = 16 * pg;                                        // first particle of the group
+    int sg_open = 0;                                             // shift row of block row 0
+    char *zsb = reinterpret_cast<char *>(zs);
+    const uint4 *zp = zs + w * 16 * 64 + (lane ^ w);   // reference w, row 0, this lane
+
+    // wave reduction of the open particle's maxima -> its record
+    auto flush = [&](int p) {
+        float bv[2];
+        int bt[2], bs[2];
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            float v = m ? rm1 : rm0;
+            const unsigned u = __float_as_uint(v);
+            const int ix = code_top - (int)(u & ~keep);
+            int t = (lane & 15) + 64 * (lane >> 4) + 16 * (ix & 3);
+            int s_ = ix >> 2;
+            if (!(v > -INFINITY)) {   // no finite value in any sample
+                t = INT_MAX;
+                s_ = 0;
+            }
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                const float ov = __shfl_xor(v, off, kWave);
+                const int ot = __shfl_xor(t, off, kWave);
+                const int os = __shfl_xor(s_, off, kWave);
+                if (ov > v || (ov == v && ot < t)) {
+                    v = ov;
+                    t = ot;
+                    s_ = os;
+                }
+            }
+            bv[m] = __uint_as_float(__float_as_uint(v) & keep);
+            bt[m] = (t == INT_MAX) ? 0 : t;
+            bs[m] = min(max(s_, 0), S - 1);
+        }
+        if (p < P && n_ref < N && lane == 0) {
+            const int m = (bv[1] > bv[0]) ? 1 : 0;
+            const float val = bv[m] * rsc;
+            PeakRec o;
+            o.val = val;
+            o.m = m;
+            o.s = bs[m];
+            o.t = bt[m];
+#pragma unroll
+            for (int e = 0; e < 7; ++e) o.x[e] = val;
+            o.pad = 0;
+            rec[(size_t)p * N + n_ref] = o;
+        }
+        rm0 = rm1 = -INFINITY;
+    };
+
+    // psi transform of row r (reference w) and the running maxima
+    auto psi_row = [&](int r, int sg) {
+        const float4_t z4 = {0.f, 0.f, 0.f, 0.f};
+        const uint4 a1 = zp[r * 64];
+        const float4_t yr = mma16x32(a1, b1r, z4);
+        const float4_t yi = mma16x32(a1, b1i, z4);
+        unsigned bb[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            // (yr + i yi) e^{+2 pi i k2 t1 / 256} in FP16 pairs: yr (c, s) + yi (-s, c)
+            const h2_t y0 = {(_Float16)yr[v], (_Float16)yr[v]};
+            const h2_t y1 = {(_Float16)yi[v], (_Float16)yi[v]};
+            const h2_t o = y0 * __builtin_bit_cast(h2_t, tw[v].x) +
+                           y1 * __builtin_bit_cast(h2_t, tw[v].y);
+            bb[v] = __builtin_bit_cast(unsigned, o);
+        }
+        const uint4 b2 = make_uint4(bb[0], bb[1], bb[2], bb[3]);
+        const float4_t zr = mma16x32(a2r, b2, z4);
+        const float4_t zi = mma16x32(a2i, b2, z4);
+        const unsigned c0 = (unsigned)(code_top - 4 * sg);
+        float e[2][4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const unsigned cv = c0 - v;
+            e[0][v] = __uint_as_float((__float_as_uint(zr[v]) & keep) | (cv & ~keep));
+            e[1][v] = __uint_as_float((__float_as_uint(zi[v]) & keep) | (cv & ~keep));
+        }
+        rm0 = vmax3(rm0, vmax3(e[0][0], e[0][1], e[0][2]), e[0][3]);
+        rm1 = vmax3(rm1, vmax3(e[1][0], e[1][1], e[1][2]), e[1][3]);
+    };
+
+    for (int jb = 0; jb < S_pad; ++jb) {
+        const unsigned pblk = (unsigned)jb * BB;
+        // ---- contraction: this wavefront's 16 slots for both 8-row tiles
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int s = i % kRtRing;
+            // every step from a zero accumulator, summed on the VALU: no MFMA reads
+            // another one's result as its accumulator input (chained 16x16x32 and
+            // 16x16x16 accumulators, with the tiles' registers interleaved by the
+            // scheduler, gave wrong sums)
+            const float4_t z0 = {0.f, 0.f, 0.f, 0.f};
+            float4_t acc[2];
+#pragma unroll
+            for (int T = 0; T < 2; ++T) {
+                float4_t a = z0;
+                if (NX32) a = mmag32<BF16>(pf[s].t[T].m, rf[i].m, z0);
+                if (X16A) {
+                    const float4_t t = mmag16<BF16>(pf[s].t[T].a, rf[i].a, z0);
+                    a = NX32 ? a + t : t;
+                }
+                if (X16B) {
+                    const float4_t t = mmag16<BF16>(pf[s].t[T].b, rf[i].b, z0);
+                    a = (NX32 || X16A) ? a + t : t;
+                }
+                acc[T] = a;
+            }
+            if (i + kRtRing < 16) pload(pf[s], pblk, i + kRtRing);
+            else if (jb + 1 < S_pad) pload(pf[s], pblk + BB, i + kRtRing - 16);
+#pragma unroll
+            for (int T = 0; T < 2; ++T)
+#pragma unroll
+                for (int pp = 0; pp < 2; ++pp)
+                    *reinterpret_cast<unsigned *>(zsb + dz[i] + (8 * T + pp) * 1024) =
+                        pack_h2(acc[T][2 * pp], acc[T][2 * pp + 1]);
+            if (i == 0 && w == 0) {
+                // slot 0 packs the real bins: Z[0] = (a.u)(1 + i) (u lanes) and
+                // Z[128] = (b.v)(1 + i) (v lanes)
+#pragma unroll
+                for (int T = 0; T < 2; ++T)
+#pragma unroll
+                    for (int pp = 0; pp < 2; ++pp) {
+                        const float x = uv ? acc[T][2 * pp + 1] : acc[T][2 * pp];
+                        *reinterpret_cast<unsigned *>(zsb + dz[0] + (8 * T + pp) * 1024) =
+                            pack_h2(x, x);
+                    }
+            }
+        }
+        __syncthreads();
+
+        // ---- psi transform and search: reference w, rows 0..15.  S_pad >= 16:
+        // the rows split at most once between the open particle and the next
+        const int split = min(16, S_pad - sg_open);
+        const int rows1 = (p_open < P) ? min(split, max(S - sg_open, 0)) : 0;
+        for (int r = 0; r < rows1; ++r) psi_row(r, sg_open + r);
+        if (split < 16) {
+            flush(p_open);
+            ++p_open;
+            const int rows2 = (p_open < P) ? min(16 - split, S) : 0;
+            for (int r = 0; r < rows2; ++r) psi_row(split + r, r);
+            sg_open = 16 - split;
+        } else {
+            sg_open += 16;
+            if (sg_open == S_pad) {
+                flush(p_open);
+                ++p_open;
+                sg_open = 0;
+            }
+        }
+        __syncthreads();   // the next block's contraction overwrites zs
+    }
+}
+
+
 }  // namespace
 
 GemmLayout gemm_layout(int n_ring) {
@@ -691,6 +1031,32 @@ hipError_t launch_mref_gemm(const void *pfrag, const void *rfrag, const float *r
     const int nbits = gemm_index_bits(n_shift);
     if (nbits > 12) return hipErrorInvalidValue;   // more than 1024 shifts
     const int S_pad = gemm_rows_per_particle(n_shift);
+    if (n_ring <= kRtMaxRings) {
+        // reference-resident tiles; k_pad = 8, 16, ..., 48 selects the instantiation
+        const dim3 grid((unsigned)(((n_img + 15) / 16) * ((n_ref + kRtRefs - 1) / kRtRefs)));
+        const unsigned char *pf = (const unsigned char *)pfrag;
+        const unsigned char *rf = (const unsigned char *)rfrag;
+#define ISAC_RT_LAUNCH(B, KP)                                                                   \
+    hipLaunchKernelGGL((k_mref_rt<B, KP>), grid, dim3(kRtThreads), 0, st, pf, rf, rinv, n_img,  \
+                       n_shift, S_pad, n_ref, nbits, rec)
+        switch (L.k_pad * 2 + (bf16 ? 1 : 0)) {
+        case 16: ISAC_RT_LAUNCH(false, 8); break;
+        case 17: ISAC_RT_LAUNCH(true, 8); break;
+        case 32: ISAC_RT_LAUNCH(false, 16); break;
+        case 33: ISAC_RT_LAUNCH(true, 16); break;
+        case 48: ISAC_RT_LAUNCH(false, 24); break;
+        case 49: ISAC_RT_LAUNCH(true, 24); break;
+        case 64: ISAC_RT_LAUNCH(false, 32); break;
+        case 65: ISAC_RT_LAUNCH(true, 32); break;
+        case 80: ISAC_RT_LAUNCH(false, 40); break;
+        case 81: ISAC_RT_LAUNCH(true, 40); break;
+        case 96: ISAC_RT_LAUNCH(false, 48); break;
+        case 97: ISAC_RT_LAUNCH(true, 48); break;
+        default: return hipErrorInvalidValue;
+        }
+#undef ISAC_RT_LAUNCH
+        return hipGetLastError();
+    }
     const dim3 grid((unsigned)(((n_img + 15) / 16) * ((n_ref + 15) / 16)));
     if (bf16)
         hipLaunchKernelGGL((k_mref_gemm<true>), grid, dim3(kGemmThreads), 0, st,
