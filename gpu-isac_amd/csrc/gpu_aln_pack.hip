@@ -1520,6 +1520,29 @@ int isac_mref_align_device_noalloc(const float *d_sbj, const float *d_ref, const
                             (hipStream_t)stream);
 }
 
+int isac_multi_align_stability_device(const double *d_params, unsigned int n_total,
+                                      const int *d_class_off, unsigned int n_class,
+                                      unsigned int n_runs, float mir_stab_thld, float grp_err_thld,
+                                      float err_thld, float diameter, float *d_pix, float *d_avg,
+                                      int *d_stable, float *d_class, float *d_global, void *stream) {
+    if (n_class == 0) return 0;
+    if (n_runs < 2 || n_runs > (unsigned)isac::kStabMaxRuns) {
+        set_error(kErrArg, "isac_multi_align_stability_device: %u runs (need 2..%d)", n_runs,
+                  isac::kStabMaxRuns);
+        return kErrArg;
+    }
+    if (!d_params || !d_class_off || !d_pix || !d_avg || !d_stable || !d_class || !d_global) {
+        set_error(kErrArg, "isac_multi_align_stability_device: null pointer");
+        return kErrArg;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    HIP_OK(isac::launch_stability(d_params, (int)n_total, d_class_off, (int)n_class, (int)n_runs,
+                                  mir_stab_thld, grp_err_thld, err_thld, (double)diameter, d_pix,
+                                  d_avg, d_stable, d_class, d_global, st));
+    HIP_OK(hipStreamSynchronize(st));
+    return 0;
+}
+
 int isac_mref_align_mixed_device(const float *d_sbj, const float *d_ref, const float *d_acc_xy,
                                  unsigned int n_sbj, unsigned int n_ref, unsigned int img_dim,
                                  unsigned int ring_num, unsigned int ring_len, float shift_step,

@@ -279,4 +279,16 @@ hipError_t launch_mref_gemm(const void *pfrag, const void *rfrag, const float *r
                             int n_shift, int n_ring, int n_ref, bool bf16, PeakRec *rec,
                             hipStream_t st);
 
+// Multi-alignment stability test, one workgroup per class (stability.hip).
+// prm [L][n_total][4] FP64 {angle deg, sx, sy, mirror}; class c = members
+// [class_off[c], class_off[c+1]).  Outputs per member: pix (-1 when not mirror
+// consistent or the class is rejected), avg [4], stable (0/1); per class: cls [2]
+// = {mirror-consistent rate, error (-1 when rejected)}, gout [L][4] = fitted
+// {alpha deg, tx, ty, flip} per run.
+constexpr int kStabMaxRuns = 12;
+hipError_t launch_stability(const double *prm, int n_total, const int *class_off, int n_class,
+                            int L, float mir_thld, float grp_thld, float err_thld, double d,
+                            float *pix, float *avg, int *stable, float *cls, float *gout,
+                            hipStream_t st);
+
 }  // namespace isac

@@ -35,6 +35,7 @@ EXTRA_SYMBOLS = (
     "isac_profile_read", "isac_mref_align_batched_device", "isac_batched_report",
     "isac_mref_proposal_device", "isac_assign_groups_device", "isac_assign_groups",
     "isac_update_references_device", "isac_mref_align_device_noalloc",
+    "isac_multi_align_stability_device",
 )
 
 # Size of the static device array of isac_mref_align_device_noalloc (ISAC_NOALLOC_SCRATCH_BYTES).
@@ -140,6 +141,10 @@ def load(path: Optional[str] = None):
                                            c_float, c_float, c_float, vp, vp, c_uint, vp]
     lib.isac_mref_align_device.restype = c_int
     lib.isac_mref_align_device_noalloc.argtypes = lib.isac_mref_align_device.argtypes
+    lib.isac_multi_align_stability_device.argtypes = [vp, c_uint, vp, c_uint, c_uint, c_float,
+                                                      c_float, c_float, c_float, vp, vp, vp, vp,
+                                                      vp, vp]
+    lib.isac_multi_align_stability_device.restype = c_int
     lib.isac_mref_align_device_noalloc.restype = c_int
     lib.isac_mref_align_mixed_device.argtypes = [vp, vp, vp, c_uint, c_uint, c_uint, c_uint, c_uint,
                                                  c_float, c_float, c_float, c_int, vp, vp, c_uint, vp]
@@ -409,6 +414,56 @@ def mref_align_device_noalloc(sbj, ref, ring_num: int, shift_step: float, shift_
         check(lib)
         raise GpuAlignError(f"isac_mref_align_device_noalloc returned {rc}")
     return peak, d
+
+
+def multi_align_stability_device(classes, mir_stab_thld: float, grp_err_thld: float,
+                                 err_thld: float, d: float, stream=None):
+    """Stability test of many classes in one device call.
+
+    classes: one entry per class in the driver's layout (aln_param[k]: L flat
+    lists [angle, sx, sy, mirror] * n_k, bin/sp_isac2_gpu.py:1381-1393); every
+    class has the same L.  Returns per class (stable_set, mirror_consistent_rate,
+    err, extra) like sp_pixel_error.multi_align_stability
+    (bin/sp_isac2_gpu.py:1491): stable_set = [[pixel_error, member index,
+    [angle, sx, sy, mirror]], ...] sorted by (pixel error, index); extra holds the
+    per-member pixel errors and average alignments and the fitted transforms."""
+    import torch
+    lib = load()
+    arrs = [np.asarray(c, np.float64).reshape(len(c), -1, 4) for c in classes]
+    L = arrs[0].shape[0]
+    assert all(a.shape[0] == L for a in arrs), "every class needs the same number of runs"
+    counts = [a.shape[1] for a in arrs]
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    prm = np.concatenate(arrs, axis=1) if arrs else np.zeros((L, 0, 4))
+    n_total, n_class = int(off[-1]), len(arrs)
+    dev = torch.device("cuda")
+    d_prm = torch.from_numpy(np.ascontiguousarray(prm)).to(dev)
+    d_off = torch.from_numpy(off).to(dev)
+    pix = torch.empty(max(n_total, 1), dtype=torch.float32, device=dev)
+    avg = torch.empty((max(n_total, 1), 4), dtype=torch.float32, device=dev)
+    stab = torch.empty(max(n_total, 1), dtype=torch.int32, device=dev)
+    cls = torch.empty((max(n_class, 1), 2), dtype=torch.float32, device=dev)
+    glob = torch.empty((max(n_class, 1), L, 4), dtype=torch.float32, device=dev)
+    st = (stream or torch.cuda.current_stream()).cuda_stream
+    rc = lib.isac_multi_align_stability_device(d_prm.data_ptr(), n_total, d_off.data_ptr(), n_class,
+                                               L, mir_stab_thld, grp_err_thld, err_thld, d,
+                                               pix.data_ptr(), avg.data_ptr(), stab.data_ptr(),
+                                               cls.data_ptr(), glob.data_ptr(), st)
+    if rc:
+        check(lib)
+        raise GpuAlignError(f"isac_multi_align_stability_device returned {rc}")
+    pix, avg, stab = pix.cpu().numpy(), avg.cpu().numpy(), stab.cpu().numpy()
+    cls, glob = cls.cpu().numpy(), glob.cpu().numpy()
+    out = []
+    for c in range(n_class):
+        a, b = off[c], off[c + 1]
+        st_set = [[float(pix[a + i]), int(i), [float(x) for x in avg[a + i, :3]] + [int(avg[a + i, 3])]]
+                  for i in range(b - a) if stab[a + i]]
+        st_set.sort(key=lambda e: (e[0], e[1]))
+        extra = {"pix": pix[a:b].copy(), "avg": avg[a:b].copy(), "stable": stab[a:b].copy(),
+                 "G": glob[c].copy(), "flips": glob[c, :, 3].astype(int)}
+        out.append((st_set, float(cls[c, 0]), float(cls[c, 1]), extra))
+    return out
 
 
 def mref_align_mixed_device(sbj, ref, ring_num: int, shift_step: float, shift_rng_x: float,

@@ -315,6 +315,31 @@ int isac_update_references_device(const float *d_imgs, unsigned int n_img, unsig
                                   float cutoff, float falloff, float *d_refs, float *d_params,
                                   int *d_members, float *d_shift_avg, void *stream);
 
+/* Multi-alignment stability test of ISAC classes on the device: the step the
+ * driver runs per class on the host with SPHIRE's
+ * sp_pixel_error.multi_align_stability(aln_param[k], 0.0, 10000.0, thld_err,
+ * False, last_ring*2) (bin/sp_isac2_gpu.py:1481-1491; the module is not in the
+ * reference tree).  d_params [n_runs][n_total][4] FP64 {angle deg, sx, sy,
+ * mirror} of every member in every repeated run (the driver's aln_param[k],
+ * :1381-1393, classes concatenated); class c holds members
+ * [d_class_off[c], d_class_off[c+1]).  Per class: the mirror-flip pattern held
+ * by most members selects the mirror-consistent members; one global rigid
+ * transform per run (the last run fixed) is fitted to minimise the mean squared
+ * pixel error over a circle of the given diameter, leaving out members above
+ * 3 err_thld at the start; then per member: d_pix = pixel error (-1 when not
+ * mirror consistent or when the class is rejected: consistent rate <=
+ * mir_stab_thld or fewer than 2 members to fit), d_avg [4] = average
+ * {angle, sx, sy, mirror} in the last run's frame, d_stable = 1 when
+ * d_pix <= err_thld and the class error <= grp_err_thld; per class: d_class
+ * [2] = {mirror-consistent rate, class error sqrt(mean pix^2) or -1},
+ * d_global [n_runs][4] = fitted {alpha deg, tx, ty, flip}.  2 <= n_runs <= 12.
+ * Queued on `stream`; returns after completion. */
+int isac_multi_align_stability_device(const double *d_params, unsigned int n_total,
+                                      const int *d_class_off, unsigned int n_class,
+                                      unsigned int n_runs, float mir_stab_thld, float grp_err_thld,
+                                      float err_thld, float diameter, float *d_pix, float *d_avg,
+                                      int *d_stable, float *d_class, float *d_global, void *stream);
+
 /* No counterpart in the reference.  When enabled, the library brackets every
  * launch of its contraction+psi kernel with HIP events on the stream it
  * launches on, and accumulates launch count and total milliseconds. */

@@ -1,7 +1,7 @@
 """FP64 CPU oracle of the multi-alignment stability test.
 
 TEST INFRASTRUCTURE ONLY.  Only ``tests/`` import this module, and only as the
-checker of the library's stability kernel (``isac_stability_device``).
+checker of the library's stability kernel (``isac_multi_align_stability_device``).
 
 What it restates.  After the GPU class refinement, the driver calls, per class
 k (bin/sp_isac2_gpu.py:1481-1491):
