@@ -135,6 +135,8 @@ struct Workspace {
     DevBuf cand, rec_c, best, second, tie, val64;   // candidate re-scoring
     DevBuf items, smask, fb_idx, fb_count, fb_img, fb_acc;   // narrowed search + fallback
     DevBuf edge, smask2;                                     // edge flags, level-2 shifts
+    DevBuf pre_tp, pre_mir, pre_part, pre_avg, pre_shift, pre_ref_of, pre_work, pre_twd;
+    int pre_twd_dim = 0;
     std::vector<float2> shifts_uploaded;
     BatchWs batch;
     void release() {
@@ -143,9 +145,11 @@ struct Workspace {
                          &rspec, &rnorm, &rsum0, &rec, &peak, &dvals, &acc,
                          &mix_p, &mix_pn, &mix_r, &mix_rn, &cand, &rec_c, &best, &second,
                          &tie, &val64, &items, &smask, &fb_idx, &fb_count, &fb_img, &fb_acc,
-                         &edge, &smask2};
+                         &edge, &smask2, &pre_tp, &pre_mir, &pre_part, &pre_avg, &pre_shift,
+                         &pre_ref_of, &pre_work, &pre_twd};
         for (DevBuf *b : all) b->release();
         tmpl_rings = 0;
+        pre_twd_dim = 0;
         shifts_uploaded.clear();
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
@@ -619,7 +623,7 @@ int run_proposal(const float *d_sbj, const float *d_ref, ShiftView acc, int P, i
 // reference index of particle i at ref_of[i * ref_stride].
 int run_single(const float *d_sbj, DevParam *prm, const int *ref_of, int ref_stride, int cnt,
                const float *d_ref, int N, int D, int R, const std::vector<float2> &shifts,
-               float shift_limit, hipStream_t st) {
+               float shift_limit, hipStream_t st, bool sync = true) {
     const int S = (int)shifts.size();
     if (int e = check_shape(D, R, kRingLen, shifts.size())) return e;
     if (cnt <= 0) return 0;
@@ -650,7 +654,7 @@ int run_single(const float *d_sbj, DevParam *prm, const int *ref_of, int ref_str
         HIP_OK(isac::launch_decode_single(ws.rec.as<PeakRec>(), n, prm + p0,
                                           ws.shifts_f.as<float2>(), S, shift_limit, st));
     }
-    HIP_OK(hipStreamSynchronize(st));
+    if (sync) HIP_OK(hipStreamSynchronize(st));
     return 0;
 }
 
@@ -1179,6 +1183,68 @@ size_t spec_bytes_per_particle(const AlignConfig *c) {
     return S * c->ring_num * kSlots * sizeof(float2) + S * sizeof(float2) + sizeof(PeakRec);
 }
 
+// --------------------------------------------------- pre-alignment loop (device)
+// bin/sp_isac_applications.py:1486-1693 with every iteration on the device: per
+// shift step the grid of (range, step) (reset_shifts); per iteration the average
+// of the particles transformed with their current parameters, optionally moved
+// by the driver's average-shift centring (center == -1, from the parameters of
+// the previous alignment) and low-passed (cutoff > 0), then every particle
+// aligned to it (pre_align_run); the last iteration of the last step only builds
+// the average (:1563).  Nothing returns to the host between iterations.
+int run_pre_align_loop(const float *d_imgs, int P, int D, int R, const float *ranges,
+                       const float *steps, int n_steps, int max_iter, int center, float cutoff,
+                       float falloff, DevParam *prm, float *d_average, hipStream_t st) {
+    if (P <= 0 || n_steps <= 0 || max_iter <= 0) return 0;
+    HIP_OK(ws.pre_tp.reserve((size_t)P * sizeof(float4)));
+    HIP_OK(ws.pre_mir.reserve((size_t)P * sizeof(int)));
+    HIP_OK(ws.pre_part.reserve(isac::pre_average_scratch_floats(P, D) * sizeof(float)));
+    HIP_OK(ws.pre_avg.reserve((size_t)D * D * sizeof(float)));
+    HIP_OK(ws.pre_shift.reserve(sizeof(float2)));
+    HIP_OK(ws.pre_ref_of.reserve((size_t)P * sizeof(int)));
+    HIP_OK(ws.pre_work.reserve((size_t)D * (D / 2 + 1) * sizeof(float2)));
+    HIP_OK(hipMemsetAsync(ws.pre_ref_of.p, 0, (size_t)P * sizeof(int), st));   // one reference
+    if (ws.pre_twd_dim != D) {
+        std::vector<float2> tw(D);
+        for (int m = 0; m < D; ++m) {
+            const double a = -2.0 * M_PI * (double)m / (double)D;
+            tw[m] = make_float2((float)cos(a), (float)sin(a));
+        }
+        HIP_OK(ws.pre_twd.reserve(D * sizeof(float2)));
+        HIP_OK(hipMemcpy(ws.pre_twd.p, tw.data(), D * sizeof(float2), hipMemcpyHostToDevice));
+        ws.pre_twd_dim = D;
+    }
+    float *avg = ws.pre_avg.as<float>();
+    const float lim = (float)(D - R - 2);
+    const int total = n_steps * max_iter;
+    int it_glob = 0;
+    for (int n = 0; n < n_steps; ++n) {
+        const std::vector<float2> sh = make_shift_grid(ranges[n], ranges[n], steps[n]);
+        if (int e = check_shape(D, R, kRingLen, sh.size())) return e;
+        for (int it = 0; it < max_iter; ++it) {
+            ++it_glob;
+            HIP_OK(isac::launch_pre_average(d_imgs, P, D, prm, ws.pre_tp.as<float4>(),
+                                            ws.pre_mir.as<int>(), ws.pre_part.as<float>(), avg, st));
+            // the driver's sums start at zero; later they come from the last alignment
+            const bool shift_avg = (center == -1) && it_glob > 1;
+            if (shift_avg) HIP_OK(isac::launch_pre_center(prm, P, ws.pre_shift.as<float2>(), st));
+            if (shift_avg || cutoff > 0.0f)
+                HIP_OK(isac::launch_filter_shift(avg, 1, D, cutoff, falloff,
+                                                 shift_avg ? ws.pre_shift.as<float2>() : nullptr,
+                                                 ws.pre_work.as<float2>(), ws.pre_twd.as<float2>(),
+                                                 st));
+            if (it_glob != total)
+                if (int e = run_single(d_imgs, prm, ws.pre_ref_of.as<int>(), 1, P, avg, 1, D, R, sh,
+                                       lim, st, false))
+                    return e;
+        }
+    }
+    if (d_average)
+        HIP_OK(hipMemcpyAsync(d_average, avg, (size_t)D * D * sizeof(float), hipMemcpyDeviceToDevice,
+                              st));
+    HIP_OK(hipStreamSynchronize(st));
+    return 0;
+}
+
 // ------------------------------------------------- allocation-free entry point
 // isac_mref_align_device_noalloc works in one fixed device array per GPU
 // (`scratch` below, kNoAllocBytes): tables, reference spectra and one particle
@@ -1518,6 +1584,31 @@ int isac_mref_align_device_noalloc(const float *d_sbj, const float *d_ref, const
     return run_mref_noalloc(d_sbj, d_ref, acc, (int)n_sbj, (int)n_ref, (int)img_dim,
                             (int)ring_num, sh, d_peak_list, 4 * (int)n_sbj, d_d, (int)d_pitch,
                             (hipStream_t)stream);
+}
+
+int isac_pre_align_loop_device(const float *d_imgs, unsigned int n_img, unsigned int img_dim,
+                               unsigned int ring_num, const float *shift_ranges,
+                               const float *shift_steps, unsigned int n_steps,
+                               unsigned int max_iter, int center, float cutoff, float falloff,
+                               void *d_aln_param, float *d_average, void *stream) {
+    if (int e = ensure_device_ws(-1)) return e;
+    if (n_img == 0 || n_steps == 0 || max_iter == 0) return 0;
+    if (!d_imgs || !shift_ranges || !shift_steps || !d_aln_param) {
+        set_error(kErrArg, "isac_pre_align_loop_device: null pointer");
+        return kErrArg;
+    }
+    if (center != 0 && center != -1) {
+        set_error(kErrUnsupported, "isac_pre_align_loop_device: center %d (0 or -1)", center);
+        return kErrUnsupported;
+    }
+    if (cutoff > 0.0f && !(falloff > 0.0f)) {
+        set_error(kErrArg, "isac_pre_align_loop_device: falloff must be > 0 with a cutoff");
+        return kErrArg;
+    }
+    return run_pre_align_loop(d_imgs, (int)n_img, (int)img_dim, (int)ring_num, shift_ranges,
+                              shift_steps, (int)n_steps, (int)max_iter, center, cutoff, falloff,
+                              static_cast<DevParam *>(d_aln_param), d_average,
+                              (hipStream_t)stream);
 }
 
 int isac_multi_align_stability_device(const double *d_params, unsigned int n_total,

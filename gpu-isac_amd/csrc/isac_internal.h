@@ -200,6 +200,15 @@ hipError_t launch_decode_single(const PeakRec *rec, int cnt, DevParam *prm,
 hipError_t launch_transform(const float *imgs, int n, int img_dim, const DevParam *prm,
                             float *out, hipStream_t st);
 
+// Pre-alignment loop: avg = mean over P particles of T_p(imgs[p]) (the transform of
+// launch_transform), tp / mir hold P entries, part pre_average_scratch_floats(P, D);
+// shift[0] = the centring move (-sx_sum / P, -sy_sum / P) of the driver's
+// average-shift centring.
+size_t pre_average_scratch_floats(int P, int D);
+hipError_t launch_pre_average(const float *imgs, int P, int D, const DevParam *prm, float4 *tp,
+                              int *mir, float *part, float *avg, hipStream_t st);
+hipError_t launch_pre_center(const DevParam *prm, int P, float2 *shift, hipStream_t st);
+
 // Per-class mean of transformed images into refs.
 hipError_t launch_class_average(const float *tr, int img_dim, const int *cid_idx, int n_cls,
                                 float *refs, hipStream_t st);
@@ -211,6 +220,7 @@ hipError_t launch_tanl_filter(float *refs, int n, int img_dim, float cutoff, flo
 
 // Tanh low-pass with absolute frequency (cycles per pixel) and the phase ramp that
 // moves image n's content by shift[n]; output scaled by 1 / (D*D) (a proper inverse).
+// cutoff <= 0: no low-pass, the phase ramp alone.
 hipError_t launch_filter_shift(float *refs, int n, int img_dim, float cutoff, float falloff,
                                const float2 *shift, float2 *work, const float2 *tw_d,
                                hipStream_t st);

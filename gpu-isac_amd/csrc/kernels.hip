@@ -1677,6 +1677,89 @@ __global__ __launch_bounds__(256) void k_class_average(const float *__restrict__
     }
 }
 
+// ------------------------------------------------------ pre-alignment loop
+// The average of every particle transformed with its current parameters (the
+// image transform of k_transform), without materialising the transformed stack:
+// per particle the rotation (cos, sin), shifts and mirror; partial sums over
+// chunks of kPreChunk particles per pixel; then the chunks in order, divided by P.
+constexpr int kPreChunk = 256;
+
+__global__ __launch_bounds__(256) void k_pre_tp(const DevParam *__restrict__ prm, int P,
+                                                float4 *__restrict__ tp, int *__restrict__ mir) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const DevParam q = prm[p];
+    const float ang = (3.141592653589793f / 180.0f) * q.angle;   // as k_transform
+    tp[p] = make_float4(cosf(ang), sinf(ang), q.shift_x, q.shift_y);
+    mir[p] = q.mirror ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_pre_partial(const float *__restrict__ imgs, int D, int P,
+                                                     const float4 *__restrict__ tp,
+                                                     const int *__restrict__ mir,
+                                                     float *__restrict__ part) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (idx >= D * D) return;
+    const int x = idx % D, y = idx / D;
+    const float cen = (float)(D / 2);
+    const int a = c * kPreChunk, b = min(a + kPreChunk, P);
+    float s = 0.0f;
+    for (int p = a; p < b; ++p) {
+        const float4 t = tp[p];
+        float sx = mir[p] ? (float)(D - x) : (float)x;
+        float sy = (float)y;
+        sx -= cen;
+        sy -= cen;
+        const float tx = sx;
+        sx = tx * t.x - sy * t.y;
+        sy = tx * t.y + sy * t.x;
+        sx += cen;
+        sy += cen;
+        s += bilinear(imgs + (size_t)p * D * D, D, sx + t.z, sy + t.w);
+    }
+    part[(size_t)c * D * D + idx] = s;
+}
+
+__global__ __launch_bounds__(256) void k_pre_finish(const float *__restrict__ part, int D, int P,
+                                                    int n_chunks, float *__restrict__ avg) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= D * D) return;
+    double s = 0.0;
+    for (int c = 0; c < n_chunks; ++c) s += (double)part[(size_t)c * D * D + idx];
+    avg[idx] = (float)(s / (double)P);
+}
+
+// Centring shift of the average from the particles' shifts after the driver's
+// rotation (bin/sp_isac_applications.py:1664-1690): sx_sum = sum of +-shift_x'
+// (minus for mirrored particles), sy_sum = sum of shift_y'; the average is then
+// moved by (-sx_sum / P, -sy_sum / P) (:1547-1552).  One workgroup, FP64 sums.
+__global__ __launch_bounds__(256) void k_pre_center(const DevParam *__restrict__ prm, int P,
+                                                    float2 *__restrict__ shift) {
+    __shared__ double rx[256], ry[256];
+    double sx = 0.0, sy = 0.0;
+    for (int p = threadIdx.x; p < P; p += blockDim.x) {
+        const DevParam q = prm[p];
+        const double a = (double)q.angle * (M_PI / 180.0);
+        const double c = cos(a), sn = -sin(a);
+        const double nx = -(double)q.shift_x, ny = -(double)q.shift_y;
+        const double px = nx * c - ny * sn, py = nx * sn + ny * c;
+        sx += q.mirror ? -px : px;
+        sy += py;
+    }
+    rx[threadIdx.x] = sx;
+    ry[threadIdx.x] = sy;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            rx[threadIdx.x] += rx[threadIdx.x + o];
+            ry[threadIdx.x] += ry[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) shift[0] = make_float2((float)(-rx[0] / P), (float)(-ry[0] / P));
+}
+
 // ------------------------------------------------------- reference update
 // Device form of the multi-reference driver's reference update
 // (bin/sp_isac2_gpu.py:1146-1232): every particle is transformed with the
@@ -1883,7 +1966,9 @@ __global__ void k_col_filter(float2 *__restrict__ F, int D, const float2 *__rest
         float yy = ((float)ky < half) ? (float)ky : (float)(D - ky);
         yy = yy / half * 0.5f;
         const float dd = sqrtf(xx * xx + yy * yy);
-        const float h = 0.5f * (tanhf(cc * (dd + cutoff)) - tanhf(cc * (dd - cutoff)));
+        // cutoff <= 0: no low-pass (the phase ramp alone)
+        const float h = cutoff > 0.0f ? 0.5f * (tanhf(cc * (dd + cutoff)) - tanhf(cc * (dd - cutoff)))
+                                      : 1.0f;
         re *= h;
         im *= h;
         if (shift) {
@@ -2174,6 +2259,27 @@ hipError_t launch_transform(const float *imgs, int n, int img_dim, const DevPara
                             float *out, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_transform, dim3(n), dim3(256), 0, st, imgs, img_dim, prm, out);
+    return hipGetLastError();
+}
+
+size_t pre_average_scratch_floats(int P, int D) {
+    return (size_t)((P + kPreChunk - 1) / kPreChunk) * D * D;
+}
+
+hipError_t launch_pre_average(const float *imgs, int P, int D, const DevParam *prm, float4 *tp,
+                              int *mir, float *part, float *avg, hipStream_t st) {
+    if (P <= 0) return hipSuccess;
+    const int nc = (P + kPreChunk - 1) / kPreChunk;
+    hipLaunchKernelGGL(k_pre_tp, dim3((P + 255) / 256), dim3(256), 0, st, prm, P, tp, mir);
+    hipLaunchKernelGGL(k_pre_partial, dim3((D * D + 255) / 256, nc), dim3(256), 0, st, imgs, D, P,
+                       tp, mir, part);
+    hipLaunchKernelGGL(k_pre_finish, dim3((D * D + 255) / 256), dim3(256), 0, st, part, D, P, nc,
+                       avg);
+    return hipGetLastError();
+}
+
+hipError_t launch_pre_center(const DevParam *prm, int P, float2 *shift, hipStream_t st) {
+    hipLaunchKernelGGL(k_pre_center, dim3(1), dim3(256), 0, st, prm, P, shift);
     return hipGetLastError();
 }
 

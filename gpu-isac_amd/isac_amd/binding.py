@@ -35,7 +35,7 @@ EXTRA_SYMBOLS = (
     "isac_profile_read", "isac_mref_align_batched_device", "isac_batched_report",
     "isac_mref_proposal_device", "isac_assign_groups_device", "isac_assign_groups",
     "isac_update_references_device", "isac_mref_align_device_noalloc",
-    "isac_multi_align_stability_device",
+    "isac_multi_align_stability_device", "isac_pre_align_loop_device",
 )
 
 # Size of the static device array of isac_mref_align_device_noalloc (ISAC_NOALLOC_SCRATCH_BYTES).
@@ -145,6 +145,9 @@ def load(path: Optional[str] = None):
                                                       c_float, c_float, c_float, vp, vp, vp, vp,
                                                       vp, vp]
     lib.isac_multi_align_stability_device.restype = c_int
+    lib.isac_pre_align_loop_device.argtypes = [vp, c_uint, c_uint, c_uint, vp, vp, c_uint, c_uint,
+                                               c_int, c_float, c_float, vp, vp, vp]
+    lib.isac_pre_align_loop_device.restype = c_int
     lib.isac_mref_align_device_noalloc.restype = c_int
     lib.isac_mref_align_mixed_device.argtypes = [vp, vp, vp, c_uint, c_uint, c_uint, c_uint, c_uint,
                                                  c_float, c_float, c_float, c_int, vp, vp, c_uint, vp]
@@ -414,6 +417,43 @@ def mref_align_device_noalloc(sbj, ref, ring_num: int, shift_step: float, shift_
         check(lib)
         raise GpuAlignError(f"isac_mref_align_device_noalloc returned {rc}")
     return peak, d
+
+
+def pre_align_loop_device(imgs, ring_num: int, shift_ranges, shift_steps, max_iter: int,
+                          center: int = 0, cutoff: float = 0.0, falloff: float = 0.0,
+                          acc_xy=None, stream=None):
+    """The pre-alignment driver loop on the device (isac_pre_align_loop_device):
+    imgs (P, D, D) float32 cuda tensor; one entry of shift_ranges / shift_steps per
+    shift step, max_iter iterations each.  Returns (params dict with the
+    AlignParam fields as numpy arrays, final average as a (D, D) cuda tensor)."""
+    import torch
+    lib = load()
+    assert imgs.is_cuda and imgs.dtype == torch.float32 and imgs.is_contiguous()
+    P, D = imgs.shape[0], imgs.shape[-1]
+    recs = (AlignParam * P)()
+    if acc_xy is not None:
+        acc = np.asarray(acc_xy, np.float64).reshape(P, 2)
+        for i in range(P):
+            recs[i].shift_x, recs[i].shift_y = float(acc[i, 0]), float(acc[i, 1])
+    raw = np.frombuffer(bytes(recs), dtype=np.uint8)
+    d_prm = torch.from_numpy(raw.copy()).to(imgs.device)
+    avg = torch.empty((D, D), dtype=torch.float32, device=imgs.device)
+    rng_ = (ctypes.c_float * len(shift_ranges))(*[float(x) for x in shift_ranges])
+    stp = (ctypes.c_float * len(shift_steps))(*[float(x) for x in shift_steps])
+    assert len(shift_ranges) == len(shift_steps)
+    st = (stream or torch.cuda.current_stream()).cuda_stream
+    rc = lib.isac_pre_align_loop_device(imgs.data_ptr(), P, D, ring_num, ctypes.cast(rng_, ctypes.c_void_p),
+                                        ctypes.cast(stp, ctypes.c_void_p), len(shift_ranges),
+                                        max_iter, center, cutoff, falloff, d_prm.data_ptr(),
+                                        avg.data_ptr(), st)
+    if rc:
+        check(lib)
+        raise GpuAlignError(f"isac_pre_align_loop_device returned {rc}")
+    out = (AlignParam * P).from_buffer_copy(d_prm.cpu().numpy().tobytes())
+    prm = {"angle": np.array([r.angle for r in out]), "shift_x": np.array([r.shift_x for r in out]),
+           "shift_y": np.array([r.shift_y for r in out]),
+           "mirror": np.array([int(r.mirror) for r in out])}
+    return prm, avg
 
 
 def multi_align_stability_device(classes, mir_stab_thld: float, grp_err_thld: float,

@@ -315,6 +315,31 @@ int isac_update_references_device(const float *d_imgs, unsigned int n_img, unsig
                                   float cutoff, float falloff, float *d_refs, float *d_params,
                                   int *d_members, float *d_shift_avg, void *stream);
 
+/* Pre-alignment driver loop on the device: bin/sp_isac_applications.py:1486-1693
+ * (the N_step x max_iter iterations of ali2d_base_gpu_isac_CLEAN, whose device
+ * step is pre_align_fetch / pre_align_run, cuda/gpu_aln_noref.cu:565-769) with no
+ * host round trip between iterations.  d_imgs [n_img][img_dim][img_dim] (device,
+ * mean-subtracted by the caller as the driver does); shift_ranges / shift_steps
+ * (host, n_steps each) replace the driver's xrng / step lists (reset_shifts per
+ * step); d_aln_param: n_img AlignParam records in device memory, the starting
+ * parameters on entry (the driver starts from zero) and the final ones on
+ * return, in the convention of pre_align_init's records.  Per iteration: the
+ * average of every particle transformed with its current parameters (the
+ * reference-free path's image transform in place of EMAN2's sum_oe); with
+ * center = -1 the average is moved by minus the mean of the particles' shifts
+ * after the driver's rotation (its "average shift" centring, :1547-1552); with
+ * cutoff > 0 it is low-passed with the tanh filter (absolute frequency; a
+ * stand-in for SPHIRE's ref_ali2d user function, which the reference does not
+ * ship); then every particle is aligned to the average and its shift
+ * accumulated (pre_align_run).  The last iteration builds the average only
+ * (:1563), which goes to d_average (device, may be NULL).  Queued on `stream`;
+ * returns after completion. */
+int isac_pre_align_loop_device(const float *d_imgs, unsigned int n_img, unsigned int img_dim,
+                               unsigned int ring_num, const float *shift_ranges,
+                               const float *shift_steps, unsigned int n_steps,
+                               unsigned int max_iter, int center, float cutoff, float falloff,
+                               void *d_aln_param, float *d_average, void *stream);
+
 /* Multi-alignment stability test of ISAC classes on the device: the step the
  * driver runs per class on the host with SPHIRE's
  * sp_pixel_error.multi_align_stability(aln_param[k], 0.0, 10000.0, thld_err,

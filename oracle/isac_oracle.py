@@ -489,7 +489,8 @@ def filter_shift(imgs, cutoff, falloff, dx, dy):
     """Per image n: multiply the 2-D spectrum by H(r) exp(-2 pi i (kx dx_n + ky
     dy_n) / D) and invert (numpy irfft2: the real part of the kx = 0 and kx = D/2
     columns), so the content moves by (dx_n, dy_n).
-    H(r) = 0.5 (tanh(c (r + f)) - tanh(c (r - f))), c = pi / (2 falloff f),
+    H(r) = 0.5 (tanh(c (r + f)) - tanh(c (r - f))), c = pi / (2 falloff f)
+    (H = 1 when cutoff <= 0),
     r = sqrt(kx^2 + min(ky, D - ky)^2) / D cycles per pixel; ky is signed
     (ky = D/2 taken as +D/2) in the phase ramp."""
     imgs = np.asarray(imgs, dtype=np.float64)
@@ -498,13 +499,57 @@ def filter_shift(imgs, cutoff, falloff, dx, dy):
     kx = np.arange(D // 2 + 1, dtype=np.float64)
     ky = np.arange(D, dtype=np.float64)
     r = np.sqrt((kx[None, :] / D) ** 2 + (np.minimum(ky, D - ky)[:, None] / D) ** 2)
-    cc = np.pi / (2.0 * falloff * cutoff)
-    H = 0.5 * (np.tanh(cc * (r + cutoff)) - np.tanh(cc * (r - cutoff)))
+    if cutoff > 0:
+        cc = np.pi / (2.0 * falloff * cutoff)
+        H = 0.5 * (np.tanh(cc * (r + cutoff)) - np.tanh(cc * (r - cutoff)))
+    else:                                                    # no low-pass: the ramp alone
+        H = np.ones_like(r)
     kys = np.where(2 * ky > D, ky - D, ky)
     dx = np.asarray(dx, np.float64).reshape(-1, 1, 1)
     dy = np.asarray(dy, np.float64).reshape(-1, 1, 1)
     ramp = np.exp(-2j * np.pi * (kx[None, None, :] * dx + kys[None, :, None] * dy) / D)
     return np.fft.irfft2(F * H[None] * ramp, s=(D, D))
+
+
+def pre_align_loop(sbj, ring_num, ranges, steps, max_iter, center=0, cutoff=0.0, falloff=0.0,
+                   acc_xy=None, return_table=False):
+    """The pre-alignment driver loop, bin/sp_isac_applications.py:1486-1693, with
+    the device path's stand-ins for EMAN2 (isac_pre_align_loop_device): per shift
+    step (reset_shifts, :1489) and iteration, the average of the particles
+    transformed with their current parameters (transform_images; sum_oe,
+    :1495), optionally moved by minus the mean shift after the driver's rotation
+    (center == -1, :1547-1552; the sums of the previous alignment, :1664-1690,
+    zero before the first) and low-passed (cutoff > 0), then each particle
+    aligned to it (single_ref_table + decode_single_ref, the pre_align_run
+    step); the last iteration builds the average only (:1563).
+    Returns (params dict of the last alignment, final average), plus the CCF
+    table of the last alignment with return_table."""
+    sbj = np.asarray(sbj, dtype=np.float64)
+    P, D = sbj.shape[0], sbj.shape[-1]
+    acc = np.zeros((P, 2)) if acc_xy is None else np.asarray(acc_xy, np.float64).reshape(P, 2).copy()
+    angle = np.zeros(P)
+    mirror = np.zeros(P, dtype=int)
+    total, it = len(ranges) * max_iter, 0
+    prm = tab = None
+    for rng_, step in zip(ranges, steps):
+        shifts = shift_grid(rng_, rng_, step)
+        for _ in range(max_iter):
+            it += 1
+            avg = transform_images(sbj, angle, acc[:, 0], acc[:, 1], mirror).mean(0)
+            shift_avg = center == -1 and it > 1
+            if shift_avg or cutoff > 0:
+                dx = dy = 0.0
+                if shift_avg:
+                    px, py = driver_post_rotation(angle, acc[:, 0], acc[:, 1])
+                    dx = -np.where(mirror != 0, -px, px).sum() / P
+                    dy = -py.sum() / P
+                avg = filter_shift(avg[None], cutoff, falloff, [dx], [dy])[0]
+            if it != total:
+                tab = single_ref_table(sbj, avg[None], np.zeros(P, int), acc, ring_num, shifts)
+                prm = decode_single_ref(tab, acc, shifts, D, ring_num)
+                acc = np.stack([prm["shift_x"], prm["shift_y"]], 1)
+                angle, mirror = prm["angle"], prm["mirror"]
+    return (prm, avg, tab) if return_table else (prm, avg)
 
 
 def update_references(imgs, peak_list, cls, n_ref, cutoff, falloff):

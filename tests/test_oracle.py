@@ -515,3 +515,34 @@ def test_update_references_recovers_references_from_exact_peaks():
     for k in range(3):
         back = O.filter_shift(refs[k:k + 1], *_pass_all(), [avg[k, 0]], [avg[k, 1]])[0]
         assert np.corrcoef(back[inside], ds.ref[k][inside])[0, 1] > 0.999
+
+
+def test_pre_align_loop_identical_particles_stay_put():
+    # copies of one image: the average is the image, each copy matches itself at
+    # zero shift / angle / mirror, so the loop keeps every parameter at zero and
+    # returns the image (with and without the average-shift centring)
+    img = synth.render_refs(synth.make_blobs(1, 10.0, 3), 33)[0].astype(np.float64)
+    sbj = np.repeat(img[None], 4, axis=0)
+    for center in (0, -1):
+        prm, avg = O.pre_align_loop(sbj, 12, [2.0, 1.0], [1.0, 1.0], 2, center=center)
+        assert np.all(prm["mirror"] == 0)
+        assert np.all(prm["shift_x"] == 0) and np.all(prm["shift_y"] == 0)
+        assert np.abs((prm["angle"] + 180.0) % 360.0 - 180.0).max() < 0.5
+        np.testing.assert_allclose(avg, img, atol=1e-9 * np.abs(img).max())
+
+
+def test_pre_align_loop_recovers_relative_shifts():
+    # noise-free copies moved by known integer shifts: after the loop the found
+    # shifts differ from the true ones by one common offset, and the angles and
+    # mirrors agree, so the transformed particles coincide
+    D = 33
+    img = synth.render_refs(synth.make_blobs(1, 8.0, 5), D)[0].astype(np.float64)
+    true = np.array([[0, 0], [2, -1], [-1, 2], [1, 1], [-2, -2], [0, 2]])
+    # particle content moved by (a, b): its pixel (x, y) holds img(x - a, y - b)
+    sbj = np.stack([np.roll(np.roll(img, b, 0), a, 1) for a, b in true])
+    prm, avg = O.pre_align_loop(sbj, 10, [3.0], [1.0], 3)
+    off = np.stack([prm["shift_x"], prm["shift_y"]], 1) - true
+    assert np.abs(off - off[0]).max() < 1e-9, off
+    assert np.all(prm["mirror"] == prm["mirror"][0])
+    da = (prm["angle"] - prm["angle"][0] + 180.0) % 360.0 - 180.0
+    assert np.abs(da).max() < 0.5
