@@ -381,6 +381,7 @@ struct BestOut {
     float *peak = nullptr;   // [P][4] angle, sx, sy, mirror
     int *tie = nullptr;      // [P] 1 when the runner-up was within tie_rel (re-scored in FP64)
     float tie_rel = 1e-5f;
+    const int *idx = nullptr;   // [P] or NULL: particle p's outputs go to slot idx[p]
 };
 
 constexpr int kCandidates = 8;
@@ -451,8 +452,9 @@ int run_mref_mixed(const float *d_sbj, const float *d_ref, ShiftView acc, int P,
                                           N, bf16, ws.rec.as<PeakRec>(), st));
             if (g.prof) { g.prof_ccf_ms += ps.finish(); g.prof_ccf_n++; }
         }
-        HIP_OK(isac::launch_decode_mref(ws.rec.as<PeakRec>(), n, N, a, ws.shifts_f.as<float2>(), S,
-                                        d_peak, peak_row, p0, d_d, d_pitch, st));
+        if (d_peak)
+            HIP_OK(isac::launch_decode_mref(ws.rec.as<PeakRec>(), n, N, a, ws.shifts_f.as<float2>(),
+                                            S, d_peak, peak_row, p0, d_d, d_pitch, st));
         if (bo) {
             const int K = kCandidates;
             HIP_OK(isac::launch_topk(ws.rec.as<PeakRec>(), n, N, K, ws.cand.as<int>(), st));
@@ -471,7 +473,7 @@ int run_mref_mixed(const float *d_sbj, const float *d_ref, ShiftView acc, int P,
             HIP_OK(isac::launch_decode_best(ws.rec_c.as<PeakRec>(), ws.cand.as<int>(),
                                             ws.best.as<int>(), ws.tie.as<int>(), n, K, a,
                                             ws.shifts_f.as<float2>(), S, p0, bo->ref, bo->val,
-                                            bo->peak, bo->tie, st));
+                                            bo->peak, bo->tie, st, bo->idx));
         }
     }
     HIP_OK(hipStreamSynchronize(st));
@@ -493,7 +495,7 @@ int run_proposal(const float *d_sbj, const float *d_ref, ShiftView acc, int P, i
                  int n_bins, float psi_half, int hw, float threshold, int edge_fb, int level2,
                  const float *conf, float conf_min, int *best_ref, float *best_val,
                  float *best_peak, int *fallback, unsigned *n_fb_out, unsigned *n_l2_out,
-                 hipStream_t st) {
+                 hipStream_t st, int fallback_form = 0) {
     const int S = (int)shifts.size();
     if (int e = check_shape(D, R, kRingLen, shifts.size())) return e;
     if (S > 64 || nx * ny != S) {
@@ -587,6 +589,31 @@ int run_proposal(const float *d_sbj, const float *d_ref, ShiftView acc, int P, i
                           (size_t)K * sizeof(int);
     int n_fb = 0;
     if (int e = flagged(1, INT_MAX, n_fb)) return e;
+    // level 3, MFMA form: the mixed-precision search with exact FP32 re-scoring of
+    // each particle's 8 best references (FP64 for near-ties), results scattered to
+    // the flagged particles
+    if (n_fb > 0 && fallback_form != 0) {
+        const int fb = batch_for((size_t)D * D * sizeof(float) + sizeof(float2), 0, 0.25f, n_fb);
+        HIP_OK(ws.fb_img.reserve((size_t)fb * D * D * sizeof(float)));
+        HIP_OK(ws.fb_acc.reserve((size_t)fb * sizeof(float2)));
+        for (int j0 = 0; j0 < n_fb; j0 += fb) {
+            const int m = std::min(fb, n_fb - j0);
+            const int *idx = ws.fb_idx.as<int>() + j0;
+            HIP_OK(isac::launch_gather(d_sbj, D, acc, idx, m, ws.fb_img.as<float>(),
+                                       ws.fb_acc.as<float2>(), st));
+            BestOut bo;
+            bo.ref = best_ref;
+            bo.val = best_val;
+            bo.peak = best_peak;
+            bo.idx = idx;
+            if (int e = run_mref_mixed(ws.fb_img.as<float>(), d_ref, ShiftView{ws.fb_acc.as<float>(), 2},
+                                       m, N, D, R, shifts, fallback_form == 2, nullptr, 0, nullptr,
+                                       0, 0.9f, st, &bo))
+                return e;
+        }
+        if (n_fb_out) *n_fb_out = (unsigned)n_fb;
+        return 0;
+    }
     // level 3: the exhaustive search
     if (n_fb > 0) {
         const int fb = batch_for(per_fb, 0, 0.9f, n_fb);
@@ -1687,9 +1714,10 @@ int isac_mref_proposal_device(const float *d_sbj, const float *d_ref, const floa
                               float shift_rng_x, float shift_rng_y, const int *d_prop,
                               unsigned int n_prop, unsigned int n_psi_bins, float psi_half_deg,
                               unsigned int shift_half, float threshold, int edge_fallback,
-                              int level2, const float *d_conf, float conf_min, int *d_best_ref,
-                              float *d_best_val, float *d_best_peak, int *d_fallback,
-                              unsigned int *n_level2, unsigned int *n_fallback, void *stream) {
+                              int level2, const float *d_conf, float conf_min,
+                              int fallback_form, int *d_best_ref, float *d_best_val,
+                              float *d_best_peak, int *d_fallback, unsigned int *n_level2,
+                              unsigned int *n_fallback, void *stream) {
     if (int e = ensure_device_ws(-1)) return e;
     std::vector<float2> sh = make_shift_grid(shift_rng_x, shift_rng_y, shift_step);
     if (int e = check_shape((int)img_dim, (int)ring_num, (int)ring_len, sh.size())) return e;
@@ -1699,9 +1727,9 @@ int isac_mref_proposal_device(const float *d_sbj, const float *d_ref, const floa
     if (!d_sbj || !d_ref || !d_prop || !d_best_ref || !d_best_val || !d_best_peak || !d_fallback ||
         n_prop < 1 || n_prop > (unsigned)isac::kMaxProposals || n_psi_bins < 1 || n_psi_bins > 256 ||
         !(psi_half_deg > 0.0f && psi_half_deg <= 180.0f) || shift_half > 2 ||
-        n_sbj > (1u << 30) || n_ref > (1u << 20)) {
+        n_sbj > (1u << 30) || n_ref > (1u << 20) || fallback_form < 0 || fallback_form > 2) {
         set_error(kErrArg, "isac_mref_proposal_device: bad arguments (1..8 proposals, psi bins "
-                           "1..256, 0 < psi_half <= 180, shift_half <= 2)");
+                           "1..256, 0 < psi_half <= 180, shift_half <= 2, fallback form 0..2)");
         return kErrArg;
     }
     int ny = 0;
@@ -1712,7 +1740,7 @@ int isac_mref_proposal_device(const float *d_sbj, const float *d_ref, const floa
                         (int)ring_num, sh, nx, ny, d_prop, (int)n_prop, (int)n_psi_bins, psi_half_deg,
                         (int)shift_half, threshold, edge_fallback != 0, level2 != 0, d_conf,
                         conf_min, d_best_ref, d_best_val, d_best_peak, d_fallback, n_fallback,
-                        n_level2, st);
+                        n_level2, st, fallback_form);
 }
 
 int isac_mref_align_batched_device(const IsacMrefSubset *subsets, unsigned int n_subsets,

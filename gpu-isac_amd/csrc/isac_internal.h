@@ -180,11 +180,13 @@ hipError_t launch_rescore_f64(const float2 *spec, const float2 *norm, const floa
                               const int *tie, int n_img, int K, int n_shift, int n_ring,
                               int n_ref, double *val64, hipStream_t st);
 
-// Per-particle best candidate -> outputs at particle p_off + p.
+// Per-particle best candidate -> outputs at particle p_off + p (idx != NULL: at
+// particle idx[p_off + p]); tie_out may be NULL.
 hipError_t launch_decode_best(const PeakRec *rec, const int *cand, const int *best,
                               const int *tie, int n_img, int K, ShiftView acc,
                               const float2 *shifts_f, int n_shift, int p_off, int *best_ref,
-                              float *best_val, float *best_peak, int *tie_out, hipStream_t st);
+                              float *best_val, float *best_peak, int *tie_out, hipStream_t st,
+                              const int *idx = nullptr);
 
 // Multi-reference decode into the caller's d / peak_list layout.
 hipError_t launch_decode_mref(const PeakRec *rec, int n_img, int n_ref, ShiftView acc,

@@ -1560,7 +1560,7 @@ __global__ void k_decode_best(const PeakRec *__restrict__ rec, const int *__rest
                               int K, ShiftView acc, const float2 *__restrict__ shifts_f,
                               int n_shift, int p_off, int *__restrict__ best_ref,
                               float *__restrict__ best_val, float *__restrict__ best_peak,
-                              int *__restrict__ tie_out) {
+                              int *__restrict__ tie_out, const int *__restrict__ idx) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_img) return;
     const int k = max(best[p], 0);
@@ -1571,14 +1571,15 @@ __global__ void k_decode_best(const PeakRec *__restrict__ rec, const int *__rest
         ay = acc.base[(size_t)p * acc.stride + 1];
     }
     const float2 sh = shifts_f[min(max(r.s, 0), n_shift - 1)];
-    const size_t pp = (size_t)p_off + p;
+    // output slot: p_off + p, or idx[p_off + p] (a scatter to the caller's particles)
+    const size_t pp = idx ? (size_t)idx[(size_t)p_off + p] : (size_t)p_off + p;
     best_ref[pp] = cand[(size_t)p * K + k];
     best_val[pp] = r.val;
     best_peak[4 * pp + 0] = wizard_angle(r);
     best_peak[4 * pp + 1] = ax + sh.x;
     best_peak[4 * pp + 2] = ay + sh.y;
     best_peak[4 * pp + 3] = r.m ? 1.0f : 0.0f;
-    tie_out[pp] = tie[p];
+    if (tie_out) tie_out[pp] = tie[p];
 }
 
 // Multi-reference decode (reference wizard, cuda/gpu_aln_noref.cu:1758-1793, with
@@ -2228,11 +2229,12 @@ hipError_t launch_rescore_f64(const float2 *spec, const float2 *norm, const floa
 hipError_t launch_decode_best(const PeakRec *rec, const int *cand, const int *best,
                               const int *tie, int n_img, int K, ShiftView acc,
                               const float2 *shifts_f, int n_shift, int p_off, int *best_ref,
-                              float *best_val, float *best_peak, int *tie_out, hipStream_t st) {
+                              float *best_val, float *best_peak, int *tie_out, hipStream_t st,
+                              const int *idx) {
     if (n_img <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_decode_best, dim3((n_img + 127) / 128), dim3(128), 0, st, rec, cand, best,
                        tie, n_img, K, acc, shifts_f, n_shift, p_off, best_ref, best_val, best_peak,
-                       tie_out);
+                       tie_out, idx);
     return hipGetLastError();
 }
 

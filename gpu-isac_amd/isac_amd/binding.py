@@ -159,7 +159,7 @@ def load(path: Optional[str] = None):
     lib.isac_mref_proposal_device.argtypes = [vp, vp, vp, c_uint, c_uint, c_uint, c_uint, c_uint,
                                               c_float, c_float, c_float, vp, c_uint, c_uint,
                                               c_float, c_uint, c_float, c_int, c_int, vp, c_float,
-                                              vp, vp, vp, vp, ctypes.POINTER(c_uint),
+                                              c_int, vp, vp, vp, vp, ctypes.POINTER(c_uint),
                                               ctypes.POINTER(c_uint), vp]
     lib.isac_mref_proposal_device.restype = c_int
     lib.isac_assign_groups_device.argtypes = [vp, c_uint, c_uint, c_uint, vp, vp]
@@ -570,7 +570,7 @@ def mref_proposal_device(sbj, ref, ring_num: int, shift_step: float, shift_rng_x
                          shift_rng_y: float, prop, threshold: float, n_psi_bins: int = 72,
                          psi_half: float = 10.0, shift_half: int = 1, edge_fallback: bool = True,
                          level2: bool = False, conf=None, conf_min: float = 0.0, acc_xy=None,
-                         stream=None):
+                         stream=None, fallback_form: str = "fp32"):
     """Narrowed search around proposals prop [P, K, 4] int32 (ref, mirror, psi
     bin, shift index).  Particles below ``threshold`` (or, with
     ``edge_fallback``, whose maximum lies on the edge of the admitted region)
@@ -578,12 +578,15 @@ def mref_proposal_device(sbj, ref, ring_num: int, shift_step: float, shift_rng_x
     those still below the threshold the exhaustive search, as do the particles
     whose ``conf`` [P] (float32) is below ``conf_min``.  Returns a dict of
     device tensors best_ref, best_val, best_peak [P, 4], fallback, plus
-    n_level2 and n_fallback (ints)."""
+    n_level2 and n_fallback (ints).  ``fallback_form`` "fp32" runs the exhaustive
+    search in FP32; "fp16" / "bf16" run it on MFMA with each particle's 8 best
+    references re-scored in FP32 (near-ties in FP64)."""
     import torch
     lib = load()
     P, D = sbj.shape[0], sbj.shape[-1]
     N = ref.shape[0]
     assert prop.dtype == torch.int32 and prop.is_contiguous() and prop.shape[0] == P
+    form = {"fp32": 0, "fp16": 1, "bf16": 2}[fallback_form]
     dev = sbj.device
     out = {"best_ref": torch.empty(P, dtype=torch.int32, device=dev),
            "best_val": torch.empty(P, dtype=torch.float32, device=dev),
@@ -595,7 +598,7 @@ def mref_proposal_device(sbj, ref, ring_num: int, shift_step: float, shift_rng_x
         sbj.data_ptr(), ref.data_ptr(), acc_xy.data_ptr() if acc_xy is not None else None, P, N, D,
         ring_num, RING_LEN, shift_step, shift_rng_x, shift_rng_y, prop.data_ptr(), prop.shape[1],
         n_psi_bins, psi_half, shift_half, threshold, 1 if edge_fallback else 0,
-        1 if level2 else 0, conf.data_ptr() if conf is not None else None, conf_min,
+        1 if level2 else 0, conf.data_ptr() if conf is not None else None, conf_min, form,
         out["best_ref"].data_ptr(), out["best_val"].data_ptr(),
         out["best_peak"].data_ptr(), out["fallback"].data_ptr(), ctypes.byref(nl2),
         ctypes.byref(nfb), st)

@@ -119,6 +119,8 @@ class Proposer:
     shift_half: int = 1
     level2: bool = True
     conf_min: float = 0.0
+    # exhaustive fallback on MFMA with exact FP32 re-scoring of the 8 best references
+    fallback_form: str = "fp16"
 
     def run(self, sbj, ref, acc_xy=None):
         prop, conf = self.net.propose(sbj, self.k, with_conf=True)
@@ -126,7 +128,8 @@ class Proposer:
                                             self.shift_rng, self.shift_rng, prop, self.threshold,
                                             n_psi_bins=self.net.n_psi, psi_half=self.psi_half,
                                             shift_half=self.shift_half, level2=self.level2,
-                                            conf=conf, conf_min=self.conf_min, acc_xy=acc_xy)
+                                            conf=conf, conf_min=self.conf_min, acc_xy=acc_xy,
+                                            fallback_form=self.fallback_form)
 
 
 def calibrate(net, sbj, ref, d, peak, ring_num, shift_rng, target=0.997, use_conf=True, **kw):

@@ -225,7 +225,11 @@ int isac_mref_best_mixed_device(const float *d_sbj, const float *d_ref, const fl
  *     the exhaustive search (fallback = 1, *n_fallback particles); so do,
  *     directly, the particles whose d_conf (or NULL: none) is below conf_min
  *     (fallback = 2), e.g. the network's own estimate that its proposals hold
- *     the true reference.
+ *     the true reference.  fallback_form selects that search: 0 = the FP32
+ *     search of isac_mref_align_device; 1 (FP16) or 2 (BF16) = the MFMA search
+ *     with each particle's 8 best references re-scored in FP32 and near-ties
+ *     in FP64 (isac_mref_best_mixed_device), which equals form 0 whenever the
+ *     FP32 optimum is among the 8.
  * Outputs per particle: best reference, its peak value, {angle, shift_x,
  * shift_y, mirror} in d_best_peak [n_sbj][4], the fallback flag.  Returns after
  * the work has completed. */
@@ -235,9 +239,10 @@ int isac_mref_proposal_device(const float *d_sbj, const float *d_ref, const floa
                               float shift_rng_x, float shift_rng_y, const int *d_prop,
                               unsigned int n_prop, unsigned int n_psi_bins, float psi_half_deg,
                               unsigned int shift_half, float threshold, int edge_fallback,
-                              int level2, const float *d_conf, float conf_min, int *d_best_ref,
-                              float *d_best_val, float *d_best_peak, int *d_fallback,
-                              unsigned int *n_level2, unsigned int *n_fallback, void *stream);
+                              int level2, const float *d_conf, float conf_min,
+                              int fallback_form, int *d_best_ref, float *d_best_val,
+                              float *d_best_peak, int *d_fallback, unsigned int *n_level2,
+                              unsigned int *n_fallback, void *stream);
 
 /* No counterpart in the reference (the reference aligns one subset per
  * multi_ref_align call, cuda/gpu_aln_noref.cu:326-554, driven once per ISAC

@@ -82,6 +82,31 @@ def test_fallback_is_the_exhaustive_search(gpu):
     assert bool(torch.all(o2["best_val"][~fb] >= thr))
 
 
+def test_mfma_fallback_equals_the_exhaustive_search(gpu):
+    # level 3 on MFMA: every particle falls back; the 8 best references by the
+    # FP16 search are re-scored with the FP32 search's own arithmetic, so the
+    # best reference, value and peak are the exhaustive search's, except where the
+    # runner-up is within 1e-5 (there FP64 decides)
+    import torch
+    sbj, ref = _data(P=200, N=20, seed=5)
+    d, peak = _exhaustive(sbj, ref)
+    bref, bval, bpeak = binding.best_of_exhaustive(d, peak)
+    prop = _labels_prop(d, peak)
+    prop[:, 0] = (prop[:, 0] + 1) % 20        # wrong references everywhere
+    for form in ("fp16", "bf16"):
+        o = binding.mref_proposal_device(sbj, ref, R, 1.0, RNG, RNG, prop[:, None, :].contiguous(),
+                                         threshold=3.0e38, fallback_form=form)
+        torch.cuda.synchronize()
+        assert o["n_fallback"] == 200 and int(o["fallback"].sum()) == 200
+        top2 = torch.topk(d, 2, dim=0).values
+        tie = (top2[0] - top2[1]) <= 1e-5 * top2[0].abs()
+        ok = ~tie
+        print(form, "near-ties", int(tie.sum()), flush=True)
+        assert torch.equal(o["best_ref"][ok], bref[ok])
+        assert torch.equal(o["best_val"][ok], bval[ok])
+        assert torch.equal(o["best_peak"][ok], bpeak[ok])
+
+
 def test_narrowed_search_matches_window_oracle(gpu):
     """Random proposals (near and far from the optimum) against the FP64 oracle
     of the windows: same reference, mirror and shift, psi within 0.5 degrees,
