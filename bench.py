@@ -38,12 +38,48 @@ WORKLOADS = {
     "p1k_n8_d64": (1000, 8, 64, 28.0, 0.1, 2.0, True, 29, 2.0),
 }
 PEAK_FP32_TFLOPS = 157.3     # MI355X published FP32 peak
+PEAK_F16_TFLOPS = 2500.0     # MI355X published dense FP16 / BF16 MFMA peak
+PEAK_HBM_TBS = 8.0           # MI355X HBM3E
 RING_LEN = 256
 
 
 def flops_per_pair(S, R, L=RING_LEN):
     K = L // 2 + 1
     return 8.0 * S * R * K + 2.0 * S * 2.5 * L * math.log2(L)
+
+
+def mfma_issued_flops(P, N, S, R):
+    """MFMA flops the reference-resident search kernel (k_mref_rt, rings <= 48)
+    issues for P particles x N references: per workgroup (16 particles x 8
+    references) and 16-row block, 8 wavefronts x 16 slots x 2 tiles of 16 x 16
+    contraction steps (16x16x32 over rings 0..31, 16x16x16 over a further 16-ring
+    step or the 8-ring tail), and per valid (row, reference) pair 4 16x16x32 psi
+    steps.  Padding rows / references / rings count as issued."""
+    full16, rem = R // 16, R % 16
+    tail = 0 if rem == 0 else (8 if rem <= 8 else 16)
+    nks = full16 + (1 if tail == 16 else 0)
+    f32, f16 = 16 * 16 * 32 * 2, 16 * 16 * 16 * 2
+    per_tile = (f32 if nks >= 2 else 0) + (f16 if nks % 2 else 0) + (f16 if tail == 8 else 0)
+    groups, nb8, s_pad = -(-P // 16), -(-N // 8), max(S, 16)
+    contraction = groups * nb8 * s_pad * 8 * 16 * 2 * per_tile
+    psi = P * S * nb8 * 8 * 4 * f32
+    return float(contraction + psi)
+
+
+def algorithmic_bound_ms(P, N, S, R, L=RING_LEN, D=76):
+    """max(F_contr / P_mfma, (F_ifft + F_fwd) / P_fp32, B / BW) per step: the ring
+    contraction at the MFMA peak, the psi inverse FFTs and the ring FFTs at the
+    FP32 peak, the image bytes (read once) at the HBM peak."""
+    K = L // 2 + 1
+    pairs = P * N
+    f_contr = 8.0 * S * R * K
+    f_ifft = 2.0 * S * 2.5 * L * math.log2(L)
+    f_fwd = (S * R * 2.5 * L * math.log2(L)) / N       # per pair, amortised over references
+    b = 4.0 * D * D / N + 20.0
+    t = (pairs * f_contr / (PEAK_F16_TFLOPS * 1e12), pairs * (f_ifft + f_fwd) / (PEAK_FP32_TFLOPS * 1e12),
+         pairs * b / (PEAK_HBM_TBS * 1e12))
+    return max(t) * 1e3, {"contraction_mfma_ms": t[0] * 1e3, "ffts_fp32_ms": t[1] * 1e3,
+                          "hbm_ms": t[2] * 1e3}
 
 
 def wilson(k, n, z=1.96):
@@ -532,6 +568,8 @@ def main():
     if not args.no_forms:
         # the mixed-precision (MFMA) forms on the same workload, same timing protocol
         forms = {}
+        import ctypes
+        bound_ms, bound_parts = algorithmic_bound_ms(P, N, S, R, D=D)
         for name, fn in (
                 ("fp16", lambda: binding.mref_align_mixed_device(sbj, ref, R, 1.0, rng, rng, "fp16",
                                                                  acc_xy=acc, out=(peak2, dval2))),
@@ -542,12 +580,31 @@ def main():
             fn()
             torch.cuda.synchronize()
             k = max(2, args.steps // 2)
+            lib.isac_profile_enable(1)
             t0 = time.perf_counter()
             for _ in range(k):
                 fn()
             torch.cuda.synchronize()
             dt = (time.perf_counter() - t0) / k
-            forms[name] = {"value": P * N / dt, "ms_per_step": dt * 1e3, "speedup_vs_fp32": ms_step / (dt * 1e3)}
+            n_g, ms_g = ctypes.c_int(0), ctypes.c_double(0.0)
+            n_p, ms_p = ctypes.c_int(0), ctypes.c_double(0.0)
+            lib.isac_profile_read(ctypes.byref(n_g), ctypes.byref(ms_g), ctypes.byref(n_p),
+                                  ctypes.byref(ms_p))
+            lib.isac_profile_enable(0)
+            forms[name] = {"value": P * N / dt, "ms_per_step": dt * 1e3, "speedup_vs_fp32": ms_step / (dt * 1e3),
+                           "dtype": "fp16" if name != "bf16" else "bf16"}
+            if R <= 48 and n_g.value > 0:
+                # the search kernel's issued MFMA rate, its launches timed with HIP events
+                g_ms_step = ms_g.value / k
+                tflops = mfma_issued_flops(P, N, S, R) / (g_ms_step * 1e-3) / 1e12
+                forms[name]["roofline"] = {
+                    "bound": "mfma", "kernel": "k_mref_rt (16-bit ring contraction + FP16 psi DFT16 x DFT16 + search)",
+                    "achieved": tflops, "peak": PEAK_F16_TFLOPS, "unit": "TFLOP/s",
+                    "frac": tflops / PEAK_F16_TFLOPS, "kernel_ms_per_step": g_ms_step,
+                    "issued_flop_per_step": mfma_issued_flops(P, N, S, R)}
+            forms[name]["algorithmic_bound_ms_per_step"] = bound_ms
+            forms[name]["frac_of_algorithmic_bound"] = bound_ms / (dt * 1e3)
+        forms["algorithmic_bound_parts_ms"] = bound_parts
         res["mfma_forms"] = forms
     if not args.no_host_abi:
         # not `value`: the same search behind the reference's host-pointer ABI,

@@ -884,6 +884,17 @@ __global__ __launch_bounds__(kRtThreads, 1) void k_mref_rt(
         rm1 = vmax3(rm1, vmax3(e[1][0], e[1][1], e[1][2]), e[1][3]);
     };
 
+    // rows r0 .. r0 + n - 1 with shift rows sg0 .. : two rows per step, so their
+    // MFMA -> VALU -> MFMA chains interleave
+    auto psi_rows = [&](int r0, int n, int sg0) {
+        int r = 0;
+        for (; r + 2 <= n; r += 2) {
+            psi_row(r0 + r, sg0 + r);
+            psi_row(r0 + r + 1, sg0 + r + 1);
+        }
+        if (r < n) psi_row(r0 + r, sg0 + r);
+    };
+
     for (int jb = 0; jb < S_pad; ++jb) {
         const unsigned pblk = (unsigned)jb * BB;
         // ---- contraction: this wavefront's 16 slots for both 8-row tiles
@@ -937,12 +948,12 @@ __global__ __launch_bounds__(kRtThreads, 1) void k_mref_rt(
         // the rows split at most once between the open particle and the next
         const int split = min(16, S_pad - sg_open);
         const int rows1 = (p_open < P) ? min(split, max(S - sg_open, 0)) : 0;
-        for (int r = 0; r < rows1; ++r) psi_row(r, sg_open + r);
+        psi_rows(0, rows1, sg_open);
         if (split < 16) {
             flush(p_open);
             ++p_open;
             const int rows2 = (p_open < P) ? min(16 - split, S) : 0;
-            for (int r = 0; r < rows2; ++r) psi_row(split + r, r);
+            psi_rows(split, rows2, 0);
             sg_open = 16 - split;
         } else {
             sg_open += 16;
