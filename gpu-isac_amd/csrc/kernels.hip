@@ -17,6 +17,7 @@
 #include <cfloat>
 #include <climits>
 #include <cmath>
+#include <type_traits>
 
 namespace isac {
 namespace {
@@ -328,6 +329,7 @@ __device__ __forceinline__ float h2hi(unsigned v) {
 }
 
 constexpr int kFragMaxRings = 40;
+constexpr int kFragBorder = 8;   // border of k_polar_frag's image copy, pixels
 
 template <bool BF16>
 __global__ __launch_bounds__(256, 2) void k_polar_frag(
@@ -375,9 +377,13 @@ __global__ __launch_bounds__(256, 2) void k_polar_frag(
     double isd = sqrt(fmax(t2 / (double)(D * D) - im * im, 0.0));
     if (!(isd > 1e-30)) isd = 1.0;
     const float mimg = (float)im, rs = (float)(1.0 / isd);
-    for (int i = tid; i < (D + 2) * (D + 2); i += 256) {   // bordered copy (bilinear_bordered)
-        const int r = i / (D + 2), c = i - r * (D + 2);
-        img[i] = (src[min(max(r - 1, 0), D - 1) * D + min(max(c - 1, 0), D - 1)] - mimg) * rs;
+    // standardised copy with a kFragBorder-pixel border that repeats the edge
+    // (cell c of a row holds pixel clamp(c - kFragBorder))
+    const int W = D + 2 * kFragBorder;
+    for (int i = tid; i < W * W; i += 256) {
+        const int r = i / W, c = i - r * W;
+        img[i] = (src[min(max(r - kFragBorder, 0), D - 1) * D + min(max(c - kFragBorder, 0), D - 1)] -
+                  mimg) * rs;
     }
     fill_fft_twiddles(tw, twg, tid, 256);
     __syncthreads();
@@ -392,51 +398,70 @@ __global__ __launch_bounds__(256, 2) void k_polar_frag(
     float2 *b0 = fb + wave * 512;
     float2 *b1 = b0 + 256;
     const int pl = (64 - lane) & 63;
-    for (int j = wave; j < n_rows; j += 4) {
-        const int2 si = shift_i[j];
-        const float cx = (float)(D / 2 + si.x) + ax, cy = (float)(D / 2 + si.y) + ay;
-        // ring i, slot lane + 64 h: packed FP16 (Re + Im, Re - Im); slot 0: (S[0], S[128])
-        unsigned sp[kFragMaxRings][2];
-        double sA = 0.0, sQ = 0.0;
-#pragma unroll
-        for (int i = 0; i < kFragMaxRings; i += 2) {
-            if (i < R) {   // uniform
-                const bool hb = (i + 1) < R;
-                const float ra = (float)(i + 1), rb = (float)(i + 2);
-                float2 v[4];
-                float pa = 0.f, qa = 0.f, pb = 0.f, qb = 0.f;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const float va = bilinear_bordered(img, D, cx + ra * cq[q], cy + ra * sq[q]);
-                    const float vb = hb ? bilinear_bordered(img, D, cx + rb * cq[q], cy + rb * sq[q]) : 0.0f;
-                    v[q] = f2(va, vb);
-                    pa += va;
-                    qa += va * va;
-                    pb += vb;
-                    qb += vb * vb;
-                }
-                sA += (double)ra * pa + (double)rb * pb;
-                sQ += (double)ra * qa + (double)rb * qb;
-                fft256_wave<false>(v, b0, b1, tw, lane);
-                const float2 u2 = shfl2(v[2], pl), u3 = shfl2(v[3], pl);
-                const float2 zl = v[0], zlp = lane ? u3 : v[0];
-                const float2 zh = v[1], zhp = lane ? u2 : v[3];
-                float2 al = f2(0.5f * (zl.x + zlp.x), 0.5f * (zl.y - zlp.y));
-                float2 bl = f2(0.5f * (zl.y + zlp.y), 0.5f * (zlp.x - zl.x));
-                const float2 ah = f2(0.5f * (zh.x + zhp.x), 0.5f * (zh.y - zhp.y));
-                const float2 bh = f2(0.5f * (zh.y + zhp.y), 0.5f * (zhp.x - zh.x));
-                if (lane == 0) {
-                    // slot 0 keeps the two real bins (S[0], S[128]) instead of (Re+Im, Re-Im)
-                    sp[i][0] = h2pack(zl.x, v[2].x);
-                    if (i + 1 < kFragMaxRings) sp[i + 1][0] = h2pack(zl.y, v[2].y);
-                } else {
-                    sp[i][0] = h2pack(al.x + al.y, al.x - al.y);
-                    if (i + 1 < kFragMaxRings) sp[i + 1][0] = h2pack(bl.x + bl.y, bl.x - bl.y);
-                }
-                sp[i][1] = h2pack(ah.x + ah.y, ah.x - ah.y);
-                if (i + 1 < kFragMaxRings) sp[i + 1][1] = h2pack(bh.x + bh.y, bh.x - bh.y);
-            }
+    const float Wf = (float)W;
+
+    // bilinear sample at (X, Y) of the bordered copy (coordinates include the
+    // border).  INSIDE: the caller guarantees 0 <= floor(X), floor(Y) <= W - 2,
+    // so the cell index is one exact FP32 multiply-add.  Otherwise the cell is
+    // clamped to the copy, which reads the same edge values as clamping each tap
+    // to the image.
+    auto tap = [&](float X, float Y, auto inside_tag) __attribute__((always_inline)) {
+        const float x0 = floorf(X), y0 = floorf(Y);
+        const float fx = X - x0, fy = Y - y0;
+        const float *q;
+        if constexpr (decltype(inside_tag)::value) {
+            q = img + (int)fmaf(y0, Wf, x0);
+        } else {
+            const int cx = min(max((int)x0, 0), W - 2), cy = min(max((int)y0, 0), W - 2);
+            q = img + (int)__umul24((unsigned)cy, (unsigned)W) + cx;
         }
+        const float v00 = q[0], v01 = q[1], v10 = q[W], v11 = q[W + 1];
+        const float t0 = fmaf(fx, v01 - v00, v00), t1 = fmaf(fx, v11 - v10, v10);
+        return fmaf(fy, t1 - t0, t0);
+    };
+
+    // ring pair (i, i + 1) of one shift row from the lane's samples of both
+    // rings: normalisation sums, one complex FFT, both spectra packed as FP16
+    // (Re + Im, Re - Im); slot 0: (S[0], S[128])
+    auto ring_pair = [&](int i, const float (&va)[4], const float (&vb)[4],
+                         unsigned (&sp)[kFragMaxRings][2], double &sA, double &sQ)
+        __attribute__((always_inline)) {
+        const float ra = (float)(i + 1), rb = (float)(i + 2);
+        float2 v[4];
+        float pa = 0.f, qa = 0.f, pb = 0.f, qb = 0.f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            v[q] = f2(va[q], vb[q]);
+            pa += va[q];
+            qa = fmaf(va[q], va[q], qa);
+            pb += vb[q];
+            qb = fmaf(vb[q], vb[q], qb);
+        }
+        sA += (double)ra * pa + (double)rb * pb;
+        sQ += (double)ra * qa + (double)rb * qb;
+        fft256_wave<false>(v, b0, b1, tw, lane);
+        const float2 u2 = shfl2(v[2], pl), u3 = shfl2(v[3], pl);
+        const float2 zl = v[0], zlp = lane ? u3 : v[0];
+        const float2 zh = v[1], zhp = lane ? u2 : v[3];
+        const float2 al = f2(0.5f * (zl.x + zlp.x), 0.5f * (zl.y - zlp.y));
+        const float2 bl = f2(0.5f * (zl.y + zlp.y), 0.5f * (zlp.x - zl.x));
+        const float2 ah = f2(0.5f * (zh.x + zhp.x), 0.5f * (zh.y - zhp.y));
+        const float2 bh = f2(0.5f * (zh.y + zhp.y), 0.5f * (zhp.x - zh.x));
+        // slot 0 (lane 0) keeps the two real bins (S[0], S[128]) instead of
+        // (Re + Im, Re - Im): selected, not branched
+        const unsigned na = h2pack(al.x + al.y, al.x - al.y), za = h2pack(zl.x, v[2].x);
+        sp[i][0] = lane ? na : za;
+        if (i + 1 < kFragMaxRings) {
+            const unsigned nb = h2pack(bl.x + bl.y, bl.x - bl.y), zb = h2pack(zl.y, v[2].y);
+            sp[i + 1][0] = lane ? nb : zb;
+        }
+        sp[i][1] = h2pack(ah.x + ah.y, ah.x - ah.y);
+        if (i + 1 < kFragMaxRings) sp[i + 1][1] = h2pack(bh.x + bh.y, bh.x - bh.y);
+    };
+
+    // row j's normalisation and 16-bit operand stores
+    auto store_row = [&](int j, const unsigned (&sp)[kFragMaxRings][2], double sA, double sQ)
+        __attribute__((always_inline)) {
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
             sA += __shfl_xor(sA, off, kWave);
@@ -512,6 +537,43 @@ __global__ __launch_bounds__(256, 2) void k_polar_frag(
                     }
             }
         }
+    };
+
+    // all rings of row j: sample position centre + shift + (i + 1)(cos, sin)
+    auto row = [&](int j, float cx, float cy, auto inside_tag) __attribute__((always_inline)) {
+        unsigned sp[kFragMaxRings][2];
+        double sA = 0.0, sQ = 0.0;
+#pragma unroll
+        for (int i = 0; i < kFragMaxRings; i += 2) {
+            if (i < R) {   // uniform
+                const bool hb = (i + 1) < R;
+                const float ra = (float)(i + 1), rb = (float)(i + 2);
+                float va[4], vb[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    // ring i + 2 of an odd R is sampled (inside the checked extent) and
+                    // then zeroed, so the loop body has no branch per sample
+                    va[q] = tap(fmaf(ra, cq[q], cx), fmaf(ra, sq[q], cy), inside_tag);
+                    const float t = tap(fmaf(rb, cq[q], cx), fmaf(rb, sq[q], cy), inside_tag);
+                    vb[q] = hb ? t : 0.0f;
+                }
+                ring_pair(i, va, vb, sp, sA, sQ);
+            }
+        }
+        store_row(j, sp, sA, sQ);
+    };
+
+    for (int j = wave; j < n_rows; j += 4) {
+        const int2 sh = shift_i[j];
+        const float cx = (float)(D / 2 + kFragBorder + sh.x) + ax;
+        const float cy = (float)(D / 2 + kFragBorder + sh.y) + ay;
+        // uniform: every cell of the row's rings lies in the bordered copy (a
+        // one-pixel margin covers the rounding of the positions)
+        const float ext = (float)(R + 1), hi = (float)(W - 2);
+        if (cx - ext >= 0.0f && cy - ext >= 0.0f && cx + ext <= hi && cy + ext <= hi)
+            row(j, cx, cy, std::true_type{});
+        else
+            row(j, cx, cy, std::false_type{});
     }
 }
 
@@ -2033,11 +2095,14 @@ hipError_t prepare_launches() {
         e = hipFuncSetAttribute((const void *)k_polar_spectra<true>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, lds);
         if (e != hipSuccess) return e;
+        const int lds_frag = (int)((256 + 4 * 512) * sizeof(float2) +
+                                   (kMaxImgDim + 2 * kFragBorder) * (kMaxImgDim + 2 * kFragBorder) *
+                                       sizeof(float));
         e = hipFuncSetAttribute((const void *)k_polar_frag<false>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds_frag);
         if (e != hipSuccess) return e;
         e = hipFuncSetAttribute((const void *)k_polar_frag<true>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds_frag);
         if (e != hipSuccess) return e;
         lds_raised = true;
     }
@@ -2066,7 +2131,8 @@ hipError_t launch_polar_frag(const float *imgs, int n_img, int img_dim, ShiftVie
     if (!polar_frag_supported(n_ring)) return hipErrorInvalidValue;
     const GemmLayout L = gemm_layout(n_ring);
     const int S_pad = gemm_rows_per_particle(n_shift);
-    const size_t lds = (256 + 4 * 512) * sizeof(float2) + (size_t)(img_dim + 2) * (img_dim + 2) * sizeof(float);
+    const size_t lds = (256 + 4 * 512) * sizeof(float2) +
+                       (size_t)(img_dim + 2 * kFragBorder) * (img_dim + 2 * kFragBorder) * sizeof(float);
     if (hipError_t e = prepare_launches()) return e;
     const dim3 grid((unsigned)((n_img + 15) / 16 * 16));
     if (bf16)
