@@ -17,6 +17,12 @@ Also reported:
                 FP64 build checks the GPU parameters on that sample
   host_abi      the same search behind the reference's host-pointer ABI
                 (multi_ref_align: host images in, PCIe both ways), median of 5
+  mfma_forms    the FP16 / BF16 matrix-core forms and the FP16 form with the exact
+                FP32 best per particle, each with its own roofline: the MFMA flops
+                the ring kernel (k_mref_rt) issues per step over its HIP-event
+                time against the 2.5 PFLOP/s dense peak, and the step time against
+                the algorithmic bound max(contraction / 2.5 PFLOP/s, (psi IFFTs +
+                ring FFTs) / 157.3 TFLOP/s, image bytes / 8 TB/s)
 """
 from __future__ import annotations
 
@@ -381,14 +387,18 @@ def learned_report(n_train=200000, n_cal=10000, n_test=100000, epochs=16, oracle
     te, _ = make(n_test, 31)
     d, peak = learned.exhaustive(te, ref, R, rng)
 
-    def timed(fn, reps=2):
+    def timed(fn, reps=3):
+        # one warm-up, then the median of reps timed runs (a host stall in one run
+        # does not move it)
         fn()
         torch.cuda.synchronize()
-        t = time.perf_counter()
+        ts = []
         for _ in range(reps):
+            t = time.perf_counter()
             r = fn()
-        torch.cuda.synchronize()
-        return (time.perf_counter() - t) / reps, r
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t)
+        return sorted(ts)[len(ts) // 2], r
 
     t_plain, _ = timed(lambda: learned.exhaustive(te, ref, R, rng, chunk=n_test))
     t_mfma, _ = timed(lambda: binding.mref_best_mixed_device(te, ref, R, 1.0, rng, rng, "fp16"))

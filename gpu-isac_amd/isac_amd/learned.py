@@ -26,7 +26,10 @@ NET = dict(channels=(32, 64, 128, 32), hidden=384, pool4=False)
 TRAIN = dict(batch=256, lr=2e-3, wd=1e-4, warmup=1000, sigma_psi=1.0, sigma_shift=0.7)
 # Agreement with the exhaustive search that the fallback threshold is calibrated
 # for, on the calibration set (the held-out agreement comes out a little lower).
-TARGET = 0.998
+# Held-out, 100 000 particles: 0.997 -> 7.1 % fallback, 99.62 % agreement with the
+# FP64 oracle (10 000-particle sample); 0.9975 -> 12.6 %, 99.71 %; 0.998 -> 20.4 %,
+# 99.78 %.  0.9975 keeps the sample's 95 % interval above 99.5 %.
+TARGET = 0.9975
 
 
 def exhaustive(sbj, ref, ring_num, shift_rng, shift_step=1.0, chunk=20000):
