@@ -340,12 +340,12 @@ def iteration_report(steps=5, warmup=1, fl=0.25, ff=0.2):
             "filter": {"cutoff": fl, "falloff": ff}}
 
 
-def learned_report(n_train=200000, n_cal=10000, n_test=100000, epochs=16, oracle_sample=10000,
+def learned_report(n_train=200000, n_cal=None, n_test=100000, epochs=None, oracle_sample=10000,
                    target=None, snr=0.1, integer_shifts=True, log=None):
     """Config 5: the pose-proposal network trained on exhaustive-search labels of
     200,000 particles drawn like the small workload's (blob references, uniform psi and mirror,
     integer shifts, SNR 0.1) at 76 px, 100 references, +-3 px (49 shifts);
-    threshold calibrated on 10,000 more; evaluated on 100,000 held-out
+    threshold calibrated on 40,000 more; evaluated on 100,000 held-out
     particles: agreement with the exhaustive FP32 search and with the FP64
     oracle (sample), fallback rate, end-to-end time against the plain FP32
     search and the MFMA form, and the same search with an untrained network."""
@@ -355,6 +355,8 @@ def learned_report(n_train=200000, n_cal=10000, n_test=100000, epochs=16, oracle
     from oracle import coracle
     log = log or (lambda m: print(m, file=sys.stderr, flush=True))
     target = learned.TARGET if target is None else target
+    n_cal = learned.N_CAL if n_cal is None else n_cal
+    epochs = learned.EPOCHS if epochs is None else epochs
     N, D, R, rng = 100, 76, 35, 3.0
 
     def make(P, seed):

@@ -770,72 +770,113 @@ __global__ void k_relu_bwd(const unsigned short *__restrict__ y, const unsigned 
 }
 
 // ------------------------------------------------------------ weight gradient
-// Tile: 64 output channels x 64 columns k of [K+1] (k = tap*Cin + ci; k == K is
-// the bias column, input 1).  Blocks along z take pixel ranges [z*chunk,
-// (z+1)*chunk); partial sums go to part[z][Cout][K+1].
+// dW[co][k] = sum over pixels P of dz[P][co] x_col[P][k], k = tap Cin + ci (k == K
+// is the bias column, input 1), on the matrix cores with the pixels as the
+// contraction.  Tile: 64 output channels x 64 columns k per workgroup; wavefront
+// (wm, wn) owns the 32 x 32 quadrant (co 32 wm.., k 32 wn..) on
+// v_mfma_f32_32x32x16_bf16.  Per stage of 64 pixels both operands are staged in
+// LDS transposed to [column][pixel], two pixels per dword, so a lane's 8
+// consecutive pixels are one 16-byte read (A: co = l % 32, pixels 16 s + 8 (l / 32)
+// ..; B: k = l % 32, the same pixels).  Products of bf16 values are exact in FP32.
+// Blocks along z take pixel ranges [z chunk, (z + 1) chunk) and write partial
+// sums part[z][Cout][K + 1], added in a fixed order by k_sum_parts.
+constexpr int kWgPix = 64;              // pixels per stage
+constexpr int kWgRow = kWgPix + 8;      // bf16 per LDS row: rows 8 apart fall 32 banks apart
+
 __global__ __launch_bounds__(256) void k_wgrad(const unsigned short *__restrict__ x,
                                                const unsigned short *__restrict__ dz, int n, int H,
                                                int W, int Cin, int Cout, int KS, long long chunk,
                                                float *__restrict__ part) {
-    __shared__ float sd[32][64];
-    __shared__ float sx[32][64];
-    const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+    __shared__ __align__(16) unsigned short sd[64 * kWgRow];
+    __shared__ __align__(16) unsigned short sx[64 * kWgRow];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int K = KS * KS * Cin, K1 = K + 1;
     const int k0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
     const long long NP = (long long)n * H * W;
     const long long p0 = (long long)blockIdx.z * chunk;
     const long long p1 = min(NP, p0 + chunk);
-    float acc[4][4];
+    const int wm = wave >> 1, wn = wave & 1;
+    // staging: pixel pair pp (stage pixels 2 pp, 2 pp + 1), column group cg (8 columns)
+    const int pp = tid & 31, cg = tid >> 5;
+    const int cb = c0 + 8 * cg, kb = k0 + 8 * cg;
+    const bool dvec = (Cout % 8 == 0) && cb + 8 <= Cout;
+    const bool xvec = (Cin % 8 == 0) && kb + 8 <= K;   // 8 channels of one tap
+    float16_t acc = {};
+    for (long long pc = p0; pc < p1; pc += kWgPix) {
+        unsigned dw[8], xw[8];   // (pixel 2 pp, pixel 2 pp + 1) per column
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+        for (int e = 0; e < 8; ++e) dw[e] = xw[e] = 0u;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
-    for (long long pc = p0; pc < p1; pc += 32) {
-        __syncthreads();
-        for (int e = tid; e < 32 * 64; e += 256) {
-            const int pr = e >> 6, col = e & 63;
-            const long long P = pc + pr;
-            float dv = 0.0f, xv = 0.0f;
-            if (P < p1) {
-                const int co = c0 + col;
-                if (co < Cout) dv = bf2f(dz[(size_t)P * Cout + co]);
-                const int k = k0 + col;
-                if (k == K) {
-                    xv = 1.0f;
-                } else if (k < K) {
-                    const int t = k / Cin, ci = k - t * Cin;
-                    const int img = (int)(P / ((long long)H * W));
-                    const int rem = (int)(P - (long long)img * H * W);
-                    const int sy = rem / W + t / KS - KS / 2, sx2 = rem % W + t % KS - KS / 2;
-                    if (sy >= 0 && sy < H && sx2 >= 0 && sx2 < W)
-                        xv = bf2f(x[(((size_t)img * H + sy) * W + sx2) * Cin + ci]);
+        for (int h = 0; h < 2; ++h) {
+            const long long P = pc + 2 * pp + h;
+            if (P >= p1) continue;
+            unsigned short d8[8], x8[8];
+            if (dvec) {
+                const uint4 v = *reinterpret_cast<const uint4 *>(dz + (size_t)P * Cout + cb);
+                const unsigned vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int e = 0; e < 8; ++e) d8[e] = (unsigned short)(vv[e / 2] >> (16 * (e & 1)));
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e)
+                    d8[e] = (cb + e < Cout) ? dz[(size_t)P * Cout + cb + e] : (unsigned short)0;
+            }
+            const int img = (int)(P / ((long long)H * W));
+            const int rem = (int)(P - (long long)img * H * W);
+            const int py = rem / W, px = rem - py * W;
+            if (xvec) {
+                const int t = kb / Cin, ci = kb - t * Cin;
+                const int sy = py + t / KS - KS / 2, sx2 = px + t % KS - KS / 2;
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                if (sy >= 0 && sy < H && sx2 >= 0 && sx2 < W)
+                    v = *reinterpret_cast<const uint4 *>(x + (((size_t)img * H + sy) * W + sx2) * Cin + ci);
+                const unsigned vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int e = 0; e < 8; ++e) x8[e] = (unsigned short)(vv[e / 2] >> (16 * (e & 1)));
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int k = kb + e;
+                    unsigned short xv = 0;
+                    if (k == K) {
+                        xv = 0x3f80;   // bf16 1.0: the bias column
+                    } else if (k < K) {
+                        const int t = k / Cin, ci = k - t * Cin;
+                        const int sy = py + t / KS - KS / 2, sx2 = px + t % KS - KS / 2;
+                        if (sy >= 0 && sy < H && sx2 >= 0 && sx2 < W)
+                            xv = x[(((size_t)img * H + sy) * W + sx2) * Cin + ci];
+                    }
+                    x8[e] = xv;
                 }
             }
-            sd[pr][col] = dv;
-            sx[pr][col] = xv;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                dw[e] |= (unsigned)d8[e] << (16 * h);
+                xw[e] |= (unsigned)x8[e] << (16 * h);
+            }
+        }
+        __syncthreads();   // the previous stage's operand reads are done
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            *reinterpret_cast<unsigned *>(sd + (8 * cg + e) * kWgRow + 2 * pp) = dw[e];
+            *reinterpret_cast<unsigned *>(sx + (8 * cg + e) * kWgRow + 2 * pp) = xw[e];
         }
         __syncthreads();
-#pragma unroll 4
-        for (int pr = 0; pr < 32; ++pr) {
-            const float4 a = *reinterpret_cast<const float4 *>(&sd[pr][ty * 4]);
-            const float4 b = *reinterpret_cast<const float4 *>(&sx[pr][tx * 4]);
-            const float av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] += av[i] * bv[j];
+        for (int s = 0; s < kWgPix / 16; ++s) {
+            const int off = 16 * s + 8 * (lane >> 5);
+            const uint4 a = *reinterpret_cast<const uint4 *>(sd + (32 * wm + (lane & 31)) * kWgRow + off);
+            const uint4 b = *reinterpret_cast<const uint4 *>(sx + (32 * wn + (lane & 31)) * kWgRow + off);
+            acc = mfma32_bf16(a, b, acc);
         }
     }
+    // D: co = 8 (v / 4) + 4 (l / 32) + v % 4 of the quadrant, k = l % 32
     float *out = part + (size_t)blockIdx.z * Cout * K1;
+    const int k = k0 + 32 * wn + (lane & 31);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int co = c0 + ty * 4 + i;
-        if (co >= Cout) continue;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int k = k0 + tx * 4 + j;
-            if (k < K1) out[(size_t)co * K1 + k] = acc[i][j];
-        }
+    for (int v = 0; v < 16; ++v) {
+        const int co = c0 + 32 * wm + 8 * (v / 4) + 4 * (lane >> 5) + (v & 3);
+        if (co < Cout && k < K1) out[(size_t)co * K1 + k] = acc[v];
     }
 }
 
@@ -1336,7 +1377,7 @@ int isac_nn_conv_wgrad(const uint16_t *x, const uint16_t *dz, int n, int H, int 
     nparts = std::min<long long>(nparts, std::max<long long>(1, NP / 512));
     nparts = std::min<long long>(nparts, 512);
     if (nparts < 1) return fail(2, "isac_nn_conv_wgrad: scratch smaller than one partial sum");
-    const long long chunk = ((NP + nparts - 1) / nparts + 31) / 32 * 32;
+    const long long chunk = ((NP + nparts - 1) / nparts + kWgPix - 1) / kWgPix * kWgPix;
     nparts = (NP + chunk - 1) / chunk;
     hipStream_t st = (hipStream_t)stream;
     dim3 grid((unsigned)((K1 + 63) / 64), (unsigned)((Cout + 63) / 64), (unsigned)nparts);

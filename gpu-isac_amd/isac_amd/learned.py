@@ -24,12 +24,17 @@ TIE_REL = 1e-5
 # Gaussian targets over neighbouring psi bins (sigma 1 bin) and shifts (0.7 step).
 NET = dict(channels=(32, 64, 128, 32), hidden=384, pool4=False)
 TRAIN = dict(batch=256, lr=2e-3, wd=1e-4, warmup=1000, sigma_psi=1.0, sigma_shift=0.7)
+# Training length and calibration of the learned form.  Four network seeds
+# (24 epochs, 40 000 calibration particles, target 0.997; 100 000 held out): 3.07-3.71x
+# the exhaustive FP32 search at 99.66-99.76 % agreement with the FP64 oracle
+# (10 000-particle sample).  With 10 000 calibration particles the confidence cut
+# rests on a few dozen particles and the fallback share varied from 13 to 32 %
+# between seeds at the same target.
+EPOCHS = 24
+N_CAL = 40000
 # Agreement with the exhaustive search that the fallback threshold is calibrated
-# for, on the calibration set (the held-out agreement comes out a little lower).
-# Held-out, 100 000 particles: 0.997 -> 7.1 % fallback, 99.62 % agreement with the
-# FP64 oracle (10 000-particle sample); 0.9975 -> 12.6 %, 99.71 %; 0.998 -> 20.4 %,
-# 99.78 %.  0.9975 keeps the sample's 95 % interval above 99.5 %.
-TARGET = 0.9975
+# for, on the calibration set.
+TARGET = 0.997
 
 
 def exhaustive(sbj, ref, ring_num, shift_rng, shift_step=1.0, chunk=20000):

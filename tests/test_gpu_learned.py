@@ -57,7 +57,7 @@ def test_trained_network_narrows_the_search(gpu):
 
 def test_config5_learned_form_against_fp64_oracle(gpu):
     """The learned form at its own workload (100 references, 76 px, +-3 px, SNR
-    0.1, integer shifts; 200 000 training and 10 000 calibration particles, the
+    0.1, integer shifts; 200 000 training and 40 000 calibration particles, the
     report's recipe) scored on 3 000 held-out particles against the FP64 C
     oracle: best reference (either of the oracle's top two when they tie within
     1e-5), mirror and shift exact, psi within 0.5 degree, for >= 99.5 %."""
@@ -74,10 +74,10 @@ def test_config5_learned_form_against_fp64_oracle(gpu):
     tr, ref = make(200000, 11)
     lab, _, _ = learned.labels(tr, ref, R, RNG)
     net = posenet.PoseNet(n_ref, 49, D=D, seed=0, **learned.NET)
-    hist = net.train(tr, lab, epochs=16, **learned.TRAIN)
+    hist = net.train(tr, lab, epochs=learned.EPOCHS, **learned.TRAIN)
     print("loss per epoch", [round(h, 3) for h in hist])
     del tr, lab
-    cal, _ = make(10000, 21)
+    cal, _ = make(learned.N_CAL, 21)
     _, dc, pc = learned.labels(cal, ref, R, RNG)
     pr, raw = learned.calibrate(net, cal, ref, dc, pc, R, RNG, target=learned.TARGET)
     del cal, dc, pc
