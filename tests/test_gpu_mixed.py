@@ -64,3 +64,45 @@ def test_mixed_vs_oracle_p20k_n100_d76(gpu, precision):
     print(precision, "100 refs, 76 px", st)
     assert st["d_rel_max"] <= (2e-3 if precision == "fp16" else 2e-2)
     assert st["agree_separated"] >= 0.995
+
+
+@pytest.mark.parametrize("precision,max_rel", [("fp16", 2e-3), ("bf16", 2e-2)])
+def test_mixed_rows_past_the_copy_border(gpu, precision, max_rel):
+    """Accumulated shifts of 8..11 px move some shift rows' discs (64 px, rings
+    1..29, +-2 px) past the operand kernel's 8-pixel image border: those rows take
+    the clamped taps, the others the clamp-free ones, within one particle.  Both
+    must match the FP64 oracle, which clamps every tap to the image.
+
+    With the disc partly outside the image some references correlate weakly
+    (|d| a few % of the particle's best), and the 16-bit rounding, which scales
+    with the operands rather than with d, is then a larger fraction of d.  So the
+    value bound is relative to the particle's largest |d| over references, and
+    the plain relative bound applies to pairs at least half as strong as that."""
+    import torch
+    ds = synth.dataset_p1k_n8_d64(n_particles=96)
+    rng = np.random.default_rng(5)
+    mag = rng.uniform(8.0, 11.0, (96, 2))
+    acc = (mag * rng.choice([-1.0, 1.0], (96, 2))).astype(np.float32)
+    acc[::4] = 0.0   # every fourth particle stays inside on every row
+    s = torch.from_numpy(ds.sbj).cuda()
+    r = torch.from_numpy(ds.ref).cuda()
+    a = torch.from_numpy(acc).cuda()
+    print(f"{precision}: 96 particles, |accumulated shift| up to {np.abs(acc).max():.1f} px", flush=True)
+    peak, d = binding.mref_align_mixed_device(s, r, 29, 1.0, 2.0, 2.0, precision, acc_xy=a)
+    torch.cuda.synchronize()
+    ref = coracle.mref(ds.sbj, ds.ref, acc, 29, 2.0, 2.0, 1.0)
+    st = _stats(peak.cpu().numpy(), d.cpu().numpy(), ref, precision)
+    print(precision, "rows past the border", st)
+    dg = d.cpu().numpy()
+    rel = np.abs(dg - ref["d"]) / np.abs(ref["d"])
+    scale = np.abs(ref["d"]) / np.abs(ref["d"]).max(axis=0, keepdims=True)
+    for i in np.argsort(rel, axis=None)[-5:][::-1]:
+        n, p_ = np.unravel_index(i, rel.shape)
+        print(f"  pair (ref {n}, particle {p_}): rel err {rel[n, p_]:.2e}, |d| / particle max "
+              f"{scale[n, p_]:.3f}, acc {acc[p_]}")
+    strong = scale >= 0.5
+    print(f"  error / particle max |d|: {(rel * scale).max():.2e}; "
+          f"relative on pairs >= half the max: {rel[strong].max():.2e}")
+    assert (rel * scale).max() <= max_rel / 2
+    assert rel[strong].max() <= max_rel
+    assert st["agree_separated"] >= 0.995
