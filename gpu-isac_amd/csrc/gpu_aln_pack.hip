@@ -847,13 +847,13 @@ int tune_group(const std::vector<BatchSub> &subs, BatchGroup &gr, int n_cu, bool
     return 0;
 }
 
-// Group workspace cap: 16 GiB (beyond it a larger group buys nothing), or
-// ISAC_BATCH_WORKSPACE_MB when set.
+// Group workspace cap: 16 GiB, or ISAC_BATCH_WORKSPACE_MB when set (the group
+// budget is also bounded by the free device memory).
 size_t batch_cap() {
     size_t cap = (size_t)16 << 30;
     if (const char *env = getenv("ISAC_BATCH_WORKSPACE_MB")) {
         const long long mb = atoll(env);
-        if (mb > 0) cap = std::min(cap, (size_t)mb << 20);
+        if (mb > 0) cap = (size_t)mb << 20;
     }
     return cap;
 }
