@@ -614,6 +614,11 @@ def main():
                     "achieved": tflops, "peak": PEAK_F16_TFLOPS, "unit": "TFLOP/s",
                     "frac": tflops / PEAK_F16_TFLOPS, "kernel_ms_per_step": g_ms_step,
                     "issued_flop_per_step": mfma_issued_flops(P, N, S, R)}
+                # per stage: the ring kernel (contraction + psi + search) and the rest of
+                # the step (16-bit operands from the images, decode, and for the exact-best
+                # form the FP32 spectra and re-scoring), beside the bound's terms
+                forms[name]["stages_ms_per_step"] = {
+                    "ring_kernel": g_ms_step, "operands_decode_rescoring": dt * 1e3 - g_ms_step}
             forms[name]["algorithmic_bound_ms_per_step"] = bound_ms
             forms[name]["frac_of_algorithmic_bound"] = bound_ms / (dt * 1e3)
         forms["algorithmic_bound_parts_ms"] = bound_parts
