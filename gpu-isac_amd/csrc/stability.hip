@@ -161,7 +161,9 @@ __global__ __launch_bounds__(kStabThreads) void k_stability(
     float *__restrict__ avg, int *__restrict__ stable, float *__restrict__ cls,
     float *__restrict__ gout) {
     const int c = blockIdx.x;
-    ClassView cv{prm, n_total, off[c], off[c + 1] - off[c], L};
+    int i0 = off[c], i1 = off[c + 1];
+    if (i0 < 0 || i1 < i0 || i1 > n_total) i0 = i1 = 0;   // a malformed range: an empty class
+    ClassView cv{prm, n_total, i0, i1 - i0, L};
     const int n = cv.n;
     const int npat = 1 << (L - 1);
     __shared__ int hist[1 << (kStabMaxRuns - 1)];

@@ -352,7 +352,8 @@ int isac_pre_align_loop_device(const float *d_imgs, unsigned int n_img, unsigned
  * reference tree).  d_params [n_runs][n_total][4] FP64 {angle deg, sx, sy,
  * mirror} of every member in every repeated run (the driver's aln_param[k],
  * :1381-1393, classes concatenated); class c holds members
- * [d_class_off[c], d_class_off[c+1]).  Per class: the mirror-flip pattern held
+ * [d_class_off[c], d_class_off[c+1]) (a range outside [0, n_total] or
+ * decreasing is taken as an empty class).  Per class: the mirror-flip pattern held
  * by most members selects the mirror-consistent members; one global rigid
  * transform per run (the last run fixed) is fitted to minimise the mean squared
  * pixel error over a circle of the given diameter, leaving out members above

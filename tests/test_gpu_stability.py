@@ -139,3 +139,41 @@ def test_rejections_and_mirror_counts(gpu):
     (st, rate, err, ex), = binding.multi_align_stability_device([c], 0.0, 1e4, 1e-6, 60.0)
     ref = SO.multi_align_stability(c, 0.0, 1e4, 1e-6, 60.0)
     assert st == [] and err == -1.0 == ref[2]
+
+
+def test_malformed_class_ranges_are_empty(gpu):
+    # through the C ABI: ranges that run backwards, start below 0 or end past
+    # n_total are empty classes (rate 0, error -1, nothing read or written)
+    # while the valid classes around them are computed as usual
+    import torch
+    rng = np.random.default_rng(3)
+    c0 = np.asarray(_class(rng, 3, 30, 1.0, 0.2, 0.0)).reshape(3, -1, 4)
+    c1 = np.asarray(_class(rng, 3, 25, 1.0, 0.2, 0.0)).reshape(3, -1, 4)
+    prm = np.ascontiguousarray(np.concatenate([c0, c1], axis=1))
+    n_total = prm.shape[1]
+    # classes 0 and 1 valid; 2: decreasing, 3: ends below 0, 4: starts below 0,
+    # 5: ends past n_total, 6: starts past n_total
+    off = np.array([0, 30, 55, 20, -5, 10, 90, 60], np.int32)
+    n_class = len(off) - 1
+    lib = binding.load()
+    dev = torch.device("cuda")
+    d_prm = torch.from_numpy(prm).to(dev)
+    d_off = torch.from_numpy(off).to(dev)
+    pix = torch.full((n_total,), 7.0, dtype=torch.float32, device=dev)
+    avg = torch.zeros((n_total, 4), dtype=torch.float32, device=dev)
+    stab = torch.zeros(n_total, dtype=torch.int32, device=dev)
+    cls = torch.full((n_class, 2), 7.0, dtype=torch.float32, device=dev)
+    glob = torch.zeros((n_class, 3, 4), dtype=torch.float32, device=dev)
+    rc = lib.isac_multi_align_stability_device(d_prm.data_ptr(), n_total, d_off.data_ptr(), n_class, 3,
+                                               0.0, 1e4, 1.0, 60.0, pix.data_ptr(), avg.data_ptr(),
+                                               stab.data_ptr(), cls.data_ptr(), glob.data_ptr(),
+                                               torch.cuda.current_stream().cuda_stream)
+    assert rc == 0
+    cls = cls.cpu().numpy()
+    for c in (2, 3, 4, 5, 6):   # malformed: empty
+        assert cls[c, 0] == 0.0 and cls[c, 1] == -1.0, (c, cls[c])
+    ref = binding.multi_align_stability_device([c0.reshape(3, -1).tolist(), c1.reshape(3, -1).tolist()],
+                                               0.0, 1e4, 1.0, 60.0)
+    assert cls[0, 0] == pytest.approx(ref[0][1]) and cls[0, 1] == pytest.approx(ref[0][2])
+    assert cls[1, 0] == pytest.approx(ref[1][1]) and cls[1, 1] == pytest.approx(ref[1][2])
+    np.testing.assert_array_equal(pix.cpu().numpy(), np.concatenate([ref[0][3]["pix"], ref[1][3]["pix"]]))
