@@ -187,6 +187,11 @@ Workspace ws;
 std::vector<float2> make_shift_grid(float rx, float ry, float step) {
     std::vector<float2> v;
     if (!(step > 0.0f)) return v;
+    // a grid beyond any caller's limit (more than 65536 shifts) comes back with
+    // 65537 entries, which every shape check rejects, instead of being enumerated
+    // (a step far below the range's float spacing would never end the loop)
+    const double nx = 2.0 * (double)rx / step + 2.0, ny = 2.0 * (double)ry / step + 2.0;
+    if (nx * ny > 65537.0) return std::vector<float2>(65537);
     for (float sx = -rx; sx <= rx; sx += step)
         for (float sy = -ry; sy <= ry; sy += step) v.push_back(make_float2(sx, sy));
     return v;
@@ -1222,6 +1227,13 @@ int run_pre_align_loop(const float *d_imgs, int P, int D, int R, const float *ra
                        const float *steps, int n_steps, int max_iter, int center, float cutoff,
                        float falloff, DevParam *prm, float *d_average, hipStream_t st) {
     if (P <= 0 || n_steps <= 0 || max_iter <= 0) return 0;
+    // every step's grid is checked before any work, so an invalid step leaves the
+    // caller's parameters untouched
+    std::vector<std::vector<float2>> grids(n_steps);
+    for (int n = 0; n < n_steps; ++n) {
+        grids[n] = make_shift_grid(ranges[n], ranges[n], steps[n]);
+        if (int e = check_shape(D, R, kRingLen, grids[n].size())) return e;
+    }
     HIP_OK(ws.pre_tp.reserve((size_t)P * sizeof(float4)));
     HIP_OK(ws.pre_mir.reserve((size_t)P * sizeof(int)));
     HIP_OK(ws.pre_part.reserve(isac::pre_average_scratch_floats(P, D) * sizeof(float)));
@@ -1245,8 +1257,7 @@ int run_pre_align_loop(const float *d_imgs, int P, int D, int R, const float *ra
     const int total = n_steps * max_iter;
     int it_glob = 0;
     for (int n = 0; n < n_steps; ++n) {
-        const std::vector<float2> sh = make_shift_grid(ranges[n], ranges[n], steps[n]);
-        if (int e = check_shape(D, R, kRingLen, sh.size())) return e;
+        const std::vector<float2> &sh = grids[n];
         for (int it = 0; it < max_iter; ++it) {
             ++it_glob;
             HIP_OK(isac::launch_pre_average(d_imgs, P, D, prm, ws.pre_tp.as<float4>(),

@@ -121,3 +121,26 @@ def test_reset_shifts_keeps_count(gpu):
     assert lib.gpu_last_error() == 2
     lib.gpu_clear_error()
     lib.gpu_clear()
+
+
+def test_oversized_shift_grid_is_rejected_promptly(gpu):
+    # a range of 1e6 px in steps of 1 px (or a step far below the range's float
+    # spacing) is beyond any grid the library takes: the calls return an error at
+    # once instead of enumerating the grid
+    import time
+    lib = binding.load()
+    t0 = time.perf_counter()
+    lib.reset_shifts(1.0e6, 1.0)              # no context yet: a state error
+    assert lib.gpu_last_error() != 0
+    lib.gpu_clear_error()
+    cfg = binding.make_config(4, 1, 32, 10, 1.0, 2.0, 2.0, "pre_aln")
+    lib.pre_align_init(4, cfg, 0)
+    binding.check(lib)
+    lib.reset_shifts(1.0e6, 1.0)              # 2e6 x 2e6 shifts: not the context's count
+    assert lib.gpu_last_error() == 2
+    lib.gpu_clear_error()
+    lib.reset_shifts(3.0e8, 1.0)              # a step below the float spacing of the range
+    assert lib.gpu_last_error() == 2
+    lib.gpu_clear_error()
+    lib.gpu_clear()
+    assert time.perf_counter() - t0 < 5.0
