@@ -852,10 +852,11 @@ int tune_group(const std::vector<BatchSub> &subs, BatchGroup &gr, int n_cu, bool
     return 0;
 }
 
-// Group workspace cap: 16 GiB, or ISAC_BATCH_WORKSPACE_MB when set (the group
-// budget is also bounded by the free device memory).
+// Group workspace cap: 64 GiB (a config-4 batch of 182 053 particles in 6 groups,
+// one persistent launch each), or ISAC_BATCH_WORKSPACE_MB when set; the group
+// budget is also bounded by the free device memory.
 size_t batch_cap() {
-    size_t cap = (size_t)16 << 30;
+    size_t cap = (size_t)64 << 30;
     if (const char *env = getenv("ISAC_BATCH_WORKSPACE_MB")) {
         const long long mb = atoll(env);
         if (mb > 0) cap = (size_t)mb << 20;
