@@ -288,7 +288,7 @@ size_t free_device_bytes() {
 int batch_for(size_t per_item, size_t fixed_bytes, float mem_frac, int total) {
     double frac = (mem_frac > 0.0f && mem_frac <= 1.0f) ? mem_frac : 0.9;
     size_t have = (size_t)(frac * (double)(free_device_bytes() + ws.spec.cap + ws.rec.cap + ws.norm.cap));
-    const size_t cap = (size_t)16 << 30;  // beyond 16 GiB a larger batch buys nothing
+    const size_t cap = (size_t)48 << 30;  // 20 000 particles of 76 px in one batch
     size_t budget = std::min(have > fixed_bytes ? have - fixed_bytes : 0, cap);
     size_t b = per_item ? budget / per_item : (size_t)total;
     b = std::max<size_t>(b, 1);
