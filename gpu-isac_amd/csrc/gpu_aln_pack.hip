@@ -334,10 +334,11 @@ int ref_spectra(const float *d_ref, int N, int D, int R, hipStream_t st) {
 // Device-resident multi-reference pipeline.
 // before_batch(p0, n), when given, runs on the host before a particle batch's
 // launches are queued (the host-pointer entry copies that batch's images there).
+// max_batch > 0 caps the particles per batch below what memory allows.
 int run_mref(const float *d_sbj, const float *d_ref, ShiftView acc, int P, int N, int D, int R,
              const std::vector<float2> &shifts, float *d_peak, int peak_row, float *d_d,
              int d_pitch, float mem_frac, hipStream_t st,
-             const std::function<int(int, int)> &before_batch = {}) {
+             const std::function<int(int, int)> &before_batch = {}, int max_batch = 0) {
     const int S = (int)shifts.size();
     if (int e = check_shape(D, R, kRingLen, shifts.size())) return e;
     if (P <= 0 || N <= 0) return 0;
@@ -346,7 +347,8 @@ int run_mref(const float *d_sbj, const float *d_ref, ShiftView acc, int P, int N
     if (int e = ref_spectra(d_ref, N, D, R, st)) return e;
     const size_t per = (size_t)S * R * kSlots * sizeof(float2) + (size_t)S * sizeof(float2) +
                        (size_t)N * sizeof(PeakRec);
-    const int pb = batch_for(per, 0, mem_frac, P);
+    int pb = batch_for(per, 0, mem_frac, P);
+    if (max_batch > 0) pb = std::min(pb, max_batch);
     HIP_OK(ws.spec.reserve((size_t)pb * S * R * kSlots * sizeof(float2)));
     HIP_OK(ws.norm.reserve((size_t)pb * S * sizeof(float2)));
     HIP_OK(ws.rec.reserve((size_t)pb * N * sizeof(PeakRec)));
@@ -1539,7 +1541,9 @@ void multi_ref_align(const float **sbj_data, const float **ref_data, float *isac
     // particle images: each batch is gathered into pinned memory and copied on a
     // second stream just before its launches are queued, and the search stream
     // waits for that copy only; so the host gathers batch b + 1, and the copy
-    // engine moves it, while the device searches batch b
+    // engine moves it, while the device searches batch b. Batches of at most a
+    // quarter of the particles keep that overlap when memory would allow one
+    // batch: only the first quarter's gather and copy are exposed.
     const size_t img_bytes = (size_t)D * D * sizeof(float);
     if (reserve_stage((size_t)P * img_bytes)) return;
     if (g.sbj_img.reserve((size_t)P * img_bytes) != hipSuccess ||
@@ -1576,7 +1580,7 @@ void multi_ref_align(const float **sbj_data, const float **ref_data, float *isac
     ShiftView acc{&g.prm.as<DevParam>()[0].shift_x, (int)(sizeof(DevParam) / sizeof(float))};
     if (run_mref(g.sbj_img.as<float>(), g.ref_img.as<float>(), acc, P, N, D, R, g.shifts,
                  ws.peak.as<float>(), 4 * P, ws.dvals.as<float>(), P, cuda_mem_request, st,
-                 stage_batch))
+                 stage_batch, (P + 3) / 4))
         return;
     if (hipMemcpy(isac_peak_list, ws.peak.p, (size_t)N * 4 * P * sizeof(float),
                   hipMemcpyDeviceToHost) != hipSuccess ||
