@@ -63,7 +63,8 @@ def test_device_entry_matches_abi(gpu):
 def test_abi_in_many_particle_batches(gpu):
     """A tiny memory request forces one-particle batches: each batch's images
     are gathered and copied on the copy stream just before its launches; the
-    results equal the one-batch call bit for bit."""
+    results equal the default call's (batches of a quarter of the particles)
+    bit for bit."""
     ds = synth.dataset_p1k_n8_d64(n_particles=37)
     acc = np.random.default_rng(5).uniform(-1, 1, (37, 2)).astype(np.float32)
     peak1, d1 = binding.multi_ref_align(ds.sbj, ds.ref, 29, 1.0, 2.0, 2.0, acc_xy=acc)
