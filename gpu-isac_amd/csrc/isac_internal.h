@@ -60,13 +60,19 @@ struct ShiftView {
 };
 
 // One multi-reference job (a particle batch of one subset) for the FP32 kernel.
+// The candidate form (cand set) scores K listed references per particle instead
+// of all N: list i holds cand[i*K .. i*K+K-1] (negative: an empty slot), takes
+// its spectra from row pidx[i] (or i when pidx is null), and fills rec[i*K + k].
 struct MrefJob {
     const float2 *spec;     // [n_img][S][R][kSlots] raw particle spectra
     const float2 *norm;     // [n_img][S]
     const float2 *rspec;    // [N][R][kSlots] normalised, ring-weighted reference spectra
     const float *rsum0;     // [N]
-    PeakRec *rec;           // [n_img][N]
+    PeakRec *rec;           // [n_img][N], or [lists][K] in the candidate form
     int n_img, S, R, N, G, GB;   // G reference groups of 4, GB workgroups per particle
+    const int *cand = nullptr;   // candidate form: [lists][K] reference ids
+    const int *pidx = nullptr;   // candidate form: spectra row of each list, or null
+    int K = 0;
 };
 
 MrefJob mref_job(const float2 *spec, const float2 *norm, const float2 *rspec, const float *rsum0,

@@ -775,104 +775,6 @@ __global__ __launch_bounds__(256) void k_ccf_items(
     write_rec(rec + item, bv, bf, nbuf[wave], lane);
 }
 
-// Candidate items four at a time: wavefront = (particle, 4 consecutive candidate
-// slots), walking the shifts four at a time (4 rows x 4 references, the register
-// tile of k_ccf_mref), so each spectrum row load serves four references.  Per
-// (row, reference) pair the contraction order and the psi stage are those of
-// k_ccf_items<false>, and each reference keeps its own running maximum, so every
-// record is bit-identical to it.  Slots with a negative reference get the empty
-// record.  K is a multiple of 4.
-__global__ __launch_bounds__(256) void k_ccf_cand4(
-    const float2 *__restrict__ spec, const float2 *__restrict__ norm,
-    const float2 *__restrict__ rspec, const float *__restrict__ rsum0, const int *__restrict__ cand,
-    int K, int n_img, int S, int R, int N, const float2 *__restrict__ twg, PeakRec *__restrict__ rec,
-    const int *__restrict__ pidx) {
-    __shared__ float2 tw[256];
-    __shared__ float2 fbuf[4][512];
-    __shared__ float zbuf[4][256];
-    __shared__ float nbuf[4][4][2][8];
-    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    fill_fft_twiddles(tw, twg, tid, 256);
-    __syncthreads();
-    const int G = K / 4;
-    const int grp = blockIdx.x * 4 + wave;
-    if (grp >= n_img * G) return;
-    const int pi = grp / G, g = grp - pi * G;
-    const int p = pidx ? pidx[pi] : pi;   // spectra row of the particle
-    const int *cs = cand + (size_t)pi * K + 4 * g;
-    bool live[4];
-    const float2 *rb[4];
-    float rs0[4], bv[4][2];
-    int bf[4][2];
-    bool any = false;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int raw = cs[j];
-        live[j] = raw >= 0;
-        any = any || live[j];
-        // the host validates candidate ids; the clamp keeps every read in bounds
-        const int n = min(max(raw, 0), N - 1);
-        rb[j] = rspec + (size_t)n * R * kSlots;
-        rs0[j] = rsum0[n];
-        bv[j][0] = bv[j][1] = -3.40282e+38f;
-        bf[j][0] = bf[j][1] = INT_MAX;
-        if (!live[j] && lane < 8) nbuf[wave][j][0][lane] = 0.0f;
-    }
-    const size_t img_stride = (size_t)S * R * kSlots;
-    for (int s0 = 0; any && s0 < S; s0 += 4) {
-        float acc[4][4][2][4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-#pragma unroll
-                for (int h = 0; h < 2; ++h)
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) acc[r][j][h][e] = 0.0f;
-        const float2 *arow[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-            arow[r] = spec + (size_t)p * img_stride + (size_t)min(s0 + r, S - 1) * R * kSlots;
-        for (int i = 0; i < R; ++i) {
-            float2 a0[4], a1[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                load_slots(arow[r] + i * kSlots, lane, a0[r], a1[r]);
-            }
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                float2 b0v, b1v;
-                load_slots(rb[j] + i * kSlots, lane, b0v, b1v);
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    acc[r][j][0][0] += b0v.x * a0[r].x;
-                    acc[r][j][0][1] += b0v.y * a0[r].y;
-                    acc[r][j][0][2] += b0v.x * a0[r].y;
-                    acc[r][j][0][3] += b0v.y * a0[r].x;
-                    acc[r][j][1][0] += b1v.x * a1[r].x;
-                    acc[r][j][1][1] += b1v.y * a1[r].y;
-                    acc[r][j][1][2] += b1v.x * a1[r].y;
-                    acc[r][j][1][3] += b1v.y * a1[r].x;
-                }
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int s = s0 + r;
-            if (s >= S) break;
-            const float2 nm = norm[(size_t)p * S + s];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                if (!live[j]) continue;   // uniform over the wavefront
-                psi_update<false>(acc[r][j][0], acc[r][j][1], nm, rs0[j], s, lane, fbuf[wave],
-                                  fbuf[wave] + 256, tw, zbuf[wave], nbuf[wave][j], bv[j], bf[j]);
-            }
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) write_rec(rec + (size_t)pi * K + 4 * g + j, bv[j], bf[j], nbuf[wave][j], lane);
-}
-
 // Top-K references of every particle by approximate peak value (descending; ties
 // to the lower reference index).  One thread per particle.
 template <int K>
@@ -1037,22 +939,28 @@ __global__ void k_resolve_ties(const int *__restrict__ cand, const double *__res
 // through psi_update.
 constexpr int kRingChunk = 4;
 
-// RW shift rows per group x RB references per wavefront (RW RB = 16: 128 accumulators).
-template <int NW, int RW = 4, int RB = 4>
+// RW shift rows per group x RB references per wavefront (RW RB = 16: 128 accumulators),
+// rows staged CH rings at a time.
+template <int NW, int RW = 4, int RB = 4, int CH = kRingChunk>
 struct MrefLds {
     float2 tw[256];
-    float4 abuf4[2][RW * kRingChunk * kSlots / 2];   // [buf][ring][row][slot pairs]
+    float4 abuf4[2][RW * CH * kSlots / 2];   // [buf][ring][row][slot pairs]
     float2 fbuf[NW][512];
     float zbuf[NW][256];
     float nbuf[NW][RB][2][8];
 };
 
 // Work of one workgroup: particle p of job j, reference groups gb*NW .. gb*NW+NW-1
-// (the twiddles must already be in sh.tw).
-template <int NW, int RW = 4, int RB = 4>
-__device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb, MrefLds<NW, RW, RB> &sh) {
+// (the twiddles must already be in sh.tw), rows staged CH rings at a time.  CAND:
+// the candidate form of MrefJob, p is the list index; every slot of an active
+// group gets a record (an empty one for a negative id), and each listed
+// reference's record is the one the exhaustive form computes for it.
+template <int NW, int RW = 4, int RB = 4, int CH = kRingChunk, bool CAND = false>
+__device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb,
+                                           MrefLds<NW, RW, RB, CH> &sh) {
     static_assert(RW * RB == 16, "128 accumulators per lane");
-    constexpr int CH4 = RW * kRingChunk * kSlots / 2;   // float4 per staged chunk
+    static_assert(CH % 2 == 0, "rings are taken in pairs within a chunk");
+    constexpr int CH4 = RW * CH * kSlots / 2;   // float4 per staged chunk
     const int S = j.S, R = j.R, N = j.N;
     const int tid = threadIdx.x, lane = tid & 63;
     // uniform over the wavefront: the reference indices and spectrum addresses
@@ -1060,13 +968,22 @@ __device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb, Mref
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int g = gb * NW + wave;
     const bool active = g < j.G;
+    const int ps = (CAND && j.pidx) ? j.pidx[p] : p;   // spectra row
     int nref[RB];
     bool nval[RB];
 #pragma unroll
     for (int c = 0; c < RB; ++c) {
-        const int n = g * RB + c;
-        nval[c] = active && n < N;
-        nref[c] = min(n, N - 1);
+        if (CAND) {
+            // the host validates candidate ids; the clamp keeps every read in bounds
+            const int raw = active ? j.cand[(size_t)p * j.K + g * RB + c] : -1;
+            nval[c] = raw >= 0;
+            nref[c] = min(max(raw, 0), N - 1);
+            if (active && !nval[c] && lane < 8) sh.nbuf[wave][c][0][lane] = 0.0f;
+        } else {
+            const int n = g * RB + c;
+            nval[c] = active && n < N;
+            nref[c] = min(n, N - 1);
+        }
     }
     float bv[RB][2];
     int bf[RB][2];
@@ -1078,22 +995,22 @@ __device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb, Mref
         bf[c][0] = bf[c][1] = INT_MAX;
     }
     const size_t img_stride = (size_t)S * R * kSlots;
-    const int nchunk = (R + kRingChunk - 1) / kRingChunk;
+    const int nchunk = (R + CH - 1) / CH;
     const float2 *abuf = reinterpret_cast<const float2 *>(&sh.abuf4[0][0]);
 
     for (int s0 = 0; s0 < S; s0 += RW) {
         const float4 *rows4[RW];
 #pragma unroll
         for (int r = 0; r < RW; ++r)
-            rows4[r] = reinterpret_cast<const float4 *>(j.spec + (size_t)p * img_stride +
+            rows4[r] = reinterpret_cast<const float4 *>(j.spec + (size_t)ps * img_stride +
                                                         (size_t)min(s0 + r, S - 1) * R * kSlots);
-        // Stage one chunk (4 rows x kRingChunk rings, 1 KB per row-ring) with direct
+        // Stage one chunk (4 rows x CH rings, 1 KB per row-ring) with direct
         // global->LDS loads: one wavefront instruction moves one row-ring, each lane
         // 16 bytes to LDS base + 16*lane.
         auto load_chunk = [&](int ch, int buf) {
-            for (int rr = wave; rr < RW * kRingChunk; rr += NW) {
+            for (int rr = wave; rr < RW * CH; rr += NW) {
                 const int ii = rr / RW, r = rr % RW;
-                const int ring = ch * kRingChunk + ii;
+                const int ring = ch * CH + ii;
                 if (ring >= R) continue;
                 __builtin_amdgcn_global_load_lds(
                     (const void *)(rows4[r] + ring * (kSlots / 2) + lane),
@@ -1148,8 +1065,8 @@ __device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb, Mref
         __syncthreads();
         for (int ch = 0; ch < nchunk; ++ch) {
             const int buf = ch & 1;
-            const int i0 = ch * kRingChunk, i1 = min(R, i0 + kRingChunk);
-            // kRingChunk is even, so only the last chunk can end on an unpaired ring
+            const int i0 = ch * CH, i1 = min(R, i0 + CH);
+            // CH is even, so only the last chunk can end on an unpaired ring
             for (int i = i0; i < i1; i += 2) {
                 const bool last_pair = i + 2 >= i1;
                 ring(i - i0, buf, bA, [&] {
@@ -1170,7 +1087,7 @@ __device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb, Mref
         for (int r = 0; r < RW; ++r) {
             const int s = s0 + r;
             if (s >= S) break;
-            const float2 nm = j.norm[(size_t)p * S + s];
+            const float2 nm = j.norm[(size_t)ps * S + s];
 #pragma unroll
             for (int c = 0; c < RB; ++c) {
                 if (!nval[c]) continue;
@@ -1181,9 +1098,14 @@ __device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb, Mref
         }
     }
 #pragma unroll
-    for (int c = 0; c < RB; ++c)
-        if (nval[c])
+    for (int c = 0; c < RB; ++c) {
+        if (CAND) {
+            if (active)
+                write_rec(j.rec + (size_t)p * j.K + g * RB + c, bv[c], bf[c], sh.nbuf[wave][c], lane);
+        } else if (nval[c]) {
             write_rec(j.rec + (size_t)p * N + nref[c], bv[c], bf[c], sh.nbuf[wave][c], lane);
+        }
+    }
 }
 
 // Multi-reference: one workgroup = one particle x NW groups of 4 references (one
@@ -1202,6 +1124,21 @@ __global__ __launch_bounds__(NW * 64, 2) void k_ccf_mref(MrefJob j, const float2
     const int p = lb / j.GB;
     if (p >= j.n_img) return;  // uniform over the workgroup
     mref_block<NW, RW, RB>(j, p, lb % j.GB, sh);
+}
+
+// Candidate re-scoring (the candidate form of MrefJob) by the exhaustive kernel's
+// workgroup code, so every record equals k_ccf_mref's for that (particle,
+// reference): G = K / 4 groups of four listed references, NW per workgroup, GB
+// workgroups per list.  Rows are staged two rings at a time, so that as many
+// workgroups fit the LDS as the registers allow (two wavefronts per SIMD).
+template <int NW>
+__global__ __launch_bounds__(NW * 64, 2) void k_ccf_cand_mref(MrefJob j, const float2 *__restrict__ twg) {
+    __shared__ MrefLds<NW, 4, 4, 2> sh;
+    fill_fft_twiddles(sh.tw, twg, threadIdx.x, NW * 64);
+    __syncthreads();
+    const int p = blockIdx.x / j.GB;
+    if (p >= j.n_img) return;  // uniform over the workgroup
+    mref_block<NW, 4, 4, 2, true>(j, p, blockIdx.x - p * j.GB, sh);
 }
 
 // Persistent form over many jobs (batched subsets): work items (job, particle,
@@ -2252,9 +2189,20 @@ hipError_t launch_ccf_candidates(const float2 *spec, const float2 *norm, const f
     const int items = n_img * K;
     if (items <= 0) return hipSuccess;
     if (K % 4 == 0) {
-        const int groups = n_img * (K / 4);
-        hipLaunchKernelGGL(k_ccf_cand4, dim3((groups + 3) / 4), dim3(256), 0, st, spec, norm, rspec,
-                           rsum0, cand, K, n_img, n_shift, n_ring, n_ref, twiddle, rec, pidx);
+        MrefJob j = mref_job(spec, norm, rspec, rsum0, rec, n_img, n_shift, n_ring, n_ref, 1);
+        j.cand = cand;
+        j.pidx = pidx;
+        j.K = K;
+        j.G = K / 4;
+        const int nw = j.G == 1 ? 1 : j.G == 2 ? 2 : 4;
+        j.GB = (j.G + nw - 1) / nw;
+        const dim3 grid((unsigned)((long long)n_img * j.GB));
+        if (nw == 1)
+            hipLaunchKernelGGL(k_ccf_cand_mref<1>, grid, dim3(64), 0, st, j, twiddle);
+        else if (nw == 2)
+            hipLaunchKernelGGL(k_ccf_cand_mref<2>, grid, dim3(128), 0, st, j, twiddle);
+        else
+            hipLaunchKernelGGL(k_ccf_cand_mref<4>, grid, dim3(256), 0, st, j, twiddle);
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_ccf_items<false>, dim3((items + 3) / 4), dim3(256), 0, st, spec, norm,

@@ -30,7 +30,7 @@
 //    samples; a wave-wide reduction per particle decodes the winner.
 // The values are approximate (FP16/BF16 operands, FP16 psi transform, and the
 // index bits perturb a value by at most 2^-(23 - index bits) relative); the
-// best candidates are re-scored exactly in FP32 (k_ccf_cand4, kernels.hip) and
+// best candidates are re-scored exactly in FP32 (k_ccf_cand_mref, kernels.hip) and
 // near-ties in FP64 (kernels.hip).  The approximate records carry the peak
 // sample only (no sub-sample refinement).
 //
