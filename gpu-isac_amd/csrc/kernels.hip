@@ -985,6 +985,20 @@ __device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb,
             nref[c] = min(n, N - 1);
         }
     }
+    // candidate form: a wavefront whose four slots are all empty only helps stage
+    // the rows, and a workgroup whose slots are all empty skips the rows
+    bool wlive = true, wg_live = true;
+    if (CAND) {
+        wlive = false;
+#pragma unroll
+        for (int c = 0; c < RB; ++c) wlive = wlive || nval[c];
+        wg_live = false;
+        for (int w = 0; w < NW; ++w) {
+            const int gw = gb * NW + w;
+            if (gw >= j.G) break;
+            for (int c = 0; c < RB; ++c) wg_live = wg_live || j.cand[(size_t)p * j.K + gw * RB + c] >= 0;
+        }
+    }
     float bv[RB][2];
     int bf[RB][2];
     float rs0[RB];
@@ -998,7 +1012,7 @@ __device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb,
     const int nchunk = (R + CH - 1) / CH;
     const float2 *abuf = reinterpret_cast<const float2 *>(&sh.abuf4[0][0]);
 
-    for (int s0 = 0; s0 < S; s0 += RW) {
+    for (int s0 = 0; wg_live && s0 < S; s0 += RW) {
         const float4 *rows4[RW];
 #pragma unroll
         for (int r = 0; r < RW; ++r)
@@ -1033,11 +1047,16 @@ __device__ __forceinline__ void mref_block(const MrefJob &j, int p, int gb,
         // LDS reads of the ring in flight, so the waits the compiler places
         // before the next LDS reads find them a ring of FMAs old.
         auto load_refs = [&](int i, float2 (&b)[RB][2]) {
+            if (CAND && !wlive) return;
 #pragma unroll
             for (int c = 0; c < RB; ++c)
                 load_slots(j.rspec + ((size_t)nref[c] * R + i) * kSlots, lane, b[c][0], b[c][1]);
         };
         auto ring = [&](int ii, int buf, const float2 (&b)[RB][2], auto &&issue) {
+            if (CAND && !wlive) {   // uniform over the wavefront: its share of the staging only
+                issue();
+                return;
+            }
             float2 a[RW][2];
 #pragma unroll
             for (int r = 0; r < RW; ++r) {

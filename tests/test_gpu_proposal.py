@@ -161,16 +161,19 @@ def test_inputs_written_by_queued_work_are_seen(gpu):
 
 
 
-def test_level2_is_the_exhaustive_search_over_proposed_refs(gpu):
+@pytest.mark.parametrize("K", [3, 8])
+def test_level2_is_the_exhaustive_search_over_proposed_refs(gpu, K):
     """Particles whose windowed maximum is on an edge get every shift and psi of
     their proposed references: bit-identical to the exhaustive search's best
-    over those references; the others keep the windowed maximum."""
+    over those references; the others keep the windowed maximum.  K = 8 lists
+    the distinct references of 8 proposals among 6 references, so some lists
+    have empty slots and some an empty second group of four."""
     import torch
     P, N = 24, 6
     ds = synth.make_dataset(P, N, 76, 34.0, 0.1, 3.0, True, seed_refs=4)
     rng = np.random.default_rng(12)
-    prop = np.stack([rng.integers(0, N, (P, 3)), rng.integers(0, 2, (P, 3)),
-                     rng.integers(0, 72, (P, 3)), rng.integers(0, 49, (P, 3))], 2).astype(np.int32)
+    prop = np.stack([rng.integers(0, N, (P, K)), rng.integers(0, 2, (P, K)),
+                     rng.integers(0, 72, (P, K)), rng.integers(0, 49, (P, K))], 2).astype(np.int32)
     acc = np.zeros((P, 2), np.float32)
     s, r = torch.from_numpy(ds.sbj).cuda(), torch.from_numpy(ds.ref).cuda()
     d, peak = _exhaustive(s, r)
