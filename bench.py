@@ -646,7 +646,7 @@ def main():
 def compact_reports(log):
     """Precision (1 000 x 8 refs, 64 px), uneven-subset batch (64 subsets) and
     learned proposal (100 000 held-out particles, 100 refs, 76 px) reports,
-    reduced to the figures each is judged by."""
+    reduced to each report's headline figures."""
     import torch
     out = {}
     t0 = time.perf_counter()
