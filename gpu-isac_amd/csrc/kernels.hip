@@ -2207,21 +2207,18 @@ hipError_t launch_ccf_candidates(const float2 *spec, const float2 *norm, const f
                                  PeakRec *rec, hipStream_t st, const int *pidx) {
     const int items = n_img * K;
     if (items <= 0) return hipSuccess;
-    if (K % 4 == 0) {
+    if (K == 4 || K == 8) {   // one workgroup per list, one wavefront per 4 slots
         MrefJob j = mref_job(spec, norm, rspec, rsum0, rec, n_img, n_shift, n_ring, n_ref, 1);
         j.cand = cand;
         j.pidx = pidx;
         j.K = K;
         j.G = K / 4;
-        const int nw = j.G == 1 ? 1 : j.G == 2 ? 2 : 4;
-        j.GB = (j.G + nw - 1) / nw;
-        const dim3 grid((unsigned)((long long)n_img * j.GB));
-        if (nw == 1)
+        j.GB = 1;
+        const dim3 grid((unsigned)n_img);
+        if (K == 4)
             hipLaunchKernelGGL(k_ccf_cand_mref<1>, grid, dim3(64), 0, st, j, twiddle);
-        else if (nw == 2)
-            hipLaunchKernelGGL(k_ccf_cand_mref<2>, grid, dim3(128), 0, st, j, twiddle);
         else
-            hipLaunchKernelGGL(k_ccf_cand_mref<4>, grid, dim3(256), 0, st, j, twiddle);
+            hipLaunchKernelGGL(k_ccf_cand_mref<2>, grid, dim3(128), 0, st, j, twiddle);
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_ccf_items<false>, dim3((items + 3) / 4), dim3(256), 0, st, spec, norm,

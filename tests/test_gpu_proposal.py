@@ -161,13 +161,15 @@ def test_inputs_written_by_queued_work_are_seen(gpu):
 
 
 
-@pytest.mark.parametrize("K", [3, 8])
+@pytest.mark.parametrize("K", [3, 4, 8])
 def test_level2_is_the_exhaustive_search_over_proposed_refs(gpu, K):
     """Particles whose windowed maximum is on an edge get every shift and psi of
     their proposed references: bit-identical to the exhaustive search's best
-    over those references; the others keep the windowed maximum.  K = 8 lists
-    the distinct references of 8 proposals among 6 references, so some lists
-    have empty slots and some an empty second group of four."""
+    over those references; the others keep the windowed maximum.  K = 4 and 8
+    run the candidate form of the exhaustive kernel (one and two wavefronts per
+    list), K = 3 the per-item kernel.  K = 8 lists the distinct references of 8
+    proposals among 6 references, so some lists have empty slots and some an
+    empty second group of four."""
     import torch
     P, N = 24, 6
     ds = synth.make_dataset(P, N, 76, 34.0, 0.1, 3.0, True, seed_refs=4)
