@@ -106,3 +106,53 @@ def test_mixed_rows_past_the_copy_border(gpu, precision, max_rel):
     assert (rel * scale).max() <= max_rel / 2
     assert rel[strong].max() <= max_rel
     assert st["agree_separated"] >= 0.995
+
+
+@pytest.mark.parametrize("precision,max_rel", [("fp16", 2e-3), ("bf16", 2e-2)])
+@pytest.mark.parametrize("D,R,nref,rng", [(96, 44, 7, 2.0), (128, 60, 9, 3.0)])
+def test_mixed_beyond_40_rings(gpu, precision, max_rel, D, R, nref, rng):
+    """41 to 48 rings: k_mref_rt on operands built from the FP32 spectra
+    (k_frag_rows); more than 48: k_mref_gemm.  Same criteria as the other shapes,
+    except that with 7 or 9 references fewer pairs clear the BF16 gap (43 % of
+    them by the FP64 oracle), so the share of separated pairs asked for is lower."""
+    import torch
+    P = 32
+    ds = synth.make_dataset(P, nref, D, 0.42 * D, 0.2, rng, True, seed_refs=D + 1)
+    acc = np.random.default_rng(D).uniform(-1.5, 1.5, (P, 2)).astype(np.float32)
+    s = torch.from_numpy(ds.sbj).cuda()
+    r = torch.from_numpy(ds.ref).cuda()
+    a = torch.from_numpy(acc).cuda()
+    peak, d = binding.mref_align_mixed_device(s, r, R, 1.0, rng, rng, precision, acc_xy=a)
+    torch.cuda.synchronize()
+    ref = coracle.mref(ds.sbj, ds.ref, acc, R, rng, rng, 1.0)
+    st = _stats(peak.cpu().numpy(), d.cpu().numpy(), ref, precision)
+    print(precision, f"D{D} R{R} N{nref}", st)
+    assert st["d_rel_max"] <= max_rel
+    assert st["separated"] >= {"fp16": 0.6, "bf16": 0.35}[precision] * st["pairs"]
+    assert st["agree_separated"] >= 0.995
+
+
+@pytest.mark.parametrize("D,R,nref,rng", [(96, 44, 7, 2.0), (128, 60, 9, 3.0)])
+def test_best_exact_beyond_40_rings(gpu, D, R, nref, rng):
+    """The FP16 form with the exact FP32 best per particle past 40 rings: the
+    chosen reference carries exactly the FP32 search's value and parameters, and
+    is the FP32 search's own best reference except at FP64-settled near-ties."""
+    import torch
+    P = 32
+    ds = synth.make_dataset(P, nref, D, 0.42 * D, 0.2, rng, True, seed_refs=D + 1)
+    acc = np.random.default_rng(D).uniform(-1.5, 1.5, (P, 2)).astype(np.float32)
+    s = torch.from_numpy(ds.sbj).cuda()
+    r = torch.from_numpy(ds.ref).cuda()
+    a = torch.from_numpy(acc).cuda()
+    out = binding.mref_best_mixed_device(s, r, R, 1.0, rng, rng, "fp16", acc_xy=a)
+    pk32, d32 = binding.mref_align_device(s, r, R, 1.0, rng, rng, acc_xy=a)
+    torch.cuda.synchronize()
+    o = {k: v.cpu().numpy() for k, v in out.items()}
+    pk32, d32 = pk32.cpu().numpy(), d32.cpu().numpy()
+    br, cols = o["best_ref"], np.arange(P)
+    assert np.array_equal(o["best_val"], d32[br, cols])
+    for k in range(4):
+        assert np.array_equal(o["best_peak"][:, k], pk32[br, 4 * cols + k])
+    agree = float((np.argmax(d32, axis=0) == br).mean())
+    print(f"D{D} R{R}", {"agree_vs_fp32": agree, "flagged_ties": int(o["tie"].sum())})
+    assert agree >= 0.96
