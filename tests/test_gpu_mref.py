@@ -72,3 +72,35 @@ def test_abi_in_many_particle_batches(gpu):
                                         mem_request=1e-9)
     assert np.array_equal(peak1, peak2)
     assert np.array_equal(d1, d2)
+
+
+def test_profile_counts_the_search_launches(gpu):
+    """isac_profile_enable / isac_profile_read, the HIP-event timing behind the
+    benchmark's roofline: every particle batch's search launch and polar launch
+    is counted and timed while profiling is on; enabling again clears the counts."""
+    import ctypes
+    import torch
+    lib = binding.load()
+    ds = synth.dataset_p1k_n8_d64(n_particles=40)
+    s = torch.from_numpy(ds.sbj).cuda()
+    r = torch.from_numpy(ds.ref).cuda()
+
+    def read():
+        n, ms = ctypes.c_int(-1), ctypes.c_double(-1.0)
+        npol, mspol = ctypes.c_int(-1), ctypes.c_double(-1.0)
+        lib.isac_profile_read(ctypes.byref(n), ctypes.byref(ms), ctypes.byref(npol), ctypes.byref(mspol))
+        return n.value, ms.value, npol.value, mspol.value
+
+    lib.isac_profile_enable(1)
+    for _ in range(3):   # 40 particles: one batch per call
+        binding.mref_align_device(s, r, 29, 1.0, 2.0, 2.0)
+    torch.cuda.synchronize()
+    n, ms, npol, mspol = read()
+    assert (n, npol) == (3, 3) and ms > 0.0 and mspol > 0.0
+    lib.isac_profile_enable(1)
+    assert read() == (0, 0.0, 0, 0.0)
+    lib.isac_profile_enable(0)
+    binding.mref_align_device(s, r, 29, 1.0, 2.0, 2.0)
+    torch.cuda.synchronize()
+    assert read() == (0, 0.0, 0, 0.0)
+    binding.check(lib)
