@@ -208,3 +208,29 @@ def test_level2_is_the_exhaustive_search_over_proposed_refs(gpu, K):
     bref, bval, _ = binding.best_of_exhaustive(torch.from_numpy(d).cuda(),
                                                torch.from_numpy(pk.reshape(N, 4 * P)).cuda())
     assert torch.equal(o2["best_ref"], bref) and torch.equal(o2["best_val"], bval)
+
+
+def test_levels_across_particle_batches(gpu):
+    """100 000 particles x 100 references at 76 px: levels 1 and 2 run in several
+    particle batches and level 3 gathers the fallback particles into batches of
+    its own.  With every particle sent to level 2 and then to the FP32 fallback
+    (threshold above every value), the result is the exhaustive search's, bit for
+    bit, for every particle."""
+    import torch
+    from bench import WORKLOADS
+    _, N, D, radius, snr, max_shift, int_shifts, _, _ = WORKLOADS["p20k_n100_d76"]
+    P = 100000
+    sbj, ref, _ = synth.make_dataset_torch(P, N, D, radius, snr, max_shift, int_shifts,
+                                           torch.device("cuda:0"))
+    d, peak = _exhaustive(sbj, ref)
+    bref, bval, bpeak = binding.best_of_exhaustive(d, peak)
+    g = torch.Generator(device="cpu").manual_seed(3)
+    prop = torch.stack([torch.randint(0, N, (P, 8), generator=g), torch.randint(0, 2, (P, 8), generator=g),
+                        torch.randint(0, 72, (P, 8), generator=g), torch.randint(0, 49, (P, 8), generator=g)],
+                       2).to(torch.int32).cuda()
+    o = binding.mref_proposal_device(sbj, ref, R, 1.0, RNG, RNG, prop, threshold=3.0e38, level2=True)
+    torch.cuda.synchronize()
+    assert o["n_level2"] == P and o["n_fallback"] == P
+    assert torch.equal(o["best_ref"], bref)
+    assert torch.equal(o["best_val"], bval)
+    assert torch.equal(o["best_peak"], bpeak)
