@@ -47,6 +47,21 @@ def test_pre_align_one_run(gpu, batch):
     assert_params_match(g, o, f"pre-align batch {batch}")
 
 
+@pytest.mark.parametrize("D,R,rng,step", [(76, 35, 3.0, 1.0), (96, 44, 3.0, 1.0), (128, 60, 4.0, 1.0)])
+def test_pre_align_shapes(gpu, D, R, rng, step):
+    """Single-reference alignment at the larger boxes, ring counts and shift
+    ranges (81 shifts at 128 px).  Half-pixel grids are covered by the tie
+    tests: on them most pairs here are oracle ties (neighbouring shifts reach
+    the same maximum within 1e-5), which this comparison leaves out."""
+    P = 24
+    ds = synth.make_dataset(P, 1, D, 0.42 * D, 0.5, rng, True, seed_refs=D + 3)
+    acc = np.random.default_rng(D + 4).uniform(-1, 1, (P, 2)).astype(np.float32)
+    g = binding.pre_align(ds.sbj, ds.ref[0], R, rng, step, batch=P, acc_xy=acc)
+    shifts = O.shift_grid(rng, rng, step)
+    o = single_ref_oracle(ds.sbj, ds.ref, np.zeros(P, int), acc.astype(np.float64), R, shifts, D)
+    assert_params_match(g, o, f"pre-align D{D} R{R} rng {rng} step {step}")
+
+
 def test_pre_align_two_runs_accumulate(gpu):
     D, R, rng = 64, 29, 2.0
     ds = synth.make_dataset(24, 1, D, 28.0, 1.0, 2, True, seed_refs=6)
