@@ -99,6 +99,22 @@ def test_ref_free_iterations(gpu):
     assert_params_match(g, o, "ref-free x2")
 
 
+@pytest.mark.parametrize("D,R,rng", [(96, 44, 2.0), (128, 60, 2.0)])
+def test_ref_free_larger_boxes(gpu, D, R, rng):
+    """Reference-free iterations (alignment, then transform and class averages on
+    the device) at 96 and 128 px."""
+    sbj, cid, avg = _classes(2, 8, D, D + 7, 2.0)
+    g = binding.ref_free_align(sbj, cid, avg, R, rng, num_iter=2)
+    shifts = O.shift_grid(rng, rng, 1.0)
+    acc = np.zeros((sbj.shape[0], 2))
+    refs = avg.astype(np.float64)
+    for _ in range(2):
+        o = single_ref_oracle(sbj, refs, cid, acc, R, shifts, D)
+        prm, refs = O.ref_free_iteration(sbj, refs, cid, acc, R, shifts, D)
+        acc = np.stack([prm["shift_x"], prm["shift_y"]], 1)
+    assert_params_match(g, o, f"ref-free D{D} x2")
+
+
 def test_ref_free_with_tangent_filter(gpu):
     D, R, rng = 48, 20, 1.0
     sbj, cid, avg = _classes(2, 12, D, 31, 3.0)
