@@ -40,12 +40,13 @@ def _near_tie(tab, prm, rel=1e-5):
     return tie
 
 
-@pytest.mark.parametrize("center,cutoff,falloff", [(0, 0.0, 0.0), (-1, 0.0, 0.0), (-1, 0.3, 0.1)])
-def test_pre_align_loop_matches_oracle(gpu, center, cutoff, falloff):
+@pytest.mark.parametrize("P,D,R,rng0,center,cutoff,falloff",
+                         [(40, 48, 18, 2.0, 0, 0.0, 0.0), (40, 48, 18, 2.0, -1, 0.0, 0.0),
+                          (40, 48, 18, 2.0, -1, 0.3, 0.1), (32, 76, 35, 3.0, -1, 0.3, 0.1)])
+def test_pre_align_loop_matches_oracle(gpu, P, D, R, rng0, center, cutoff, falloff):
     import torch
-    P, D, R = 40, 48, 18
-    ranges, steps, max_iter = [2.0, 1.0], [1.0, 1.0], 3
-    sbj = _data(P, D, 40, 1.0)
+    ranges, steps, max_iter = [rng0, 1.0], [1.0, 1.0], 3
+    sbj = _data(P, D, 40 if D == 48 else D, 1.0)
     print(f"center {center} cutoff {cutoff}: {P} x {D} px, {R} rings, steps {ranges}, {max_iter} iterations",
           flush=True)
     g, avg_g = binding.pre_align_loop_device(torch.from_numpy(sbj).cuda(), R, ranges, steps,
